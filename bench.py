@@ -1,0 +1,198 @@
+"""Benchmark: KoalaBear coset LDE + Poseidon2 Merkle commit of a 2^20 x 256
+trace (log_blowup 1), the single-shard commit of Pico's RISC-V prover
+(TwoAdicFriPcs::commit at vm/src/machine/prover.rs:238-244).
+
+One step = one Pcs::commit of a trace already resident in HBM: LDE into a
+new 2^21 x 256 buffer, leaf hashing, all compress layers, root copied to the
+host, prover data released.  Prints one JSON line (see --help).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "KoalaBear LDE+Merkle commit cells/s at 2^20x256; p50/p99 job latency"
+
+# Poseidon2-W16 KoalaBear work per permutation: 616 modmul + 1416 modadd
+# (8 full rounds x 16 S-boxes x 2 mul; 20 partial rounds x (2 + 16); 9
+# external layers x 72 adds + 128 + 20 x 32).  Priced in VALU issue slots of
+# one lane: a Montgomery modmul = 3 quarter-rate 32-bit multiplies (12) + 2
+# full-rate ops; a modadd = 3 full-rate ops.
+SLOTS_PER_PERM = 616 * 14 + 1416 * 3
+# MI355X VALU issue peak, 32-bit lane ops: 256 CU x 4 SIMD x 16 lanes x
+# 2.4 GHz = 39.3 T/s (the published 157.3 TFLOP/s FP32 vector figure counts
+# an FMA as 2 flops and packed-FP32 dual issue as 2 lanes).
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+
+
+def percentile(xs, q):
+    s = sorted(xs)
+    k = (len(s) - 1) * q / 100.0
+    lo, hi = math.floor(k), math.ceil(k)
+    return s[lo] + (s[hi] - s[lo]) * (k - lo)
+
+
+def cpu_baseline(log_h, w, lb, budget_cols, leaf_rows):
+    """Oracle (C restatement, OpenMP) timed on a bounded sample of the same
+    job and scaled to the full job: LDE of `budget_cols` columns, leaf
+    hashing of `leaf_rows` rows, compress layers priced at the measured
+    permutation rate."""
+    import numpy as np
+
+    from oracle import oracle
+
+    f = oracle.KOALABEAR
+    h = 1 << log_h
+    n = h << lb
+    a = oracle.random_matrix(f, 1, h, budget_cols)
+    t0 = time.perf_counter()
+    oracle.coset_lde(f, a, lb, oracle.generator(f))
+    t_lde = (time.perf_counter() - t0) * (w / budget_cols)
+    rows = oracle.random_matrix(f, 2, leaf_rows, w)
+    t0 = time.perf_counter()
+    oracle.MerkleTree(f, [rows])
+    t_leaf = time.perf_counter() - t0
+    perms_sample = leaf_rows * ((w + 7) // 8) + (leaf_rows - 1)
+    perm_rate = perms_sample / t_leaf
+    perms_full = n * ((w + 7) // 8) + (n - 1)
+    t_total = t_lde + perms_full / perm_rate
+    del np
+    return {
+        "value": (h * w) / t_total,
+        "unit": "cells/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"oracle LDE of {budget_cols} of {w} columns (2^{log_h} rows, blowup {1 << lb}) "
+                   f"scaled x{w // budget_cols}; Merkle: leaf hashing of {leaf_rows} rows x {w}, "
+                   f"all {perms_full} permutations priced at the measured {perm_rate:.3g} perm/s"),
+        "est_job_seconds": t_total,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-h", type=int, default=20)
+    ap.add_argument("--width", type=int, default=256)
+    ap.add_argument("--log-blowup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cols", type=int, default=32)
+    ap.add_argument("--cpu-leaf-rows", type=int, default=1 << 15)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
+                          "part of it (see DESIGN.md for how the path would shard)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import pico_amd
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only; every library call is synchronous
+        sync = lambda: None  # noqa: E731
+
+    f = pico_amd.KOALABEAR
+    h, w, lb = 1 << args.log_h, args.width, args.log_blowup
+    ctx = pico_amd.Context(0)
+    trace = ctx.random_matrix(f, (2 << 32) | 0, h, w)  # resident before timing
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, lb)
+    dom = pcs.natural_domain_for_degree(h)
+
+    def step():
+        root, pd = pcs.commit([(dom, trace)])
+        pd.free()
+        return root
+
+    for _ in range(args.warmup):
+        step()
+    lat = []
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        root = step()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    cells = h * w
+
+    # per-kernel durations: HIP events on the launch stream, separate pass
+    ctx.profile(True)
+    ctx.profile_reset()
+    prof_steps = 2
+    for _ in range(prof_steps):
+        step()
+    ctx.profile(False)
+    fams = ["lde_single", "lde_pass_a", "lde_pass_bc", "lde_pass_d", "leaf_hash", "leaf_hash_w8",
+            "compress", "compress_inject"]
+    kern = {}
+    for fam in fams:
+        ms, n = ctx.profile_get(fam)
+        if n:
+            kern[fam] = {"avg_ms": ms / n, "launches_per_step": n / prof_steps,
+                         "ms_per_step": ms / prof_steps}
+    dominant = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    n_rows = h << lb
+    perms_leaf = n_rows * ((w + 7) // 8)
+    roofline = None
+    if dominant in ("leaf_hash_w8", "leaf_hash"):
+        avg_s = kern[dominant]["avg_ms"] / 1e3
+        achieved = perms_leaf * SLOTS_PER_PERM / avg_s / 1e12
+        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
+                    "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                    "kernel": dominant, "perms_per_launch": perms_leaf,
+                    "perm_rate_per_s": perms_leaf / avg_s}
+    # HBM view of the NTT passes: algorithmic bytes per launch / duration
+    by = 4 * w
+    pass_bytes = {"lde_pass_a": 2 * h * by, "lde_pass_bc": h * by + n_rows * by,
+                  "lde_pass_d": 2 * n_rows * by, "lde_single": h * by + n_rows * by}
+    ntt = {}
+    for fam, b in pass_bytes.items():
+        if fam in kern:
+            gbs = b / (kern[fam]["avg_ms"] / 1e3) / 1e9
+            ntt[fam] = {"bytes": b, "avg_ms": round(kern[fam]["avg_ms"], 4), "GB/s": round(gbs, 1),
+                        "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+
+    out = {
+        "metric": METRIC,
+        "value": cells / (ms_per_step / 1e3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"koalabear_pcs_commit_2^{args.log_h}x{w}_blowup{1 << lb}",
+                   "field": "KoalaBear", "rows": h, "cols": w, "log_blowup": lb,
+                   "hash": "Poseidon2-W16 sponge(8)/truncated-compress", "lde_rows": n_rows},
+        "p50_ms": percentile(lat, 50),
+        "p99_ms": percentile(lat, 99),
+        "roofline": roofline,
+        "roofline_ntt": ntt,
+        "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
+        "root": [int(x) for x in root],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.log_h, w, lb, args.cpu_cols, args.cpu_leaf_rows)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+/*
+ * pico_gpu.h -- C ABI of the MI355X (gfx950) backend for the commitment hot
+ * path of Pico's STARK prover over KoalaBear / BabyBear.
+ *
+ * This is the boundary a Plonky3-side adapter binds (see INTEGRATION.md for
+ * the Rust extern block and trait impls).  It replaces, behind the traits
+ * that Pico's StarkGenericConfig plugs in
+ * (vm/src/configs/stark_config/kb_poseidon2.rs:15-27, built at :82-93):
+ *   TwoAdicSubgroupDft::coset_lde_batch    -> pg_coset_lde
+ *   Mmcs::commit / open_batch / get_matrices / verify_batch
+ *                                          -> pg_mmcs_commit, pg_mmcs_open_batch,
+ *                                             pg_tree_matrix, pg_mmcs_verify_batch
+ *   Pcs::commit / get_evaluations_on_domain -> pg_pcs_commit, pg_pcs_evaluations
+ *   FRI commit phase, fold, PoW, query openings (p3-fri prover)
+ *                                          -> pg_fri_commit_phase, pg_fri_fold,
+ *                                             pg_fri_grind, pg_fri_answer_queries
+ *   DuplexChallenger                       -> pg_challenger_*
+ *
+ * Conventions
+ *   - Field elements are u32 in Montgomery form (x * 2^32 mod p), the
+ *     in-memory layout of p3 MontyField31, unless a `canonical` flag says
+ *     otherwise.  Matrices are row-major.
+ *   - Every function returning int returns PG_OK (0) or a negative PG_E_*
+ *     code; pg_last_error() then holds a message for the calling thread.  The
+ *     reference panics on every failure (e.g. vm/src/machine/prover.rs:593);
+ *     an adapter maps non-zero codes to panic!(pg_last_error()).
+ *   - All entry points are thread-safe.  Each call runs on a stream of its
+ *     own and returns when its device work is complete; device buffers and
+ *     trees returned by a call are ready for use by any other call.
+ *   - Heights must be powers of two (all Pico trace and LDE heights are).
+ */
+#ifndef PICO_GPU_H
+#define PICO_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_E_INVALID (-1)     /* bad argument or shape */
+#define PG_E_OOM (-2)         /* device allocation failed */
+#define PG_E_DEVICE (-3)      /* HIP runtime / kernel error */
+#define PG_E_ADMISSION (-4)   /* would exceed the context's HBM budget */
+#define PG_E_UNSUPPORTED (-5) /* valid request outside the implemented range */
+
+#define PG_KOALABEAR 0 /* p = 2^31 - 2^24 + 1 */
+#define PG_BABYBEAR 1  /* p = 2^31 - 2^27 + 1 */
+
+typedef struct pg_ctx pg_ctx;   /* one device + HBM budget + table caches */
+typedef struct pg_buf pg_buf;   /* device matrix (rows x cols u32) */
+typedef struct pg_tree pg_tree; /* MMCS prover data: matrices + digest layers */
+typedef struct pg_fri pg_fri;   /* FRI commit-phase prover data */
+
+/* DuplexChallenger<F, Poseidon2, 16, 8> state (Montgomery values).
+ * Replaces the reference's DuplexChallenger (restated in-tree at
+ * vm/src/compiler/recursion/circuit/challenger.rs:57-227). */
+typedef struct {
+    int32_t field;
+    uint32_t state[16];
+    uint32_t input[8];
+    int32_t n_input;
+    uint32_t output[8];
+    int32_t n_output;
+} pg_challenger;
+
+const char *pg_last_error(void);
+int pg_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* hbm_budget_bytes = 0 means "all free device memory". */
+int pg_create(int device, size_t hbm_budget_bytes, pg_ctx **out);
+void pg_destroy(pg_ctx *ctx);
+size_t pg_bytes_in_use(const pg_ctx *ctx);
+/* Per-kernel timing with HIP events on the launch stream (adds one host sync
+ * per launch while enabled).  Families: lde_single, lde_pass_a, lde_pass_bc,
+ * lde_pass_d, leaf_hash, leaf_hash_w8, compress, compress_inject, fri_fold.
+ * No counterpart in the reference (its spans time whole phases,
+ * vm/src/machine/prover.rs:101,414,539,588); used by the benchmark. */
+int pg_profile_enable(pg_ctx *ctx, int enable);
+int pg_profile_reset(pg_ctx *ctx);
+int pg_profile_get(pg_ctx *ctx, const char *family, double *total_ms, uint64_t *launches);
+
+/* ---- device matrices ------------------------------------------------------ */
+int pg_buf_alloc(pg_ctx *ctx, size_t rows, size_t cols, pg_buf **out);
+/* non-owning view of caller memory (e.g. a torch tensor) */
+int pg_buf_wrap(pg_ctx *ctx, void *device_ptr, size_t rows, size_t cols, pg_buf **out);
+void pg_buf_free(pg_buf *buf);
+void *pg_buf_device_ptr(const pg_buf *buf);
+size_t pg_buf_rows(const pg_buf *buf);
+size_t pg_buf_cols(const pg_buf *buf);
+/* whole-buffer upload / row-range download.  canonical != 0 means the host
+ * side holds canonical values of `field`, converted on the device. */
+int pg_upload(pg_ctx *ctx, pg_buf *buf, const uint32_t *host, int field, int canonical);
+int pg_download(pg_ctx *ctx, const pg_buf *buf, size_t row0, size_t nrows, uint32_t *host,
+                int field, int canonical);
+/* Seeded synthetic input: element i of stream `seed` is the first of
+ * SplitMix64(seed) outputs 4i..4i+3 whose top 31 bits are < p (else the last
+ * one mod p), stored in Montgomery form. */
+int pg_fill_random(pg_ctx *ctx, pg_buf *buf, int field, uint64_t seed);
+
+/* ---- TwoAdicSubgroupDft::coset_lde_batch ----------------------------------
+ * evals: h x w values on <w_h>; out: (h << added_bits) x w, the evaluations on
+ * shift * <w_N>, in bit-reversed row order when bit_reversed_rows != 0 (the
+ * order TwoAdicFriPcs commits after .bit_reverse_rows()), else natural order.
+ * h <= 2^22. */
+int pg_coset_lde(pg_ctx *ctx, int field, const pg_buf *evals, uint32_t added_bits,
+                 uint32_t shift_mont, int bit_reversed_rows, pg_buf **out);
+
+/* ---- Mmcs (MerkleTreeMmcs<.., PaddingFreeSponge<16,8,8>, TruncatedPermutation<2,8,16>, 8>)
+ * commit: matrices in caller order; take_ownership != 0 makes the tree free
+ * them (p3 commit consumes its inputs). */
+int pg_mmcs_commit(pg_ctx *ctx, int field, pg_buf *const *mats, size_t n, int take_ownership,
+                   uint32_t root[8], pg_tree **out);
+/* open_batch for n_idx query indices at once.  rows_out: n_idx * total_width
+ * words (each query: the opened row of every matrix, original order);
+ * paths_out: n_idx * log_height * 8 words (sibling digests, leaf level first). */
+int pg_mmcs_open_batch(pg_ctx *ctx, const pg_tree *tree, const uint64_t *indices, size_t n_idx,
+                       uint32_t *rows_out, uint32_t *paths_out);
+/* host-side verify_batch; returns 1 if the opening re-derives root, 0 if not,
+ * negative on bad arguments */
+int pg_mmcs_verify_batch(int field, const size_t *heights, const size_t *widths, size_t n,
+                         const uint32_t root[8], uint64_t index, const uint32_t *rows,
+                         const uint32_t *path);
+int pg_tree_root(const pg_tree *tree, uint32_t root[8]);
+size_t pg_tree_num_matrices(const pg_tree *tree);
+const pg_buf *pg_tree_matrix(const pg_tree *tree, size_t k);
+int pg_tree_log_height(const pg_tree *tree);
+size_t pg_tree_total_width(const pg_tree *tree);
+/* copy digest layer `layer` (H >> layer digests of 8 words) to host */
+int pg_tree_digest_layer(pg_ctx *ctx, const pg_tree *tree, int layer, uint32_t *host);
+void pg_tree_free(pg_tree *tree);
+
+/* ---- Pcs (TwoAdicFriPcs) ----------------------------------------------------
+ * commit: for each matrix k, LDE with shift = GENERATOR / domain_shift[k]
+ * (trace domains have shift 1), bit-reversed rows, then one MMCS tree over
+ * all LDEs (which the tree owns). */
+int pg_pcs_commit(pg_ctx *ctx, int field, const pg_buf *const *evals,
+                  const uint32_t *domain_shifts_mont, size_t n, uint32_t log_blowup,
+                  uint32_t root[8], pg_tree **out);
+/* get_evaluations_on_domain: the first `rows` committed rows of matrix k,
+ * returned in natural order (rows * width words). */
+int pg_pcs_evaluations(pg_ctx *ctx, const pg_tree *tree, size_t k, size_t rows, uint32_t *host);
+
+/* ---- FRI ----------------------------------------------------------------- */
+/* out = fold_even_odd(v, beta): v is n x 4 (ext4, bit-reversed order), out is
+ * (n/2) x 4. */
+int pg_fri_fold(pg_ctx *ctx, int field, const pg_buf *v_ext4, const uint32_t beta_mont[4],
+                pg_buf **out);
+/* Commit phase.  inputs[k] (k <= max_log, may be NULL) is the reduced-opening
+ * vector of length 2^k (ext4 rows); the largest one starts the fold.  Loop
+ * while the length exceeds 2^log_stop: commit rows of 2 ext4 elements,
+ * observe the root, sample beta, fold, add inputs of the new length.
+ * final_mode 0: the final vector must be constant and one element is
+ * observed (reference behaviour); 1: every final element is observed.
+ * roots_out: rounds * 8; final_out: 2^log_stop * 4 words.  Returns the number
+ * of rounds through *rounds_out. */
+int pg_fri_commit_phase(pg_ctx *ctx, int field, pg_challenger *ch, const pg_buf *const *inputs,
+                        int max_log, int log_stop, int final_mode, uint32_t *roots_out,
+                        uint32_t *betas_out, uint32_t *final_out, int *rounds_out,
+                        pg_fri **out);
+/* PoW: find the smallest witness w with observe(w); sample_bits(bits) == 0,
+ * then apply exactly that observe + sample to *ch. */
+int pg_fri_grind(pg_ctx *ctx, int field, pg_challenger *ch, int bits, uint32_t *witness_out);
+/* For each query index q (log_max bits) and round r: the sibling value
+ * v_r[(q >> r) ^ 1] (4 words) and the Merkle path of leaf (q >> r) >> 1 in
+ * round r's tree (log_max - 1 - r digests).  siblings_out: nq * rounds * 4;
+ * paths_out: nq * sum_r (log_max - 1 - r) * 8, query-major, round-major. */
+int pg_fri_answer_queries(pg_ctx *ctx, const pg_fri *fri, const uint64_t *indices, size_t nq,
+                          uint32_t *siblings_out, uint32_t *paths_out);
+int pg_fri_num_rounds(const pg_fri *fri);
+const pg_tree *pg_fri_round_tree(const pg_fri *fri, int round);
+void pg_fri_free(pg_fri *fri);
+
+/* ---- DuplexChallenger (host) --------------------------------------------- */
+void pg_challenger_init(pg_challenger *ch, int field);
+void pg_challenger_observe(pg_challenger *ch, uint32_t value_mont);
+uint32_t pg_challenger_sample(pg_challenger *ch);
+void pg_challenger_sample_ext(pg_challenger *ch, uint32_t out_mont[4]);
+/* low `bits` bits of the canonical value of one sample */
+uint32_t pg_challenger_sample_bits(pg_challenger *ch, int bits);
+/* observe(witness) then sample_bits(bits) == 0; returns 1 if it holds */
+int pg_challenger_check_witness(pg_challenger *ch, int bits, uint32_t witness_canonical);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PICO_GPU_H */

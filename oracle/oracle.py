@@ -1,0 +1,284 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  All arrays are
+canonical field elements (uint32 in [0, p)).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+KOALABEAR = 0
+BABYBEAR = 1
+PRIMES = {KOALABEAR: 2130706433, BABYBEAR: 2013265921}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    """Compile liboracle.so in place (make)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        u32, u64, sz, i = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int
+        P = ctypes.c_void_p
+        sig = {
+            "or_prime": (u32, [i]), "or_add": (u32, [i, u32, u32]),
+            "or_sub": (u32, [i, u32, u32]), "or_mul": (u32, [i, u32, u32]),
+            "or_pow": (u32, [i, u32, u64]), "or_inv": (u32, [i, u32]),
+            "or_generator": (u32, [i]), "or_two_adic_generator": (u32, [i, i]),
+            "or_two_adicity": (i, [i]), "or_to_monty": (u32, [i, u32]),
+            "or_from_monty": (u32, [i, u32]),
+            "or_ext_mul": (None, [i, P, P, P]), "or_ext_inv": (None, [i, P, P]),
+            "or_poseidon2": (None, [i, P]), "or_sponge": (None, [i, P, sz, P]),
+            "or_compress": (None, [i, P, P, P]),
+            "or_fill_random": (None, [i, u64, P, sz]),
+            "or_coset_lde": (None, [i, P, sz, sz, i, u32, i, P]),
+            "or_coset_lde_naive": (None, [i, P, sz, sz, i, u32, i, P]),
+            "or_mmcs_digest_words": (sz, [P, sz]),
+            "or_mmcs_commit": (i, [i, P, sz, P, P]),
+            "or_mmcs_open": (i, [P, sz, P, sz, P, P]),
+            "or_mmcs_verify": (i, [i, P, P, sz, P, sz, P, P]),
+            "or_fri_fold": (None, [i, P, sz, P, P]),
+            "or_ch_init": (None, [P, i]), "or_ch_observe": (None, [P, u32]),
+            "or_ch_sample": (u32, [P]), "or_ch_sample_ext": (None, [P, P]),
+            "or_ch_sample_bits": (u32, [P, i]), "or_ch_check_witness": (i, [P, i, u32]),
+            "or_fri_commit_phase": (i, [P, P, i, i, P, P, P, P, P]),
+            "or_num_threads": (i, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ------------------------------------------------------------------ field --
+def prime(field):
+    return PRIMES[field]
+
+
+def generator(field):
+    return lib().or_generator(field)
+
+
+def two_adic_generator(field, log_n):
+    return lib().or_two_adic_generator(field, log_n)
+
+
+def to_monty(field, a):
+    p = PRIMES[field]
+    return ((np.asarray(a, dtype=np.uint64) << np.uint64(32)) % np.uint64(p)).astype(np.uint32)
+
+
+def from_monty(field, a):
+    p = PRIMES[field]
+    rinv = pow(2 ** 32, p - 2, p)
+    return ((np.asarray(a, dtype=np.uint64) * np.uint64(rinv)) % np.uint64(p)).astype(np.uint32)
+
+
+def ext_mul(field, a, b):
+    a, b, out = _u32(a), _u32(b), np.zeros(4, np.uint32)
+    lib().or_ext_mul(field, _ptr(a), _ptr(b), _ptr(out))
+    return out
+
+
+def ext_inv(field, a):
+    a, out = _u32(a), np.zeros(4, np.uint32)
+    lib().or_ext_inv(field, _ptr(a), _ptr(out))
+    return out
+
+
+# -------------------------------------------------------------- poseidon2 --
+def poseidon2(field, state):
+    s = _u32(state).copy()
+    assert s.shape == (16,)
+    lib().or_poseidon2(field, _ptr(s))
+    return s
+
+
+def sponge(field, values):
+    v = _u32(values).ravel()
+    out = np.zeros(8, np.uint32)
+    lib().or_sponge(field, _ptr(v), v.size, _ptr(out))
+    return out
+
+
+def compress(field, left, right):
+    l, r, out = _u32(left), _u32(right), np.zeros(8, np.uint32)
+    lib().or_compress(field, _ptr(l), _ptr(r), _ptr(out))
+    return out
+
+
+def random_matrix(field, seed, h, w):
+    out = np.empty((h, w), np.uint32)
+    lib().or_fill_random(field, seed, _ptr(out), out.size)
+    return out
+
+
+# -------------------------------------------------------------------- LDE --
+def coset_lde(field, mat, added_bits, shift, bit_reverse=True, naive=False):
+    m = _u32(mat)
+    h, w = m.shape
+    out = np.empty((h << added_bits, w), np.uint32)
+    fn = lib().or_coset_lde_naive if naive else lib().or_coset_lde
+    fn(field, _ptr(m), h, w, added_bits, shift, int(bit_reverse), _ptr(out))
+    return out
+
+
+# ------------------------------------------------------------------- MMCS --
+class _Mat(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("height", ctypes.c_size_t),
+                ("width", ctypes.c_size_t)]
+
+
+class MerkleTree:
+    """Oracle commitment: keeps the matrices and every digest layer."""
+
+    def __init__(self, field, mats):
+        self.field = field
+        self.mats = [_u32(m) for m in mats]
+        self._c = (_Mat * len(self.mats))(*[
+            _Mat(_ptr(m).value, m.shape[0], m.shape[1]) for m in self.mats])
+        words = lib().or_mmcs_digest_words(self._c, len(self.mats))
+        if words == 0:
+            raise ValueError("heights must be non-zero powers of two")
+        self.digests = np.zeros(words, np.uint32)
+        self.root = np.zeros(8, np.uint32)
+        rc = lib().or_mmcs_commit(field, self._c, len(self.mats), _ptr(self.digests),
+                                  _ptr(self.root))
+        assert rc == 0
+        self.max_height = max(m.shape[0] for m in self.mats)
+        self.log_max = self.max_height.bit_length() - 1
+
+    def layer(self, j):
+        off = sum(self.max_height >> t for t in range(j))
+        n = self.max_height >> j
+        return self.digests[8 * off: 8 * (off + n)].reshape(n, 8)
+
+    def open(self, index):
+        wtot = sum(m.shape[1] for m in self.mats)
+        rows = np.zeros(max(wtot, 1), np.uint32)
+        path = np.zeros(8 * max(self.log_max, 1), np.uint32)
+        rc = lib().or_mmcs_open(self._c, len(self.mats), _ptr(self.digests), index,
+                                _ptr(rows), _ptr(path))
+        assert rc == 0
+        out, pos = [], 0
+        for m in self.mats:
+            out.append(rows[pos:pos + m.shape[1]].copy())
+            pos += m.shape[1]
+        return out, path[:8 * self.log_max].reshape(self.log_max, 8)
+
+
+def mmcs_verify(field, heights, widths, root, index, rows, path):
+    hs = np.asarray(heights, np.uint64).astype(np.uintp)
+    ws = np.asarray(widths, np.uint64).astype(np.uintp)
+    r = _u32(np.concatenate([np.asarray(x, np.uint32).ravel() for x in rows])
+             if len(rows) else np.zeros(1, np.uint32))
+    pth = _u32(path).ravel()
+    if pth.size == 0:
+        pth = np.zeros(8, np.uint32)
+    rt = _u32(root)
+    return bool(lib().or_mmcs_verify(field, _ptr(hs), _ptr(ws), len(heights), _ptr(rt), index,
+                                     _ptr(r), _ptr(pth)))
+
+
+# -------------------------------------------------------------------- FRI --
+def fri_fold(field, v, beta):
+    v = _u32(v).reshape(-1, 4)
+    n = v.shape[0]
+    out = np.zeros((n // 2, 4), np.uint32)
+    b = _u32(beta)
+    lib().or_fri_fold(field, _ptr(v), n, _ptr(b), _ptr(out))
+    return out
+
+
+class _Ch(ctypes.Structure):
+    _fields_ = [("field", ctypes.c_int), ("state", ctypes.c_uint32 * 16),
+                ("inp", ctypes.c_uint32 * 8), ("n_in", ctypes.c_int),
+                ("out", ctypes.c_uint32 * 8), ("n_out", ctypes.c_int)]
+
+
+class Challenger:
+    def __init__(self, field):
+        self._c = _Ch()
+        self.field = field
+        lib().or_ch_init(ctypes.byref(self._c), field)
+
+    def observe(self, v):
+        for x in np.atleast_1d(np.asarray(v, np.uint32)).ravel():
+            lib().or_ch_observe(ctypes.byref(self._c), int(x))
+
+    def sample(self):
+        return lib().or_ch_sample(ctypes.byref(self._c))
+
+    def sample_ext(self):
+        out = np.zeros(4, np.uint32)
+        lib().or_ch_sample_ext(ctypes.byref(self._c), _ptr(out))
+        return out
+
+    def sample_bits(self, bits):
+        return lib().or_ch_sample_bits(ctypes.byref(self._c), bits)
+
+    def check_witness(self, bits, witness):
+        return bool(lib().or_ch_check_witness(ctypes.byref(self._c), bits, witness))
+
+    def clone(self):
+        c = Challenger(self.field)
+        ctypes.memmove(ctypes.byref(c._c), ctypes.byref(self._c), ctypes.sizeof(_Ch))
+        return c
+
+    def state(self):
+        return np.array(self._c.state, np.uint32)
+
+
+def fri_commit_phase(ch, v, log_stop, inputs=None):
+    """Returns (roots[r,8], layers (list of committed vectors), final[.,4], betas[r,4])."""
+    v = _u32(v).reshape(-1, 4)
+    log_n = v.shape[0].bit_length() - 1
+    rounds = log_n - log_stop
+    roots = np.zeros((max(rounds, 1), 8), np.uint32)
+    betas = np.zeros((max(rounds, 1), 4), np.uint32)
+    total = sum((1 << (log_n - r)) for r in range(rounds))
+    layers = np.zeros((max(total, 1), 4), np.uint32)
+    final = np.zeros((1 << log_stop, 4), np.uint32)
+    keep = []
+    arr = (ctypes.c_void_p * (log_n + 1))()
+    if inputs:
+        for lg, vec in inputs.items():
+            a = _u32(vec)
+            keep.append(a)
+            arr[lg] = _ptr(a).value
+    r = lib().or_fri_commit_phase(ctypes.byref(ch._c), _ptr(v), log_n, log_stop,
+                                  arr if inputs else None, _ptr(roots), _ptr(layers),
+                                  _ptr(final), _ptr(betas))
+    assert r == rounds
+    out_layers, off = [], 0
+    for rr in range(rounds):
+        n = 1 << (log_n - rr)
+        out_layers.append(layers[off:off + n].copy())
+        off += n
+    return roots[:rounds], out_layers, final, betas[:rounds]
+
+
+def num_threads():
+    return lib().or_num_threads()

@@ -1,0 +1,426 @@
+"""Python mirror of the Plonky3 prover interfaces that Pico's StarkGenericConfig
+plugs in (vm/src/configs/stark_config/kb_poseidon2.rs:15-27, 82-93), backed by
+the gfx950 HIP library through its C ABI.
+
+Names follow the reference's types: ``Radix2DitParallel.coset_lde_batch``,
+``MerkleTreeMmcs.commit / open_batch / get_matrices / verify_batch``,
+``TwoAdicFriPcs.commit / get_evaluations_on_domain``, the FRI commit phase and
+``DuplexChallenger``.  Field elements cross this Python API as canonical
+integers (numpy uint32 in [0, p)); the conversion to the Montgomery form the
+device and the ABI use happens at the boundary.  Errors raise ``PgError``
+where the reference panics.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import BABYBEAR, KOALABEAR, Challenger as _ChStruct, PgError, check, lib
+
+PRIMES = {KOALABEAR: 2130706433, BABYBEAR: 2013265921}
+GENERATORS = {KOALABEAR: 3, BABYBEAR: 31}
+TWO_ADICITY = {KOALABEAR: 24, BABYBEAR: 27}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def to_monty(field, a):
+    p = np.uint64(PRIMES[field])
+    return ((np.asarray(a, dtype=np.uint64) << np.uint64(32)) % p).astype(np.uint32)
+
+
+def from_monty(field, a):
+    p = PRIMES[field]
+    rinv = np.uint64(pow(2 ** 32, p - 2, p))
+    return ((np.asarray(a, dtype=np.uint64) * rinv) % np.uint64(p)).astype(np.uint32)
+
+
+def two_adic_generator(field, log_n):
+    p = PRIMES[field]
+    return pow(GENERATORS[field], (p - 1) >> log_n, p)
+
+
+class Context:
+    """One device, an HBM budget (bytes, 0 = all free memory) and table caches."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = ctypes.c_void_p()
+        check(lib().pg_create(device, hbm_budget, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pg_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def bytes_in_use(self):
+        return lib().pg_bytes_in_use(self.handle)
+
+    def profile(self, enable=True):
+        check(lib().pg_profile_enable(self.handle, int(enable)))
+
+    def profile_reset(self):
+        check(lib().pg_profile_reset(self.handle))
+
+    def profile_get(self, family):
+        """(total milliseconds, launches) recorded for a kernel family."""
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        check(lib().pg_profile_get(self.handle, family.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def matrix(self, rows, cols):
+        h = ctypes.c_void_p()
+        check(lib().pg_buf_alloc(self.handle, rows, cols, ctypes.byref(h)))
+        return DeviceMatrix(self, h, owned=True)
+
+    def upload(self, host, field=None):
+        """Copy a 2-D uint32 array to a new device matrix.  With `field`, the
+        host holds canonical values, converted to Montgomery form on device;
+        without it the values are copied as they are (Montgomery)."""
+        a = np.ascontiguousarray(host, dtype=np.uint32)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        m = self.matrix(a.shape[0], a.shape[1])
+        try:
+            check(lib().pg_upload(self.handle, m.handle, _ptr(a), -1 if field is None else field,
+                                  int(field is not None)))
+        except Exception:
+            m.free()
+            raise
+        return m
+
+    def random_matrix(self, field, seed, rows, cols):
+        m = self.matrix(rows, cols)
+        check(lib().pg_fill_random(self.handle, m.handle, field, seed))
+        return m
+
+    def wrap(self, device_ptr, rows, cols):
+        h = ctypes.c_void_p()
+        check(lib().pg_buf_wrap(self.handle, ctypes.c_void_p(device_ptr), rows, cols, ctypes.byref(h)))
+        return DeviceMatrix(self, h, owned=True)  # the view object itself is freed; memory is not
+
+
+class DeviceMatrix:
+    """Row-major matrix of Montgomery-form u32 in HBM (a pg_buf)."""
+
+    def __init__(self, ctx, handle, owned):
+        self.ctx = ctx
+        self.handle = handle
+        self._owned = owned
+        self.rows = lib().pg_buf_rows(handle)
+        self.cols = lib().pg_buf_cols(handle)
+
+    @property
+    def device_ptr(self):
+        return lib().pg_buf_device_ptr(self.handle)
+
+    def height(self):
+        return self.rows
+
+    def width(self):
+        return self.cols
+
+    def to_host(self, field=None, row0=0, nrows=None):
+        """Rows [row0, row0 + nrows): canonical values of `field`, or the raw
+        Montgomery words when no field is given."""
+        nrows = self.rows - row0 if nrows is None else nrows
+        out = np.empty((nrows, self.cols), np.uint32)
+        check(lib().pg_download(self.ctx.handle, self.handle, row0, nrows, _ptr(out),
+                                -1 if field is None else field, int(field is not None)))
+        return out
+
+    def free(self):
+        if self._owned and self.handle:
+            lib().pg_buf_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ DFT --
+class Radix2DitParallel:
+    """TwoAdicSubgroupDft on the device (p3 Radix2DitParallel's role)."""
+
+    def __init__(self, ctx, field):
+        self.ctx, self.field = ctx, field
+
+    def coset_lde_batch(self, mat, added_bits, shift, bit_reversed=True):
+        """LDE of `mat` (values on <w_h>) onto shift * <w_N>; rows in committed
+        (bit-reversed) order unless bit_reversed=False.  `shift` is canonical."""
+        h = ctypes.c_void_p()
+        s = int(to_monty(self.field, shift))
+        check(lib().pg_coset_lde(self.ctx.handle, self.field, mat.handle, added_bits, s,
+                                 int(bool(bit_reversed)), ctypes.byref(h)))
+        return DeviceMatrix(self.ctx, h, owned=True)
+
+
+# ----------------------------------------------------------------- MMCS --
+class ProverData:
+    """MMCS prover data: the committed matrices and every digest layer."""
+
+    def __init__(self, ctx, field, handle, keep=()):
+        self.ctx, self.field, self.handle = ctx, field, handle
+        self._keep = list(keep)  # caller matrices the tree references
+        self.log_height = lib().pg_tree_log_height(handle)
+        self.total_width = lib().pg_tree_total_width(handle)
+        self.num_matrices = lib().pg_tree_num_matrices(handle)
+
+    def root(self):
+        r = np.zeros(8, np.uint32)
+        check(lib().pg_tree_root(self.handle, _ptr(r)))
+        return from_monty(self.field, r)
+
+    def digest_layer(self, layer):
+        n = 1 << (self.log_height - layer)
+        out = np.zeros((n, 8), np.uint32)
+        check(lib().pg_tree_digest_layer(self.ctx.handle, self.handle, layer, _ptr(out)))
+        return from_monty(self.field, out)
+
+    def free(self):
+        if self.handle:
+            lib().pg_tree_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class MerkleTreeMmcs:
+    """Mmcs with PaddingFreeSponge<Poseidon2,16,8,8> leaves and
+    TruncatedPermutation<Poseidon2,2,8,16> compression (kb_poseidon2.rs:17-20)."""
+
+    def __init__(self, ctx, field):
+        self.ctx, self.field = ctx, field
+
+    def commit(self, mats, take_ownership=False):
+        arr = (ctypes.c_void_p * len(mats))(*[m.handle for m in mats])
+        root = np.zeros(8, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().pg_mmcs_commit(self.ctx.handle, self.field, arr, len(mats), int(take_ownership),
+                                   _ptr(root), ctypes.byref(h)))
+        if take_ownership:
+            for m in mats:
+                m._owned = False  # the tree frees them
+        pd = ProverData(self.ctx, self.field, h, keep=mats)
+        return from_monty(self.field, root), pd
+
+    def commit_matrix(self, mat):
+        return self.commit([mat])
+
+    def get_matrices(self, data):
+        return [DeviceMatrix(self.ctx, lib().pg_tree_matrix(data.handle, k), owned=False)
+                for k in range(data.num_matrices)]
+
+    def open_many(self, indices, data):
+        """Batched open_batch: (rows[nq, total_width], proofs[nq, log_h, 8]), canonical."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        nq = idx.size
+        rows = np.zeros((nq, max(data.total_width, 1)), np.uint32)
+        paths = np.zeros((nq, max(data.log_height, 1), 8), np.uint32)
+        check(lib().pg_mmcs_open_batch(self.ctx.handle, data.handle, _ptr(idx), nq, _ptr(rows),
+                                       _ptr(paths)))
+        return (from_monty(self.field, rows[:, :data.total_width]),
+                from_monty(self.field, paths[:, :data.log_height]))
+
+    def open_batch(self, index, data):
+        """(opened rows per matrix in original order, sibling path)."""
+        rows, paths = self.open_many([index], data)
+        widths = [m.cols for m in self.get_matrices(data)]
+        out, pos = [], 0
+        for w in widths:
+            out.append(rows[0, pos:pos + w])
+            pos += w
+        return out, paths[0]
+
+    def verify_batch(self, commit, dimensions, index, opened_values, proof):
+        """dimensions: [(width, height)] in original order."""
+        hs = np.array([d[1] for d in dimensions], np.uintp)
+        ws = np.array([d[0] for d in dimensions], np.uintp)
+        flat = [np.asarray(v, np.uint32).ravel() for v in opened_values]
+        rows = to_monty(self.field, np.concatenate(flat) if flat else np.zeros(1, np.uint32))
+        rows = np.ascontiguousarray(rows if rows.size else np.zeros(1, np.uint32))
+        path = np.ascontiguousarray(to_monty(self.field, np.asarray(proof, np.uint32).reshape(-1)))
+        if path.size == 0:
+            path = np.zeros(8, np.uint32)
+        root = np.ascontiguousarray(to_monty(self.field, commit))
+        rc = lib().pg_mmcs_verify_batch(self.field, _ptr(hs), _ptr(ws), len(dimensions), _ptr(root),
+                                        index, _ptr(rows), _ptr(path))
+        if rc < 0:
+            check(rc)
+        return rc == 1
+
+
+# ------------------------------------------------------------------ PCS --
+class TwoAdicMultiplicativeCoset:
+    def __init__(self, log_n, shift=1):
+        self.log_n, self.shift = log_n, shift
+
+    def size(self):
+        return 1 << self.log_n
+
+
+class TwoAdicFriPcs:
+    """Pcs::commit and get_evaluations_on_domain on device-resident data."""
+
+    def __init__(self, ctx, field, log_blowup):
+        self.ctx, self.field, self.log_blowup = ctx, field, log_blowup
+
+    def natural_domain_for_degree(self, n):
+        return TwoAdicMultiplicativeCoset(n.bit_length() - 1, 1)
+
+    def commit(self, evaluations):
+        """evaluations: [(domain, DeviceMatrix)] -> (root, ProverData)."""
+        n = len(evaluations)
+        arr = (ctypes.c_void_p * n)(*[m.handle for _, m in evaluations])
+        shifts = np.ascontiguousarray(to_monty(self.field, [d.shift for d, _ in evaluations]))
+        root = np.zeros(8, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().pg_pcs_commit(self.ctx.handle, self.field, arr, _ptr(shifts), n, self.log_blowup,
+                                  _ptr(root), ctypes.byref(h)))
+        return from_monty(self.field, root), ProverData(self.ctx, self.field, h)
+
+    def get_evaluations_on_domain(self, data, idx, domain):
+        m = lib().pg_tree_matrix(data.handle, idx)
+        cols = lib().pg_buf_cols(m)
+        out = np.zeros((domain.size(), cols), np.uint32)
+        check(lib().pg_pcs_evaluations(self.ctx.handle, data.handle, idx, domain.size(), _ptr(out)))
+        return from_monty(self.field, out)
+
+
+# ----------------------------------------------------------- challenger --
+class DuplexChallenger:
+    """DuplexChallenger<F, Poseidon2, 16, 8>; values are canonical here."""
+
+    def __init__(self, field):
+        self.field = field
+        self.c = _ChStruct()
+        lib().pg_challenger_init(ctypes.byref(self.c), field)
+
+    def observe(self, values):
+        for v in np.atleast_1d(np.asarray(values, np.uint64)).ravel():
+            lib().pg_challenger_observe(ctypes.byref(self.c), int(to_monty(self.field, int(v))))
+
+    def sample(self):
+        return int(from_monty(self.field, lib().pg_challenger_sample(ctypes.byref(self.c))))
+
+    def sample_ext(self):
+        out = np.zeros(4, np.uint32)
+        lib().pg_challenger_sample_ext(ctypes.byref(self.c), _ptr(out))
+        return from_monty(self.field, out)
+
+    def sample_bits(self, bits):
+        return int(lib().pg_challenger_sample_bits(ctypes.byref(self.c), bits))
+
+    def check_witness(self, bits, witness):
+        return bool(lib().pg_challenger_check_witness(ctypes.byref(self.c), bits, witness))
+
+    def clone(self):
+        c = DuplexChallenger(self.field)
+        ctypes.memmove(ctypes.byref(c.c), ctypes.byref(self.c), ctypes.sizeof(_ChStruct))
+        return c
+
+
+# ------------------------------------------------------------------ FRI --
+def fri_fold(ctx, field, v, beta):
+    """fold_even_odd of an n x 4 ext4 device vector by canonical beta[4]."""
+    b = np.ascontiguousarray(to_monty(field, beta))
+    h = ctypes.c_void_p()
+    check(lib().pg_fri_fold(ctx.handle, field, v.handle, _ptr(b), ctypes.byref(h)))
+    return DeviceMatrix(ctx, h, owned=True)
+
+
+class FriProverData:
+    def __init__(self, ctx, field, handle, roots, betas, final):
+        self.ctx, self.field, self.handle = ctx, field, handle
+        self.commits, self.betas, self.final_poly = roots, betas, final
+        self.num_rounds = lib().pg_fri_num_rounds(handle)
+
+    def round_data(self, r):
+        t = lib().pg_fri_round_tree(self.handle, r)
+        pd = ProverData(self.ctx, self.field, t)
+        pd.free = lambda: None  # owned by the FRI data
+        return pd
+
+    def free(self):
+        if self.handle:
+            lib().pg_fri_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def fri_commit_phase(ctx, field, challenger, inputs, log_stop, final_mode=0):
+    """inputs: {log_len: DeviceMatrix (2^log_len x 4)}.  Returns FriProverData
+    (commits, betas, final vector canonical)."""
+    max_log = max(inputs)
+    arr = (ctypes.c_void_p * (max_log + 1))()
+    for k, m in inputs.items():
+        arr[k] = m.handle
+    rounds = max_log - log_stop
+    roots = np.zeros((max(rounds, 1), 8), np.uint32)
+    betas = np.zeros((max(rounds, 1), 4), np.uint32)
+    final = np.zeros((1 << log_stop, 4), np.uint32)
+    nr = ctypes.c_int()
+    h = ctypes.c_void_p()
+    check(lib().pg_fri_commit_phase(ctx.handle, field, ctypes.byref(challenger.c), arr, max_log,
+                                    log_stop, final_mode, _ptr(roots), _ptr(betas), _ptr(final),
+                                    ctypes.byref(nr), ctypes.byref(h)))
+    r = nr.value
+    return FriProverData(ctx, field, h, from_monty(field, roots[:r]), from_monty(field, betas[:r]),
+                         from_monty(field, final))
+
+
+def fri_grind(ctx, field, challenger, bits):
+    w = ctypes.c_uint32()
+    check(lib().pg_fri_grind(ctx.handle, field, ctypes.byref(challenger.c), bits, ctypes.byref(w)))
+    return w.value
+
+
+def fri_answer_queries(data, indices, log_max):
+    """Returns (siblings[nq, R, 4], paths: list over queries of list over rounds
+    of [log_max-1-r, 8] arrays), canonical."""
+    idx = np.ascontiguousarray(indices, dtype=np.uint64)
+    nq, R = idx.size, data.num_rounds
+    plens = [log_max - 1 - r for r in range(R)]
+    per_q = 8 * sum(plens)
+    sib = np.zeros((nq, max(R, 1), 4), np.uint32)
+    paths = np.zeros(max(nq * per_q, 1), np.uint32)
+    check(lib().pg_fri_answer_queries(data.ctx.handle, data.handle, _ptr(idx), nq, _ptr(sib),
+                                      _ptr(paths)))
+    sib = from_monty(data.field, sib[:, :R])
+    paths = from_monty(data.field, paths[:nq * per_q]).reshape(nq, per_q) if nq else paths[:0]
+    out = []
+    for q in range(nq):
+        lst, off = [], 0
+        for L in plens:
+            lst.append(paths[q, off:off + 8 * L].reshape(L, 8))
+            off += 8 * L
+        out.append(lst)
+    return sib, out
+
+
+__all__ = [
+    "KOALABEAR", "BABYBEAR", "PRIMES", "GENERATORS", "Context", "DeviceMatrix",
+    "Radix2DitParallel", "MerkleTreeMmcs", "ProverData", "TwoAdicFriPcs",
+    "TwoAdicMultiplicativeCoset", "DuplexChallenger", "fri_fold", "fri_commit_phase",
+    "fri_grind", "fri_answer_queries", "to_monty", "from_monty", "two_adic_generator", "PgError",
+]

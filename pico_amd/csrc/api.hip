@@ -1,0 +1,1015 @@
+// api.hip -- the C ABI declared in include/pico_gpu.h: contexts, device
+// matrices, HBM budget accounting, twiddle / coset-scale caches, and the host
+// orchestration of the MMCS, PCS and FRI prover steps.
+//
+// Host-side order of operations follows the reference's prover:
+//   Mmcs::commit       p3 MerkleTreeMmcs (stable sort by height desc, leaf
+//                      layer over the tallest matrices, compress + inject;
+//                      pinned by vm/src/compiler/recursion/circuit/fri.rs:389-436)
+//   Pcs::commit        LDE shift = GENERATOR / domain.shift, bit-reversed rows
+//                      (vm/src/machine/prover.rs:238-244, circuit/fri.rs:151-171)
+//   FRI commit phase   commit -> observe root -> sample beta -> fold -> add the
+//                      reduced opening of the new length (circuit/fri.rs:36-73,
+//                      :288-387); PoW check_witness (circuit/challenger.rs:210-222)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/pico_gpu.h"
+#include "kernels.h"
+#include "poseidon2.h"
+
+namespace pg {
+hipError_t launch_leaf_hash_w8(int field, const uint32_t* data, uint64_t stride, uint32_t width,
+                               size_t rows, uint32_t* digests, hipStream_t stream);
+}
+
+using namespace pg;
+
+// ---------------------------------------------------------------- errors --
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(PG_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define PG_TRY(expr)          \
+    do {                      \
+        int r_ = (expr);      \
+        if (r_ != PG_OK) return r_; \
+    } while (0)
+
+extern "C" const char* pg_last_error(void) { return g_err.c_str(); }
+extern "C" int pg_version(void) { return 1; }
+
+// ---------------------------------------------------------------- context --
+struct DevMem {  // device allocation accounted against the context budget
+    pg_ctx* ctx = nullptr;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct pg_ctx {
+    int device = 0;
+    size_t budget = 0;
+    std::atomic<size_t> in_use{0};
+    std::mutex mu;
+    std::vector<hipStream_t> streams;
+    size_t next_stream = 0;
+    // caches: twiddles per (field, log n); coset scale per (field, log h, b, shift)
+    std::map<std::pair<int, int>, std::shared_ptr<DevMem>> twiddles;
+    std::map<std::tuple<int, int, int, uint32_t>, std::shared_ptr<DevMem>> scales;
+    std::vector<std::tuple<int, int, int, uint32_t>> scale_order;  // insertion order for eviction
+    size_t scale_bytes = 0;
+    // per-kernel-family event timings (profiling mode only)
+    std::atomic<bool> profiling{false};
+    std::map<std::string, std::pair<double, uint64_t>> prof;
+};
+
+// Times each launch with a pair of HIP events recorded on the launch stream
+// and waits for the second one, so per-kernel durations are exact (at the
+// cost of one host sync per launch; enabled only by pg_profile_enable).
+struct EventTimer : KernelTimer {
+    pg_ctx* ctx;
+    hipEvent_t a = nullptr, b = nullptr;
+    std::string cur;
+    explicit EventTimer(pg_ctx* c) : ctx(c) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+    }
+    ~EventTimer() override {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    void begin(const char* name, hipStream_t s) override {
+        cur = name;
+        (void)hipEventRecord(a, s);
+    }
+    void end(hipStream_t s) override {
+        (void)hipEventRecord(b, s);
+        if (hipEventSynchronize(b) != hipSuccess) return;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return;
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto& e = ctx->prof[cur];
+        e.first += ms;
+        e.second += 1;
+    }
+};
+
+static std::unique_ptr<EventTimer> make_timer(pg_ctx* ctx) {
+    return ctx->profiling.load() ? std::unique_ptr<EventTimer>(new EventTimer(ctx)) : nullptr;
+}
+
+// time one launch expression when profiling
+#define TIMED(kt, name, s, expr)            \
+    do {                                    \
+        if (kt) (kt)->begin(name, s);       \
+        hipError_t te_ = (expr);            \
+        if (kt) (kt)->end(s);               \
+        HIP_TRY(te_);                       \
+    } while (0)
+
+static int ctx_alloc(pg_ctx* ctx, size_t bytes, void** out) {
+    *out = nullptr;
+    if (bytes == 0) bytes = 4;
+    size_t prev = ctx->in_use.fetch_add(bytes);
+    if (ctx->budget && prev + bytes > ctx->budget) {
+        ctx->in_use.fetch_sub(bytes);
+        return fail(PG_E_ADMISSION, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu in use)",
+                    bytes, prev, ctx->budget);
+    }
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        ctx->in_use.fetch_sub(bytes);
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(PG_E_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return PG_OK;
+}
+
+static void ctx_free(pg_ctx* ctx, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    if (bytes == 0) bytes = 4;
+    ctx->in_use.fetch_sub(bytes);
+}
+
+static std::shared_ptr<DevMem> make_devmem(pg_ctx* ctx, size_t bytes, int* rc) {
+    void* p = nullptr;
+    *rc = ctx_alloc(ctx, bytes, &p);
+    if (*rc != PG_OK) return nullptr;
+    auto d = std::shared_ptr<DevMem>(new DevMem{ctx, p, bytes}, [](DevMem* m) {
+        ctx_free(m->ctx, m->ptr, m->bytes);
+        delete m;
+    });
+    return d;
+}
+
+// Each API call leases one stream and synchronises it before returning.
+struct Lease {
+    pg_ctx* ctx;
+    hipStream_t s;
+    explicit Lease(pg_ctx* c) : ctx(c) {
+        std::lock_guard<std::mutex> g(c->mu);
+        s = c->streams[c->next_stream++ % c->streams.size()];
+    }
+};
+
+static int bind_device(pg_ctx* ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return PG_OK;
+}
+
+extern "C" int pg_create(int device, size_t budget, pg_ctx** out) {
+    if (!out) return fail(PG_E_INVALID, "out is null");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(PG_E_INVALID, "device %d of %d", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<pg_ctx> c(new pg_ctx);
+    c->device = device;
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        budget = free_b;
+    }
+    c->budget = budget;
+    for (int i = 0; i < 4; i++) {
+        hipStream_t s;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        c->streams.push_back(s);
+    }
+    *out = c.release();
+    return PG_OK;
+}
+
+extern "C" void pg_destroy(pg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    for (auto s : ctx->streams) (void)hipStreamSynchronize(s);
+    ctx->twiddles.clear();
+    ctx->scales.clear();
+    for (auto s : ctx->streams) (void)hipStreamDestroy(s);
+    delete ctx;
+}
+
+extern "C" size_t pg_bytes_in_use(const pg_ctx* ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+extern "C" int pg_profile_enable(pg_ctx* ctx, int enable) {
+    if (!ctx) return fail(PG_E_INVALID, "null context");
+    ctx->profiling.store(enable != 0);
+    return PG_OK;
+}
+
+extern "C" int pg_profile_reset(pg_ctx* ctx) {
+    if (!ctx) return fail(PG_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->prof.clear();
+    return PG_OK;
+}
+
+extern "C" int pg_profile_get(pg_ctx* ctx, const char* name, double* total_ms, uint64_t* launches) {
+    if (!ctx || !name || !total_ms || !launches) return fail(PG_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->prof.find(name);
+    *total_ms = it == ctx->prof.end() ? 0.0 : it->second.first;
+    *launches = it == ctx->prof.end() ? 0 : it->second.second;
+    return PG_OK;
+}
+
+// Twiddle table w_{2^log_n}^e, e < 2^log_n; built once per (field, log_n).
+static int get_twiddles(pg_ctx* ctx, hipStream_t s, int field, int log_n, std::shared_ptr<DevMem>* out) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto key = std::make_pair(field, log_n);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end()) {
+        *out = it->second;
+        return PG_OK;
+    }
+    int rc;
+    auto m = make_devmem(ctx, sizeof(uint32_t) << log_n, &rc);
+    if (!m) return rc;
+    HIP_TRY(launch_twiddles(field, log_n, (uint32_t*)m->ptr, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ctx->twiddles[key] = m;
+    *out = m;
+    return PG_OK;
+}
+
+static constexpr size_t SCALE_CACHE_BYTES = 512ull << 20;
+
+static int get_scale(pg_ctx* ctx, hipStream_t s, int field, int log_h, int b, uint32_t shift,
+                     std::shared_ptr<DevMem>* out) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto key = std::make_tuple(field, log_h, b, shift);
+    auto it = ctx->scales.find(key);
+    if (it != ctx->scales.end()) {
+        *out = it->second;
+        return PG_OK;
+    }
+    const size_t bytes = sizeof(uint32_t) << (log_h + b);
+    while (ctx->scale_bytes + bytes > SCALE_CACHE_BYTES && !ctx->scale_order.empty()) {
+        auto old = ctx->scale_order.front();
+        ctx->scale_order.erase(ctx->scale_order.begin());
+        auto o = ctx->scales.find(old);
+        if (o != ctx->scales.end()) {
+            ctx->scale_bytes -= o->second->bytes;
+            ctx->scales.erase(o);  // freed once the last user drops it
+        }
+    }
+    int rc;
+    auto m = make_devmem(ctx, bytes, &rc);
+    if (!m) return rc;
+    HIP_TRY(launch_coset_scale(field, log_h, b, shift, (uint32_t*)m->ptr, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ctx->scales[key] = m;
+    ctx->scale_order.push_back(key);
+    ctx->scale_bytes += bytes;
+    *out = m;
+    return PG_OK;
+}
+
+// ---------------------------------------------------------------- buffers --
+struct pg_buf {
+    pg_ctx* ctx = nullptr;
+    uint32_t* ptr = nullptr;
+    size_t rows = 0, cols = 0;
+    size_t bytes = 0;
+    bool owned = false;
+};
+
+static int is_pow2(size_t x) { return x && !(x & (x - 1)); }
+static int log2_of(size_t x) {
+    int l = 0;
+    while (((size_t)1 << l) < x) l++;
+    return l;
+}
+
+static int buf_new(pg_ctx* ctx, size_t rows, size_t cols, pg_buf** out) {
+    if (rows && cols > (SIZE_MAX / 4) / rows) return fail(PG_E_INVALID, "matrix too large");
+    std::unique_ptr<pg_buf> b(new pg_buf);
+    b->ctx = ctx;
+    b->rows = rows;
+    b->cols = cols;
+    b->bytes = rows * cols * 4;
+    b->owned = true;
+    void* p;
+    PG_TRY(ctx_alloc(ctx, b->bytes, &p));
+    b->ptr = (uint32_t*)p;
+    *out = b.release();
+    return PG_OK;
+}
+
+extern "C" int pg_buf_alloc(pg_ctx* ctx, size_t rows, size_t cols, pg_buf** out) {
+    if (!ctx || !out) return fail(PG_E_INVALID, "null argument");
+    PG_TRY(bind_device(ctx));
+    return buf_new(ctx, rows, cols, out);
+}
+
+extern "C" int pg_buf_wrap(pg_ctx* ctx, void* dev, size_t rows, size_t cols, pg_buf** out) {
+    if (!ctx || !out || (!dev && rows * cols)) return fail(PG_E_INVALID, "null argument");
+    pg_buf* b = new pg_buf;
+    b->ctx = ctx;
+    b->ptr = (uint32_t*)dev;
+    b->rows = rows;
+    b->cols = cols;
+    b->bytes = rows * cols * 4;
+    b->owned = false;
+    *out = b;
+    return PG_OK;
+}
+
+extern "C" void pg_buf_free(pg_buf* b) {
+    if (!b) return;
+    if (b->owned) {
+        (void)hipSetDevice(b->ctx->device);
+        ctx_free(b->ctx, b->ptr, b->bytes);
+    }
+    delete b;
+}
+
+extern "C" void* pg_buf_device_ptr(const pg_buf* b) { return b ? b->ptr : nullptr; }
+extern "C" size_t pg_buf_rows(const pg_buf* b) { return b ? b->rows : 0; }
+extern "C" size_t pg_buf_cols(const pg_buf* b) { return b ? b->cols : 0; }
+
+static bool field_ok(int f) { return f == PG_KOALABEAR || f == PG_BABYBEAR; }
+
+extern "C" int pg_upload(pg_ctx* ctx, pg_buf* b, const uint32_t* host, int field, int canonical) {
+    if (!ctx || !b || (!host && b->bytes)) return fail(PG_E_INVALID, "null argument");
+    if (canonical && !field_ok(field)) return fail(PG_E_INVALID, "unknown field %d", field);
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    HIP_TRY(hipMemcpyAsync(b->ptr, host, b->bytes, hipMemcpyHostToDevice, l.s));
+    if (canonical) HIP_TRY(launch_convert(field, b->ptr, b->rows * b->cols, 1, l.s));
+    HIP_TRY(hipStreamSynchronize(l.s));
+    return PG_OK;
+}
+
+extern "C" int pg_download(pg_ctx* ctx, const pg_buf* b, size_t row0, size_t nrows, uint32_t* host,
+                           int field, int canonical) {
+    if (!ctx || !b || !host) return fail(PG_E_INVALID, "null argument");
+    if (row0 > b->rows || nrows > b->rows - row0) return fail(PG_E_INVALID, "row range out of bounds");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    const size_t n = nrows * b->cols;
+    if (canonical) {
+        if (!field_ok(field)) return fail(PG_E_INVALID, "unknown field %d", field);
+        void* t;
+        PG_TRY(ctx_alloc(ctx, n * 4, &t));
+        hipError_t e = hipMemcpyAsync(t, b->ptr + row0 * b->cols, n * 4, hipMemcpyDeviceToDevice, l.s);
+        if (e == hipSuccess) e = launch_convert(field, (uint32_t*)t, n, 0, l.s);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, t, n * 4, hipMemcpyDeviceToHost, l.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(l.s);
+        ctx_free(ctx, t, n * 4);
+        HIP_TRY(e);
+        return PG_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(host, b->ptr + row0 * b->cols, n * 4, hipMemcpyDeviceToHost, l.s));
+    HIP_TRY(hipStreamSynchronize(l.s));
+    return PG_OK;
+}
+
+extern "C" int pg_fill_random(pg_ctx* ctx, pg_buf* b, int field, uint64_t seed) {
+    if (!ctx || !b || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    HIP_TRY(launch_fill_random(field, seed, b->ptr, b->rows * b->cols, 1, l.s));
+    HIP_TRY(hipStreamSynchronize(l.s));
+    return PG_OK;
+}
+
+// ------------------------------------------------------------------- LDE --
+static uint32_t gen_monty(int field) {
+    return field == PG_KOALABEAR ? to_monty<KoalaBear>(KoalaBear::GEN) : to_monty<BabyBear>(BabyBear::GEN);
+}
+static int two_adicity(int field) { return field == PG_KOALABEAR ? KoalaBear::TWO_ADICITY : BabyBear::TWO_ADICITY; }
+
+// LDE of `in` into a new N x w buffer on stream s; `scratch` (may be null)
+// is reused when large enough.
+static int lde_on_stream(pg_ctx* ctx, hipStream_t s, int field, const pg_buf* in, uint32_t b,
+                         uint32_t shift, int bitrev, pg_buf** out, std::shared_ptr<DevMem>* scratch) {
+    *out = nullptr;
+    if (!is_pow2(in->rows)) return fail(PG_E_INVALID, "height %zu is not a power of two", in->rows);
+    if (in->cols == 0) return fail(PG_E_INVALID, "width is zero");
+    const int L = log2_of(in->rows);
+    if (L > lde_max_log_h()) return fail(PG_E_UNSUPPORTED, "log height %d > %d", L, lde_max_log_h());
+    if (L + (int)b > two_adicity(field))
+        return fail(PG_E_INVALID, "LDE size 2^%d exceeds the field's two-adicity %d", L + (int)b, two_adicity(field));
+    if (b > 8) return fail(PG_E_UNSUPPORTED, "added_bits %u > 8", b);
+    std::shared_ptr<DevMem> tw, sc;
+    PG_TRY(get_twiddles(ctx, s, field, L, &tw));
+    PG_TRY(get_scale(ctx, s, field, L, (int)b, shift, &sc));
+    const size_t N = in->rows << b;
+    pg_buf* o;
+    PG_TRY(buf_new(ctx, N, in->cols, &o));
+    uint32_t* tmp = nullptr;
+    if (L > 11) {
+        const size_t need = N * in->cols * 4;
+        if (!*scratch || (*scratch)->bytes < need) {
+            scratch->reset();
+            int rc;
+            *scratch = make_devmem(ctx, need, &rc);
+            if (!*scratch) {
+                pg_buf_free(o);
+                return rc;
+            }
+        }
+        tmp = (uint32_t*)(*scratch)->ptr;
+    }
+    auto kt = make_timer(ctx);
+    hipError_t e = launch_coset_lde(field, in->ptr, in->cols, L, (uint32_t)in->cols, (int)b,
+                                    (const uint32_t*)tw->ptr, (const uint32_t*)sc->ptr, tmp, o->ptr, bitrev, s,
+                                    kt.get());
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // tables and scratch stay referenced until done
+    if (e != hipSuccess) {
+        pg_buf_free(o);
+        return fail(PG_E_DEVICE, "coset LDE failed: %s", hipGetErrorString(e));
+    }
+    *out = o;
+    return PG_OK;
+}
+
+extern "C" int pg_coset_lde(pg_ctx* ctx, int field, const pg_buf* evals, uint32_t added_bits,
+                            uint32_t shift_mont, int bitrev, pg_buf** out) {
+    if (!ctx || !evals || !out || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    std::shared_ptr<DevMem> scratch;
+    return lde_on_stream(ctx, l.s, field, evals, added_bits, shift_mont, bitrev, out, &scratch);
+}
+
+// ------------------------------------------------------------------ MMCS --
+struct pg_tree {
+    pg_ctx* ctx = nullptr;
+    int field = 0;
+    size_t H = 0;
+    int log_h = 0;
+    std::vector<pg_buf*> mats;  // original order
+    std::vector<bool> owned;
+    size_t total_width = 0;
+    uint32_t* digests = nullptr;  // layers 0..log_h, (2H - 1) * 8 words
+    size_t digest_bytes = 0;
+    void* meta = nullptr;  // device: MatRef orig[n], shift[n], coloff[n], layer_off[log_h + 1]
+    size_t meta_bytes = 0;
+    MatRef* d_mats = nullptr;
+    uint32_t* d_shift = nullptr;
+    uint32_t* d_coloff = nullptr;
+    uint64_t* d_layer_off = nullptr;
+    uint32_t root[8] = {0};
+};
+
+extern "C" void pg_tree_free(pg_tree* t) {
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    for (size_t k = 0; k < t->mats.size(); k++)
+        if (t->owned[k]) pg_buf_free(t->mats[k]);
+    ctx_free(t->ctx, t->digests, t->digest_bytes);
+    ctx_free(t->ctx, t->meta, t->meta_bytes);
+    delete t;
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const std::vector<pg_buf*>& mats,
+                                 const std::vector<bool>& owned, uint32_t root[8], pg_tree** out) {
+    *out = nullptr;
+    const size_t n = mats.size();
+    if (n == 0) return fail(PG_E_INVALID, "no matrices to commit");
+    size_t H = 0;
+    for (auto* m : mats) {
+        if (!m) return fail(PG_E_INVALID, "null matrix");
+        if (!is_pow2(m->rows)) return fail(PG_E_INVALID, "height %zu is not a power of two", m->rows);
+        H = std::max(H, m->rows);
+    }
+    std::unique_ptr<pg_tree, void (*)(pg_tree*)> t(new pg_tree, [](pg_tree* x) {
+        // on failure: release only what the tree allocated itself
+        for (size_t k = 0; k < x->owned.size(); k++) x->owned[k] = false;
+        pg_tree_free(x);
+    });
+    t->ctx = ctx;
+    t->field = field;
+    t->H = H;
+    t->log_h = log2_of(H);
+    t->mats = mats;
+    t->owned = std::vector<bool>(n, false);
+    // stable sort by height, tallest first
+    std::vector<size_t> order(n);
+    for (size_t k = 0; k < n; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return mats[a]->rows > mats[b]->rows; });
+
+    // device metadata block
+    const int lh = t->log_h;
+    const size_t off_sorted = 0;
+    const size_t off_orig = align_up(off_sorted + n * sizeof(MatRef), 256);
+    const size_t off_shift = align_up(off_orig + n * sizeof(MatRef), 256);
+    const size_t off_col = align_up(off_shift + n * 4, 256);
+    const size_t off_layer = align_up(off_col + n * 4, 256);
+    const size_t meta_bytes = off_layer + (lh + 1) * 8;
+    std::vector<uint8_t> hm(meta_bytes, 0);
+    MatRef* h_sorted = (MatRef*)(hm.data() + off_sorted);
+    MatRef* h_orig = (MatRef*)(hm.data() + off_orig);
+    uint32_t* h_shift = (uint32_t*)(hm.data() + off_shift);
+    uint32_t* h_col = (uint32_t*)(hm.data() + off_col);
+    uint64_t* h_layer = (uint64_t*)(hm.data() + off_layer);
+    size_t col = 0;
+    for (size_t k = 0; k < n; k++) {
+        h_orig[k] = MatRef{mats[k]->ptr, (uint64_t)mats[k]->cols, (uint32_t)mats[k]->cols, 0};
+        h_shift[k] = (uint32_t)(lh - log2_of(mats[k]->rows));
+        h_col[k] = (uint32_t)col;
+        col += mats[k]->cols;
+    }
+    if (col > 0xffffffffull) return fail(PG_E_INVALID, "total width too large");
+    t->total_width = col;
+    for (size_t k = 0; k < n; k++) h_sorted[k] = h_orig[order[k]];
+    {
+        uint64_t off = 0;
+        for (int j = 0; j <= lh; j++) {
+            h_layer[j] = off;
+            off += H >> j;
+        }
+    }
+    PG_TRY(ctx_alloc(ctx, meta_bytes, &t->meta));
+    t->meta_bytes = meta_bytes;
+    uint8_t* dm = (uint8_t*)t->meta;
+    MatRef* d_sorted = (MatRef*)(dm + off_sorted);
+    t->d_mats = (MatRef*)(dm + off_orig);
+    t->d_shift = (uint32_t*)(dm + off_shift);
+    t->d_coloff = (uint32_t*)(dm + off_col);
+    t->d_layer_off = (uint64_t*)(dm + off_layer);
+    HIP_TRY(hipMemcpyAsync(t->meta, hm.data(), meta_bytes, hipMemcpyHostToDevice, s));
+
+    t->digest_bytes = (2 * H - 1) * 32;
+    {
+        void* p;
+        PG_TRY(ctx_alloc(ctx, t->digest_bytes, &p));
+        t->digests = (uint32_t*)p;
+    }
+    // layer 0: every matrix of height H
+    size_t g0 = 0, g1 = 0;
+    uint32_t wsum = 0;
+    while (g1 < n && mats[order[g1]]->rows == H) wsum += (uint32_t)mats[order[g1++]]->cols;
+    const pg_buf* first = mats[order[0]];
+    auto kt = make_timer(ctx);
+    if (g1 - g0 == 1 && first->cols % 8 == 0 && first->cols > 0 && ((uintptr_t)first->ptr % 16) == 0)
+        TIMED(kt, "leaf_hash_w8", s,
+              launch_leaf_hash_w8(field, first->ptr, first->cols, (uint32_t)first->cols, H, t->digests, s));
+    else
+        TIMED(kt, "leaf_hash", s, launch_leaf_hash(field, d_sorted + g0, (int)(g1 - g0), wsum, H, t->digests, s));
+    // compress + inject
+    size_t len = H, off = 0, gi = g1;
+    while (len > 1) {
+        const size_t nl = len / 2;
+        size_t gj = gi;
+        uint32_t iw = 0;
+        while (gj < n && mats[order[gj]]->rows == nl) iw += (uint32_t)mats[order[gj++]]->cols;
+        TIMED(kt, gj > gi ? "compress_inject" : "compress", s,
+              launch_compress(field, t->digests + 8 * off, t->digests + 8 * (off + len), nl,
+                              gj > gi ? d_sorted + gi : nullptr, (int)(gj - gi), iw, s));
+        off += len;
+        len = nl;
+        gi = gj;
+    }
+    HIP_TRY(hipMemcpyAsync(t->root, t->digests + 8 * off, 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    memcpy(root, t->root, 32);
+    t->owned = owned;
+    *out = t.release();
+    return PG_OK;
+}
+
+extern "C" int pg_mmcs_commit(pg_ctx* ctx, int field, pg_buf* const* mats, size_t n, int take,
+                              uint32_t root[8], pg_tree** out) {
+    if (!ctx || !mats || !root || !out || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    std::vector<pg_buf*> v(mats, mats + n);
+    return mmcs_commit_on_stream(ctx, l.s, field, v, std::vector<bool>(n, take != 0), root, out);
+}
+
+static int open_on_stream(pg_ctx* ctx, hipStream_t s, const pg_tree* t, const uint64_t* idx, size_t nq,
+                          uint32_t* rows_out, uint32_t* paths_out) {
+    for (size_t q = 0; q < nq; q++)
+        if (idx[q] >= t->H) return fail(PG_E_INVALID, "query index %llu >= height %zu",
+                                        (unsigned long long)idx[q], t->H);
+    if (nq == 0) return PG_OK;
+    const size_t rw = nq * t->total_width * 4, pw = nq * t->log_h * 32;
+    const size_t o_rows = align_up(nq * 8, 256), o_paths = align_up(o_rows + rw, 256);
+    const size_t bytes = o_paths + pw;
+    void* d;
+    PG_TRY(ctx_alloc(ctx, bytes, &d));
+    uint8_t* dm = (uint8_t*)d;
+    hipError_t e = hipMemcpyAsync(dm, idx, nq * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_open_gather(t->d_mats, t->d_shift, t->d_coloff, (int)t->mats.size(), (uint32_t)t->total_width,
+                               t->digests, t->d_layer_off, t->log_h, (const uint64_t*)dm, (int)nq,
+                               (uint32_t*)(dm + o_rows), (uint32_t*)(dm + o_paths), s);
+    if (e == hipSuccess && rows_out && rw) e = hipMemcpyAsync(rows_out, dm + o_rows, rw, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && paths_out && pw) e = hipMemcpyAsync(paths_out, dm + o_paths, pw, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(ctx, d, bytes);
+    HIP_TRY(e);
+    return PG_OK;
+}
+
+extern "C" int pg_mmcs_open_batch(pg_ctx* ctx, const pg_tree* t, const uint64_t* idx, size_t nq,
+                                  uint32_t* rows_out, uint32_t* paths_out) {
+    if (!ctx || !t || (nq && !idx)) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    return open_on_stream(ctx, l.s, t, idx, nq, rows_out, paths_out);
+}
+
+extern "C" int pg_tree_root(const pg_tree* t, uint32_t root[8]) {
+    if (!t || !root) return fail(PG_E_INVALID, "null argument");
+    memcpy(root, t->root, 32);
+    return PG_OK;
+}
+extern "C" size_t pg_tree_num_matrices(const pg_tree* t) { return t ? t->mats.size() : 0; }
+extern "C" const pg_buf* pg_tree_matrix(const pg_tree* t, size_t k) {
+    return (t && k < t->mats.size()) ? t->mats[k] : nullptr;
+}
+extern "C" int pg_tree_log_height(const pg_tree* t) { return t ? t->log_h : -1; }
+extern "C" size_t pg_tree_total_width(const pg_tree* t) { return t ? t->total_width : 0; }
+
+extern "C" int pg_tree_digest_layer(pg_ctx* ctx, const pg_tree* t, int layer, uint32_t* host) {
+    if (!ctx || !t || !host || layer < 0 || layer > t->log_h) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    size_t off = 0;
+    for (int j = 0; j < layer; j++) off += t->H >> j;
+    HIP_TRY(hipMemcpyAsync(host, t->digests + 8 * off, (t->H >> layer) * 32, hipMemcpyDeviceToHost, l.s));
+    HIP_TRY(hipStreamSynchronize(l.s));
+    return PG_OK;
+}
+
+// host sponge / compress for verify_batch and the challenger
+template <class F>
+static void host_sponge(const uint32_t* in, size_t n, uint32_t out[8]) {
+    uint32_t st[16] = {0};
+    for (size_t o = 0; o < n; o += 8) {
+        const size_t len = std::min<size_t>(8, n - o);
+        memcpy(st, in + o, len * 4);
+        permute<F>(st);
+    }
+    memcpy(out, st, 32);
+}
+template <class F>
+static void host_compress(const uint32_t* l, const uint32_t* r, uint32_t out[8]) {
+    uint32_t st[16];
+    memcpy(st, l, 32);
+    memcpy(st + 8, r, 32);
+    permute<F>(st);
+    memcpy(out, st, 32);
+}
+
+template <class F>
+static int verify_t(const size_t* heights, const size_t* widths, size_t n, const uint32_t root[8],
+                    uint64_t index, const uint32_t* rows, const uint32_t* path) {
+    size_t H = 0;
+    for (size_t k = 0; k < n; k++) {
+        if (!is_pow2(heights[k])) return fail(PG_E_INVALID, "height not a power of two");
+        H = std::max(H, heights[k]);
+    }
+    const int lh = log2_of(H);
+    std::vector<size_t> start(n + 1, 0);
+    for (size_t k = 0; k < n; k++) start[k + 1] = start[k] + widths[k];
+    std::vector<uint32_t> buf;
+    auto gather = [&](size_t hgt) {
+        buf.clear();
+        bool any = false;
+        for (size_t k = 0; k < n; k++)
+            if (heights[k] == hgt) {
+                any = true;
+                buf.insert(buf.end(), rows + start[k], rows + start[k + 1]);
+            }
+        return any;
+    };
+    uint32_t cur[8], t[8];
+    gather(H);
+    host_sponge<F>(buf.data(), buf.size(), cur);
+    size_t hc = H;
+    for (int j = 0; j < lh; j++) {
+        if ((index >> j) & 1)
+            host_compress<F>(path + 8 * j, cur, t);
+        else
+            host_compress<F>(cur, path + 8 * j, t);
+        memcpy(cur, t, 32);
+        hc >>= 1;
+        if (gather(hc)) {
+            uint32_t rh[8];
+            host_sponge<F>(buf.data(), buf.size(), rh);
+            host_compress<F>(cur, rh, t);
+            memcpy(cur, t, 32);
+        }
+    }
+    return memcmp(cur, root, 32) == 0 ? 1 : 0;
+}
+
+extern "C" int pg_mmcs_verify_batch(int field, const size_t* heights, const size_t* widths, size_t n,
+                                    const uint32_t root[8], uint64_t index, const uint32_t* rows,
+                                    const uint32_t* path) {
+    if (!heights || !widths || !root || n == 0 || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    return field == PG_KOALABEAR ? verify_t<KoalaBear>(heights, widths, n, root, index, rows, path)
+                                 : verify_t<BabyBear>(heights, widths, n, root, index, rows, path);
+}
+
+// ------------------------------------------------------------------- PCS --
+extern "C" int pg_pcs_commit(pg_ctx* ctx, int field, const pg_buf* const* evals, const uint32_t* dshift,
+                             size_t n, uint32_t log_blowup, uint32_t root[8], pg_tree** out) {
+    if (!ctx || !evals || !root || !out || !field_ok(field) || n == 0) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    std::vector<pg_buf*> ldes;
+    std::shared_ptr<DevMem> scratch;
+    auto cleanup = [&]() {
+        for (auto* b : ldes) pg_buf_free(b);
+    };
+    const uint32_t g = gen_monty(field);
+    for (size_t k = 0; k < n; k++) {
+        if (!evals[k]) {
+            cleanup();
+            return fail(PG_E_INVALID, "null matrix %zu", k);
+        }
+        uint32_t shift = g;
+        if (dshift) {
+            if (dshift[k] == 0) {
+                cleanup();
+                return fail(PG_E_INVALID, "domain shift is zero");
+            }
+            shift = field == PG_KOALABEAR ? fmul<KoalaBear>(g, finv<KoalaBear>(dshift[k]))
+                                          : fmul<BabyBear>(g, finv<BabyBear>(dshift[k]));
+        }
+        pg_buf* o;
+        int rc = lde_on_stream(ctx, l.s, field, evals[k], log_blowup, shift, 1, &o, &scratch);
+        if (rc != PG_OK) {
+            cleanup();
+            return rc;
+        }
+        ldes.push_back(o);
+    }
+    scratch.reset();
+    int rc = mmcs_commit_on_stream(ctx, l.s, field, ldes, std::vector<bool>(n, true), root, out);
+    if (rc != PG_OK) cleanup();
+    return rc;
+}
+
+extern "C" int pg_pcs_evaluations(pg_ctx* ctx, const pg_tree* t, size_t k, size_t rows, uint32_t* host) {
+    if (!ctx || !t || !host || k >= t->mats.size()) return fail(PG_E_INVALID, "bad argument");
+    const pg_buf* m = t->mats[k];
+    if (!is_pow2(rows) || rows > m->rows) return fail(PG_E_INVALID, "domain size %zu invalid", rows);
+    std::vector<uint32_t> tmp(rows * m->cols);
+    PG_TRY(pg_download(ctx, m, 0, rows, tmp.data(), t->field, 0));
+    const int q = log2_of(rows);
+    for (size_t j = 0; j < rows; j++)
+        memcpy(host + j * m->cols, tmp.data() + (size_t)bit_rev((uint32_t)j, q) * m->cols, m->cols * 4);
+    return PG_OK;
+}
+
+// ------------------------------------------------------------ challenger --
+template <class F>
+static void ch_duplex(pg_challenger* c) {
+    for (int i = 0; i < c->n_input; i++) c->state[i] = c->input[i];
+    c->n_input = 0;
+    permute<F>(c->state);
+    memcpy(c->output, c->state, 32);
+    c->n_output = 8;
+}
+static void duplex(pg_challenger* c) {
+    if (c->field == PG_KOALABEAR)
+        ch_duplex<KoalaBear>(c);
+    else
+        ch_duplex<BabyBear>(c);
+}
+extern "C" void pg_challenger_init(pg_challenger* c, int field) {
+    memset(c, 0, sizeof *c);
+    c->field = field;
+}
+extern "C" void pg_challenger_observe(pg_challenger* c, uint32_t v) {
+    c->n_output = 0;
+    c->input[c->n_input++] = v;
+    if (c->n_input == 8) duplex(c);
+}
+extern "C" uint32_t pg_challenger_sample(pg_challenger* c) {
+    if (c->n_input > 0 || c->n_output == 0) duplex(c);
+    return c->output[--c->n_output];
+}
+extern "C" void pg_challenger_sample_ext(pg_challenger* c, uint32_t out[4]) {
+    for (int k = 0; k < 4; k++) out[k] = pg_challenger_sample(c);
+}
+extern "C" uint32_t pg_challenger_sample_bits(pg_challenger* c, int bits) {
+    const uint32_t v = pg_challenger_sample(c);
+    const uint32_t canon = c->field == PG_KOALABEAR ? from_monty<KoalaBear>(v) : from_monty<BabyBear>(v);
+    return bits >= 32 ? canon : (canon & ((1u << bits) - 1u));
+}
+extern "C" int pg_challenger_check_witness(pg_challenger* c, int bits, uint32_t w) {
+    const uint32_t wm = c->field == PG_KOALABEAR ? to_monty<KoalaBear>(w) : to_monty<BabyBear>(w);
+    pg_challenger_observe(c, wm);
+    return pg_challenger_sample_bits(c, bits) == 0;
+}
+
+// ------------------------------------------------------------------- FRI --
+extern "C" int pg_fri_fold(pg_ctx* ctx, int field, const pg_buf* v, const uint32_t beta[4], pg_buf** out) {
+    if (!ctx || !v || !beta || !out || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    if (v->cols != 4 || v->rows < 2 || !is_pow2(v->rows)) return fail(PG_E_INVALID, "v must be n x 4, n a power of two >= 2");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    const int ln = log2_of(v->rows);
+    if (ln > two_adicity(field)) return fail(PG_E_INVALID, "vector too long");
+    std::shared_ptr<DevMem> tw;
+    PG_TRY(get_twiddles(ctx, l.s, field, ln, &tw));
+    pg_buf* o;
+    PG_TRY(buf_new(ctx, v->rows / 2, 4, &o));
+    hipError_t e = launch_fri_fold(field, v->ptr, v->rows / 2, beta, (const uint32_t*)tw->ptr, o->ptr, l.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(l.s);
+    if (e != hipSuccess) {
+        pg_buf_free(o);
+        return fail(PG_E_DEVICE, "fold failed: %s", hipGetErrorString(e));
+    }
+    *out = o;
+    return PG_OK;
+}
+
+struct pg_fri {
+    pg_ctx* ctx = nullptr;
+    int field = 0;
+    int log_max = 0;
+    std::vector<pg_tree*> trees;  // round r: (2^(log_max - r - 1)) x 8 leaves, owning its vector
+};
+
+extern "C" void pg_fri_free(pg_fri* f) {
+    if (!f) return;
+    for (auto* t : f->trees) pg_tree_free(t);
+    delete f;
+}
+extern "C" int pg_fri_num_rounds(const pg_fri* f) { return f ? (int)f->trees.size() : 0; }
+extern "C" const pg_tree* pg_fri_round_tree(const pg_fri* f, int r) {
+    return (f && r >= 0 && r < (int)f->trees.size()) ? f->trees[r] : nullptr;
+}
+
+extern "C" int pg_fri_commit_phase(pg_ctx* ctx, int field, pg_challenger* ch, const pg_buf* const* inputs,
+                                   int max_log, int log_stop, int final_mode, uint32_t* roots_out,
+                                   uint32_t* betas_out, uint32_t* final_out, int* rounds_out, pg_fri** out) {
+    if (!ctx || !ch || !inputs || !out || !field_ok(field) || ch->field != field || max_log < 0 ||
+        max_log > two_adicity(field) || log_stop < 0)
+        return fail(PG_E_INVALID, "bad argument");
+    int L0 = -1;
+    for (int k = max_log; k >= 0; k--)
+        if (inputs[k]) {
+            L0 = k;
+            break;
+        }
+    if (L0 < 0) return fail(PG_E_INVALID, "no input vectors");
+    for (int k = 0; k <= max_log; k++)
+        if (inputs[k] && (inputs[k]->rows != ((size_t)1 << k) || inputs[k]->cols != 4))
+            return fail(PG_E_INVALID, "input %d must be 2^%d x 4", k, k);
+    if (log_stop > L0) return fail(PG_E_INVALID, "log_stop > input length");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    std::unique_ptr<pg_fri, void (*)(pg_fri*)> f(new pg_fri, pg_fri_free);
+    f->ctx = ctx;
+    f->field = field;
+    f->log_max = L0;
+    pg_buf* cur;
+    PG_TRY(buf_new(ctx, (size_t)1 << L0, 4, &cur));
+    HIP_TRY(hipMemcpyAsync(cur->ptr, inputs[L0]->ptr, cur->bytes, hipMemcpyDeviceToDevice, l.s));
+    int r = 0;
+    int len_log = L0;
+    while (len_log > log_stop) {
+        // commit the current vector as (n/2) x 8 leaves; the tree owns it
+        pg_buf* leaves = new pg_buf(*cur);
+        leaves->rows = cur->rows / 2;
+        leaves->cols = 8;
+        cur->owned = false;  // ownership moves to `leaves`
+        pg_buf_free(cur);
+        pg_tree* t;
+        int rc = mmcs_commit_on_stream(ctx, l.s, field, {leaves}, {true}, roots_out + 8 * r, &t);
+        if (rc != PG_OK) {
+            pg_buf_free(leaves);
+            return rc;
+        }
+        f->trees.push_back(t);
+        for (int k = 0; k < 8; k++) pg_challenger_observe(ch, roots_out[8 * r + k]);
+        uint32_t beta[4];
+        pg_challenger_sample_ext(ch, beta);
+        if (betas_out) memcpy(betas_out + 4 * r, beta, 16);
+        std::shared_ptr<DevMem> tw;
+        PG_TRY(get_twiddles(ctx, l.s, field, len_log, &tw));
+        PG_TRY(buf_new(ctx, (size_t)1 << (len_log - 1), 4, &cur));
+        auto kt = make_timer(ctx);
+        TIMED(kt, "fri_fold", l.s,
+              launch_fri_fold(field, leaves->ptr, cur->rows, beta, (const uint32_t*)tw->ptr, cur->ptr, l.s));
+        len_log--;
+        if (inputs[len_log]) HIP_TRY(launch_field_add(field, cur->ptr, inputs[len_log]->ptr, cur->rows * 4, l.s));
+        HIP_TRY(hipStreamSynchronize(l.s));
+        r++;
+    }
+    const size_t nf = (size_t)1 << len_log;
+    std::vector<uint32_t> fin(nf * 4);
+    hipError_t e = hipMemcpyAsync(fin.data(), cur->ptr, nf * 16, hipMemcpyDeviceToHost, l.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(l.s);
+    pg_buf_free(cur);
+    HIP_TRY(e);
+    if (final_out) memcpy(final_out, fin.data(), nf * 16);
+    if (final_mode == 0) {
+        for (size_t i = 1; i < nf; i++)
+            if (memcmp(&fin[4 * i], &fin[0], 16))
+                return fail(PG_E_INVALID, "final FRI vector is not constant (input not low degree)");
+        for (int k = 0; k < 4; k++) pg_challenger_observe(ch, fin[k]);
+    } else {
+        for (size_t i = 0; i < nf * 4; i++) pg_challenger_observe(ch, fin[i]);
+    }
+    if (rounds_out) *rounds_out = r;
+    *out = f.release();
+    return PG_OK;
+}
+
+extern "C" int pg_fri_grind(pg_ctx* ctx, int field, pg_challenger* ch, int bits, uint32_t* witness_out) {
+    if (!ctx || !ch || !witness_out || !field_ok(field) || ch->field != field || bits < 0 || bits > 30)
+        return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    const uint32_t P = field == PG_KOALABEAR ? KoalaBear::P : BabyBear::P;
+    void* d;
+    PG_TRY(ctx_alloc(ctx, 256, &d));
+    uint32_t* dstate = (uint32_t*)d;       // 16 words
+    uint32_t* din = dstate + 16;           // 8 words
+    uint32_t* dres = dstate + 32;          // 1 word
+    uint32_t hres = 0xffffffffu;
+    hipError_t e = hipMemcpyAsync(dstate, ch->state, 64, hipMemcpyHostToDevice, l.s);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, ch->input, 32, hipMemcpyHostToDevice, l.s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dres, &hres, 4, hipMemcpyHostToDevice, l.s);
+    const uint32_t chunk = 1u << 22;
+    for (uint32_t start = 0; e == hipSuccess && start < P; start += chunk) {
+        const uint32_t count = std::min<uint32_t>(chunk, P - start);
+        e = launch_pow_search(field, dstate, din, ch->n_input, bits, start, count, dres, l.s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&hres, dres, 4, hipMemcpyDeviceToHost, l.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(l.s);
+        if (hres != 0xffffffffu) break;
+    }
+    ctx_free(ctx, d, 256);
+    HIP_TRY(e);
+    if (hres == 0xffffffffu) return fail(PG_E_DEVICE, "no proof-of-work witness found");
+    if (!pg_challenger_check_witness(ch, bits, hres))
+        return fail(PG_E_DEVICE, "device witness %u fails the host check", hres);
+    *witness_out = hres;
+    return PG_OK;
+}
+
+extern "C" int pg_fri_answer_queries(pg_ctx* ctx, const pg_fri* f, const uint64_t* idx, size_t nq,
+                                     uint32_t* siblings_out, uint32_t* paths_out) {
+    if (!ctx || !f || (nq && !idx)) return fail(PG_E_INVALID, "bad argument");
+    for (size_t q = 0; q < nq; q++)
+        if (idx[q] >> f->log_max) return fail(PG_E_INVALID, "query index out of range");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    const int R = (int)f->trees.size();
+    size_t path_words_per_q = 0;
+    for (int r = 0; r < R; r++) path_words_per_q += 8 * (size_t)f->trees[r]->log_h;
+    std::vector<uint64_t> leaf(nq);
+    std::vector<uint32_t> rows, paths;
+    size_t path_off = 0;
+    for (int r = 0; r < R; r++) {
+        const pg_tree* t = f->trees[r];
+        for (size_t q = 0; q < nq; q++) leaf[q] = (idx[q] >> r) >> 1;
+        rows.assign(nq * 8, 0);
+        paths.assign(nq * t->log_h * 8, 0);
+        PG_TRY(open_on_stream(ctx, l.s, t, leaf.data(), nq, rows.data(), paths.data()));
+        for (size_t q = 0; q < nq; q++) {
+            const bool odd = (idx[q] >> r) & 1;
+            if (siblings_out) memcpy(siblings_out + (q * R + r) * 4, rows.data() + q * 8 + (odd ? 0 : 4), 16);
+            if (paths_out)
+                memcpy(paths_out + q * path_words_per_q + path_off, paths.data() + q * t->log_h * 8,
+                       t->log_h * 32);
+        }
+        path_off += 8 * (size_t)t->log_h;
+    }
+    return PG_OK;
+}

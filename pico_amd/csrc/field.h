@@ -1,0 +1,178 @@
+// field.h -- KoalaBear / BabyBear arithmetic in Montgomery form (R = 2^32)
+// for gfx950 kernels and for the host-side challenger.
+//
+// In-memory representation follows p3 MontyField31 (x stored as x*2^32 mod p),
+// so device buffers can alias the reference's RowMajorMatrix<KoalaBear> data
+// without conversion.  Primes: KoalaBear p = 2^31 - 2^24 + 1
+// (gnark/koalabear/koalabear.go:20), BabyBear p = 2^31 - 2^27 + 1.
+// Extension: F[x]/(x^4 - W), W = 3 (KoalaBear, vm/src/primitives/consts.rs:134),
+// W = 11 (BabyBear, consts.rs:107).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PG_HD __host__ __device__ __forceinline__
+
+namespace pg {
+
+constexpr uint32_t inv_mod_2_32(uint32_t p) {
+    uint32_t x = p;  // Newton iteration: x <- x (2 - p x), doubling correct bits
+    for (int i = 0; i < 5; i++) x *= 2u - p * x;
+    return x;
+}
+
+template <uint32_t P_, int TWO_ADICITY_, uint32_t GEN_, uint32_t W_, int RP_, int SBOX_>
+struct MontyField {
+    static constexpr uint32_t P = P_;
+    static constexpr uint32_t MU = 0u - inv_mod_2_32(P_);  // -p^-1 mod 2^32
+    static constexpr uint32_t R1 = (uint32_t)((1ull << 32) % P_);  // Montgomery one
+    static constexpr uint32_t R2 = (uint32_t)(((uint64_t)R1 * R1) % P_);
+    static constexpr int TWO_ADICITY = TWO_ADICITY_;
+    static constexpr uint32_t GEN = GEN_;       // canonical multiplicative generator
+    static constexpr uint32_t W = W_;           // extension non-residue (canonical)
+    static constexpr int ROUNDS_P = RP_;        // Poseidon2 partial rounds
+    static constexpr int SBOX = SBOX_;          // Poseidon2 S-box degree
+};
+
+// KoalaBear: 8 full + 20 partial rounds, x^3 (consts.rs:191-203).
+using KoalaBear = MontyField<0x7f000001u, 24, 3u, 3u, 20, 3>;
+// BabyBear: 8 full + 13 partial rounds, x^7 (consts.rs:176-186).
+using BabyBear = MontyField<0x78000001u, 27, 31u, 11u, 13, 7>;
+
+enum FieldId : int { FIELD_KOALABEAR = 0, FIELD_BABYBEAR = 1 };
+
+// ---- base field --------------------------------------------------------
+// Inputs in [0, p).  p < 2^31 so a + b never overflows 32 bits; the unsigned
+// min picks the reduced value (r - p wraps above r when r < p).
+template <class F>
+PG_HD uint32_t fadd(uint32_t a, uint32_t b) {
+    uint32_t r = a + b, t = r - F::P;
+    return t < r ? t : r;
+}
+template <class F>
+PG_HD uint32_t fsub(uint32_t a, uint32_t b) {
+    uint32_t r = a - b, t = r + F::P;
+    return t < r ? t : r;
+}
+template <class F>
+PG_HD uint32_t fneg(uint32_t a) {
+    return a == 0 ? 0 : F::P - a;
+}
+// Montgomery product a*b*2^-32 mod p.  t = a*b < p^2; m = t*(-p^-1) mod 2^32;
+// (t + m p) / 2^32 < 2p fits 32 bits after one conditional subtraction.
+template <class F>
+PG_HD uint32_t fmul(uint32_t a, uint32_t b) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * F::MU;
+    uint32_t u = (uint32_t)((t + (uint64_t)m * F::P) >> 32);
+    uint32_t s = u - F::P;
+    return s < u ? s : u;
+}
+template <class F>
+PG_HD uint32_t fhalve(uint32_t a) {  // a / 2: works on any representation
+    return (a & 1u) ? (uint32_t)(((uint64_t)a + F::P) >> 1) : (a >> 1);
+}
+template <class F>
+PG_HD uint32_t to_monty(uint32_t a) {
+    return fmul<F>(a, F::R2);
+}
+template <class F>
+PG_HD uint32_t from_monty(uint32_t a) {
+    return fmul<F>(a, 1u);
+}
+template <class F>
+PG_HD uint32_t fpow(uint32_t a, uint64_t e) {  // Montgomery in, Montgomery out
+    uint32_t r = F::R1;
+    while (e) {
+        if (e & 1) r = fmul<F>(r, a);
+        a = fmul<F>(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+template <class F>
+PG_HD uint32_t finv(uint32_t a) {
+    return fpow<F>(a, F::P - 2);
+}
+// canonical-valued constants at compile time
+constexpr uint32_t cpow_canon(uint32_t p, uint32_t a, uint64_t e) {
+    uint64_t r = 1, x = a % p;
+    while (e) {
+        if (e & 1) r = (r * x) % p;
+        x = (x * x) % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+template <class F>
+constexpr uint32_t cmonty(uint32_t canon) {
+    return (uint32_t)(((uint64_t)(canon % F::P) << 32) % F::P);
+}
+// two_adic_generator(k) = GEN^((p-1) >> k), canonical
+template <class F>
+constexpr uint32_t two_adic_gen_canon(int log_n) {
+    return cpow_canon(F::P, F::GEN, (uint64_t)(F::P - 1) >> log_n);
+}
+
+// ---- degree-4 extension (Montgomery coefficients c0..c3) -----------------
+struct Ext4 {
+    uint32_t c[4];
+};
+template <class F>
+PG_HD Ext4 eadd(const Ext4& a, const Ext4& b) {
+    Ext4 r;
+    for (int i = 0; i < 4; i++) r.c[i] = fadd<F>(a.c[i], b.c[i]);
+    return r;
+}
+template <class F>
+PG_HD Ext4 esub(const Ext4& a, const Ext4& b) {
+    Ext4 r;
+    for (int i = 0; i < 4; i++) r.c[i] = fsub<F>(a.c[i], b.c[i]);
+    return r;
+}
+template <class F>
+PG_HD Ext4 escale(const Ext4& a, uint32_t s) {
+    Ext4 r;
+    for (int i = 0; i < 4; i++) r.c[i] = fmul<F>(a.c[i], s);
+    return r;
+}
+// Schoolbook product with x^4 = W.  W*y for W = 3 or 11 is done by additions
+// (linear, so it is representation independent).
+template <class F>
+PG_HD uint32_t mul_by_w(uint32_t y) {
+    uint32_t y2 = fadd<F>(y, y);
+    if (F::W == 3) return fadd<F>(y2, y);
+    uint32_t y4 = fadd<F>(y2, y2), y8 = fadd<F>(y4, y4);
+    return fadd<F>(fadd<F>(y8, y2), y);  // 11 = 8 + 2 + 1
+}
+template <class F>
+PG_HD Ext4 emul(const Ext4& a, const Ext4& b) {
+    const uint32_t* x = a.c;
+    const uint32_t* y = b.c;
+    uint32_t c0 = fmul<F>(x[0], y[0]);
+    uint32_t c1 = fadd<F>(fmul<F>(x[0], y[1]), fmul<F>(x[1], y[0]));
+    uint32_t c2 = fadd<F>(fadd<F>(fmul<F>(x[0], y[2]), fmul<F>(x[1], y[1])), fmul<F>(x[2], y[0]));
+    uint32_t c3 = fadd<F>(fadd<F>(fmul<F>(x[0], y[3]), fmul<F>(x[1], y[2])),
+                          fadd<F>(fmul<F>(x[2], y[1]), fmul<F>(x[3], y[0])));
+    uint32_t c4 = fadd<F>(fadd<F>(fmul<F>(x[1], y[3]), fmul<F>(x[2], y[2])), fmul<F>(x[3], y[1]));
+    uint32_t c5 = fadd<F>(fmul<F>(x[2], y[3]), fmul<F>(x[3], y[2]));
+    uint32_t c6 = fmul<F>(x[3], y[3]);
+    Ext4 r;
+    r.c[0] = fadd<F>(c0, mul_by_w<F>(c4));
+    r.c[1] = fadd<F>(c1, mul_by_w<F>(c5));
+    r.c[2] = fadd<F>(c2, mul_by_w<F>(c6));
+    r.c[3] = c3;
+    return r;
+}
+
+// reverse the low `bits` bits of x (host and device; kernels use __brev)
+PG_HD uint32_t bit_rev(uint32_t x, int bits) {
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0f0f0f0fu) | ((x & 0x0f0f0f0fu) << 4);
+    x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
+    x = (x >> 16) | (x << 16);
+    return bits == 0 ? 0u : x >> (32 - bits);
+}
+
+}  // namespace pg

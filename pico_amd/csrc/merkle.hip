@@ -1,0 +1,181 @@
+// merkle.hip -- Poseidon2 Merkle kernels for the mixed-height MMCS:
+// leaf hashing (sponge over concatenated same-height rows), compress/inject
+// layers and the opening gather.
+//
+// Semantics (p3 MerkleTreeMmcs, pinned in-tree by the inverse verify_batch at
+// vm/src/compiler/recursion/circuit/fri.rs:389-436):
+//   layer0[i] = H(concat of row i of every matrix of the tallest height)
+//   next[i]   = C(prev[2i], prev[2i+1]); if matrices of height len(next)
+//               exist: next[i] = C(next[i], H(concat of their row i))
+//   H = PaddingFreeSponge<16, 8, 8> (overwrite mode, circuit/hash.rs:40-52)
+//   C = TruncatedPermutation (circuit/hash.rs:89-94)
+// One thread owns one permutation state for the whole row / node.
+#include "kernels.h"
+#include "poseidon2.h"
+
+namespace pg {
+
+// Absorb the concatenation of row `row` of mats[0..n) into st (which the
+// caller zeroes).  The cursor (k, j) is wave-uniform, so the matrix
+// descriptors are read through the scalar cache.
+template <class F>
+__device__ __forceinline__ void absorb_rows(uint32_t (&st)[16], const MatRef* mats, int n,
+                                            uint32_t total, uint64_t row) {
+    if (total == 0) return;
+    int k = 0;
+    uint32_t j = 0;
+    const uint32_t* base = mats[0].data + row * mats[0].stride;
+    uint32_t wk = mats[0].width;
+    for (uint32_t off = 0; off < total; off += 8) {
+        uint32_t cnt = total - off < 8u ? total - off : 8u;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            if ((uint32_t)t < cnt) {
+                while (j >= wk && k + 1 < n) {
+                    k++;
+                    j = 0;
+                    base = mats[k].data + row * mats[k].stride;
+                    wk = mats[k].width;
+                }
+                st[t] = base[j];
+                j++;
+            }
+        }
+        permute<F>(st);
+    }
+}
+
+// General leaf hash: any number of matrices, any widths.
+template <class F>
+__global__ void __launch_bounds__(256) leaf_hash_kernel(const MatRef* mats, int n, uint32_t total,
+                                                        uint64_t rows, uint32_t* digests) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    uint32_t st[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) st[t] = 0;
+    absorb_rows<F>(st, mats, n, total, i);
+    uint4* o = reinterpret_cast<uint4*>(digests + 8 * i);
+    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
+    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+// One matrix whose width is a multiple of 8 and whose rows are 32-byte
+// aligned: every sponge chunk is two 16-byte loads.
+template <class F>
+__global__ void __launch_bounds__(256) leaf_hash_w8_kernel(const uint32_t* data, uint64_t stride,
+                                                           uint32_t chunks, uint64_t rows,
+                                                           uint32_t* digests) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const uint4* rp = reinterpret_cast<const uint4*>(data + i * stride);
+    uint32_t st[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) st[t] = 0;
+    for (uint32_t c = 0; c < chunks; c++) {
+        uint4 a = rp[2 * c], b = rp[2 * c + 1];
+        st[0] = a.x; st[1] = a.y; st[2] = a.z; st[3] = a.w;
+        st[4] = b.x; st[5] = b.y; st[6] = b.z; st[7] = b.w;
+        permute<F>(st);
+    }
+    uint4* o = reinterpret_cast<uint4*>(digests + 8 * i);
+    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
+    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) compress_kernel(const uint32_t* prev, uint32_t* next,
+                                                       uint64_t n_out, const MatRef* inj,
+                                                       int n_inj, uint32_t inj_width) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const uint4* p = reinterpret_cast<const uint4*>(prev + 16 * i);
+    uint32_t st[16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint4 v = p[q];
+        st[4 * q] = v.x; st[4 * q + 1] = v.y; st[4 * q + 2] = v.z; st[4 * q + 3] = v.w;
+    }
+    permute<F>(st);
+    if (n_inj > 0) {
+        uint32_t h[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) h[t] = 0;
+        absorb_rows<F>(h, inj, n_inj, inj_width, i);
+#pragma unroll
+        for (int t = 0; t < 8; t++) st[8 + t] = h[t];
+        permute<F>(st);
+    }
+    uint4* o = reinterpret_cast<uint4*>(next + 8 * i);
+    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
+    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+__global__ void open_gather_kernel(const MatRef* mats, const uint32_t* shift,
+                                   const uint32_t* col_off, int n_mats, uint32_t total_width,
+                                   const uint32_t* digests, const uint64_t* layer_off, int log_h,
+                                   const uint64_t* idx, uint32_t* rows_out, uint32_t* paths_out) {
+    const int q = blockIdx.x;
+    const uint64_t index = idx[q];
+    for (int k = 0; k < n_mats; k++) {
+        const uint64_t r = index >> shift[k];
+        const MatRef m = mats[k];
+        uint32_t* dst = rows_out + (uint64_t)q * total_width + col_off[k];
+        for (uint32_t c = threadIdx.x; c < m.width; c += blockDim.x) dst[c] = m.data[r * m.stride + c];
+    }
+    for (int t = threadIdx.x; t < log_h * 8; t += blockDim.x) {
+        const int j = t >> 3, e = t & 7;
+        const uint64_t sib = (index >> j) ^ 1u;
+        paths_out[((uint64_t)q * log_h + j) * 8 + e] = digests[(layer_off[j] + sib) * 8 + e];
+    }
+}
+
+static unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+hipError_t launch_leaf_hash(int field, const MatRef* mats, int n, uint32_t total_width,
+                            size_t rows, uint32_t* digests, hipStream_t stream) {
+    if (rows == 0) return hipSuccess;
+    const unsigned g = grid_for(rows, 256);
+    if (field == FIELD_KOALABEAR)
+        leaf_hash_kernel<KoalaBear><<<g, 256, 0, stream>>>(mats, n, total_width, rows, digests);
+    else
+        leaf_hash_kernel<BabyBear><<<g, 256, 0, stream>>>(mats, n, total_width, rows, digests);
+    return hipGetLastError();
+}
+
+// Fast path entry used by the host when a height group is one matrix with
+// width % 8 == 0 and 32-byte aligned rows (declared here, not in the shared
+// header, because only api.hip's commit planner chooses it).
+hipError_t launch_leaf_hash_w8(int field, const uint32_t* data, uint64_t stride, uint32_t width,
+                               size_t rows, uint32_t* digests, hipStream_t stream) {
+    if (rows == 0) return hipSuccess;
+    const unsigned g = grid_for(rows, 256);
+    if (field == FIELD_KOALABEAR)
+        leaf_hash_w8_kernel<KoalaBear><<<g, 256, 0, stream>>>(data, stride, width / 8, rows, digests);
+    else
+        leaf_hash_w8_kernel<BabyBear><<<g, 256, 0, stream>>>(data, stride, width / 8, rows, digests);
+    return hipGetLastError();
+}
+
+hipError_t launch_compress(int field, const uint32_t* prev, uint32_t* next, size_t n_out,
+                           const MatRef* inj, int n_inj, uint32_t inj_width, hipStream_t stream) {
+    if (n_out == 0) return hipSuccess;
+    const unsigned g = grid_for(n_out, 256);
+    if (field == FIELD_KOALABEAR)
+        compress_kernel<KoalaBear><<<g, 256, 0, stream>>>(prev, next, n_out, inj, n_inj, inj_width);
+    else
+        compress_kernel<BabyBear><<<g, 256, 0, stream>>>(prev, next, n_out, inj, n_inj, inj_width);
+    return hipGetLastError();
+}
+
+hipError_t launch_open_gather(const MatRef* mats, const uint32_t* shift, const uint32_t* col_off,
+                              int n_mats, uint32_t total_width, const uint32_t* digests,
+                              const uint64_t* layer_off, int log_h, const uint64_t* idx, int nq,
+                              uint32_t* rows_out, uint32_t* paths_out, hipStream_t stream) {
+    if (nq == 0) return hipSuccess;
+    open_gather_kernel<<<nq, 128, 0, stream>>>(mats, shift, col_off, n_mats, total_width, digests,
+                                               layer_off, log_h, idx, rows_out, paths_out);
+    return hipGetLastError();
+}
+
+}  // namespace pg

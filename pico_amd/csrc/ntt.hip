@@ -1,0 +1,425 @@
+// ntt.hip -- batched coset LDE of row-major trace matrices on gfx950.
+//
+// Computes what p3 Radix2DitParallel::coset_lde_batch + bit_reverse_rows
+// produce for TwoAdicFriPcs::commit: for an h x w matrix E of evaluations on
+// <w_h>, output row i (of N = h * 2^b) is f(shift * w_N^{rev_N(i)}) where f is
+// the column interpolant (row order pinned by the verifier's
+// x = g * w^{rev(index)}, vm/src/compiler/recursion/circuit/fri.rs:151-171).
+// The result is unique, so any correct algorithm is bit-exact.
+//
+// Decomposition.  Committed block c (rows [c h, (c+1) h)) is the evaluation of
+// f on the coset shift_c <w_h>, shift_c = shift * w_N^{rev_b(c)}, in
+// bit-reversed order.  So the LDE is one size-h inverse DFT per column followed
+// by 2^b size-h forward DFTs of the coefficient vector scaled by shift_c^k / h.
+// A size-h transform (L = log h) is split four-step style into h = n_hi * n_lo
+// (n_hi = 2^k1, n_lo = 2^k2):
+//   pass A  (strided):  inverse DFTs of size n_hi over rows {j_lo + n_lo m},
+//                       times w_h^{-j_lo k_hi}                 in  -> out[0, h)
+//   pass BC (blocks):   inverse DFTs of size n_lo over contiguous row blocks,
+//                       then per coset: scale, forward DFTs of size n_lo,
+//                       times w_h^{j_lo k_hi}                  out -> tmp (N rows)
+//   pass D  (strided):  forward DFTs of size n_hi over rows {c h + j_lo + n_lo m},
+//                       written to bit-reversed rows            tmp -> out
+// With L <= 11 the whole column fits one tile and a single pass does it all.
+// Inverse transforms are decimation-in-frequency (natural -> bit-reversed),
+// forward ones decimation-in-time (bit-reversed -> natural), so the
+// coefficient vectors never need an explicit permutation.
+//
+// In-tile transform of size S = E * P: each thread owns E elements of one
+// column in VGPRs; P threads share a column.  Stage 1 is a size-E register
+// DFT, then a twiddle, one exchange through LDS, and stage 2 is E/P register
+// DFTs of size P.  Lanes map to adjacent columns so every global access is a
+// run of 16 consecutive words of one row.
+#include "kernels.h"
+#include "field.h"
+
+namespace pg {
+
+// ---- small roots, compile time ------------------------------------------
+template <class F>
+struct Roots64 {
+    uint32_t f[64];  // w_64^j   (Montgomery)
+    uint32_t i[64];  // w_64^-j
+};
+template <class F>
+constexpr Roots64<F> make_roots64() {
+    Roots64<F> r{};
+    const uint32_t w = two_adic_gen_canon<F>(6);
+    const uint32_t wi = cpow_canon(F::P, w, F::P - 2);
+    for (int j = 0; j < 64; j++) {
+        r.f[j] = cmonty<F>(cpow_canon(F::P, w, j));
+        r.i[j] = cmonty<F>(cpow_canon(F::P, wi, j));
+    }
+    return r;
+}
+template <class F>
+struct R64 {
+    static constexpr Roots64<F> K = make_roots64<F>();
+};
+
+// w_{2^(s+1)}^j, i.e. the twiddle of DFT level s (pair distance 2^s)
+template <class F, bool INV>
+__device__ __forceinline__ uint32_t level_root(int s, int j) {
+    return INV ? R64<F>::K.i[j << (5 - s)] : R64<F>::K.f[j << (5 - s)];
+}
+
+// In-register DIF of size 2^LOG_N: natural in, bit-reversed out.
+template <class F, int LOG_N, bool INV>
+__device__ __forceinline__ void reg_dif(uint32_t* x) {
+    constexpr int N = 1 << LOG_N;
+#pragma unroll
+    for (int s = LOG_N - 1; s >= 0; s--) {
+        const int half = 1 << s;
+#pragma unroll
+        for (int blk = 0; blk < N; blk += 2 * half)
+#pragma unroll
+            for (int j = 0; j < half; j++) {
+                const uint32_t a = x[blk + j], b = x[blk + j + half];
+                x[blk + j] = fadd<F>(a, b);
+                const uint32_t d = fsub<F>(a, b);
+                x[blk + j + half] = j == 0 ? d : fmul<F>(d, level_root<F, INV>(s, j));
+            }
+    }
+}
+
+// In-register DIT of size 2^LOG_N: bit-reversed in, natural out.
+template <class F, int LOG_N, bool INV>
+__device__ __forceinline__ void reg_dit(uint32_t* x) {
+    constexpr int N = 1 << LOG_N;
+#pragma unroll
+    for (int s = 0; s < LOG_N; s++) {
+        const int half = 1 << s;
+#pragma unroll
+        for (int blk = 0; blk < N; blk += 2 * half)
+#pragma unroll
+            for (int j = 0; j < half; j++) {
+                const uint32_t a = x[blk + j];
+                const uint32_t b = j == 0 ? x[blk + j + half]
+                                          : fmul<F>(x[blk + j + half], level_root<F, INV>(s, j));
+                x[blk + j] = fadd<F>(a, b);
+                x[blk + j + half] = fsub<F>(a, b);
+            }
+    }
+}
+
+constexpr int crev(int x, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t rev_dev(uint32_t x, int bits) {
+    return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
+}
+
+// w_h^{+-e} from the table of w_h^e, e < h (mask = h - 1)
+template <bool INV>
+__device__ __forceinline__ uint32_t tw_at(const uint32_t* __restrict__ tw, uint32_t e, uint32_t mask) {
+    return tw[INV ? ((0u - e) & mask) : e];
+}
+
+// Tile DIF.  In: x[m] = tile position p + P m.  Out: x[t] = tile position
+// p E + t, holding DFT output index rev_S(p E + t).
+template <class F, int LOG_E, int LOG_P, int C, bool INV>
+__device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
+                                         const uint32_t* __restrict__ tw, int tw_sh, uint32_t mask) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    reg_dif<F, LOG_E, INV>(x);
+    if constexpr (LOG_P > 0) {
+#pragma unroll
+        for (int m = 1; m < E; m++) {
+            const uint32_t e = ((uint32_t)p * (uint32_t)crev(m, LOG_E)) << tw_sh;
+            x[m] = fmul<F>(x[m], tw_at<INV>(tw, e, mask));
+        }
+#pragma unroll
+        for (int m = 0; m < E; m++) lds[(p + P * m) * C + c] = x[m];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < E / P; u++)
+#pragma unroll
+            for (int i = 0; i < P; i++) x[u * P + i] = lds[(P * (p * (E / P) + u) + i) * C + c];
+#pragma unroll
+        for (int u = 0; u < E / P; u++) reg_dif<F, LOG_P, INV>(&x[u * P]);
+        __syncthreads();
+    }
+}
+
+// Tile DIT.  In: x[t] = tile position p E + t (bit-reversed input order).
+// Out: x[u P + a] = natural output index (p E/P + u) + E a.
+template <class F, int LOG_E, int LOG_P, int C, bool INV>
+__device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
+                                         const uint32_t* __restrict__ tw, int tw_sh, uint32_t mask) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    reg_dit<F, LOG_E, INV>(x);
+    if constexpr (LOG_P > 0) {
+        const uint32_t rp = rev_dev((uint32_t)p, LOG_P);
+#pragma unroll
+        for (int j = 1; j < E; j++) x[j] = fmul<F>(x[j], tw_at<INV>(tw, ((uint32_t)j * rp) << tw_sh, mask));
+#pragma unroll
+        for (int j = 0; j < E; j++) lds[(E * p + j) * C + c] = x[j];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < E / P; u++)
+#pragma unroll
+            for (int a = 0; a < P; a++) x[u * P + a] = lds[(p * (E / P) + u + E * a) * C + c];
+#pragma unroll
+        for (int u = 0; u < E / P; u++) reg_dit<F, LOG_P, INV>(&x[u * P]);
+        __syncthreads();
+    }
+}
+
+template <int LOG_E, int LOG_P>
+__device__ __forceinline__ uint32_t dit_out_index(int p, int r) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    return (uint32_t)(p * (E / P) + r / P + E * (r % P));
+}
+
+struct LdeArgs {
+    const uint32_t* in;   // h x w input, row stride ld_in
+    uint64_t ld_in;
+    uint32_t* out;        // N x w output (also pass-A scratch)
+    uint32_t* tmp;        // N x w scratch (multi-pass only)
+    const uint32_t* tw;   // w_h^e, e < h
+    const uint32_t* scale;// (shift_c)^k / h, [c h + k]
+    uint64_t ntiles;
+    uint32_t w;
+    int L, k1, k2, b, bitrev;
+};
+
+enum { KIND_SINGLE = 0, KIND_A = 1, KIND_BC = 2, KIND_D = 3 };
+
+constexpr int NTT_C = 16;  // columns per tile
+constexpr int threads_for(int log_p) { return NTT_C * (1 << log_p) >= 256 ? NTT_C * (1 << log_p) : 256; }
+constexpr int groups_for(int log_p) { return threads_for(log_p) / (NTT_C * (1 << log_p)); }
+
+__device__ __forceinline__ uint64_t out_row(const LdeArgs& a, uint32_t cs, uint32_t natural) {
+    // committed (bit-reversed) or natural row of coset block cs, natural index j
+    return a.bitrev ? (uint64_t)cs * (1ull << a.L) + rev_dev(natural, a.L)
+                    : (uint64_t)rev_dev(cs, a.b) + ((uint64_t)natural << a.b);
+}
+
+template <class F, int LOG_E, int LOG_P, int KIND>
+__global__ void __launch_bounds__(threads_for(LOG_P)) lde_kernel(LdeArgs a) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P, C = NTT_C, G = groups_for(LOG_P);
+    __shared__ uint32_t lds_all[LOG_P > 0 ? G * S * C : 1];
+    const int tid = threadIdx.x;
+    const int c = tid % C, p = (tid / C) % P, g = tid / (C * P);
+    uint32_t* lds = lds_all + (LOG_P > 0 ? g * S * C : 0);
+    const uint64_t tile = (uint64_t)blockIdx.x * G + g;
+    const bool valid = tile < a.ntiles;
+    const uint32_t h = 1u << a.L, mask = h - 1;
+    const uint32_t w = a.w;
+    uint32_t x[E];
+
+    if constexpr (KIND == KIND_SINGLE) {
+        const uint32_t col = (uint32_t)tile * C + c;
+        const bool act = valid && col < w;
+#pragma unroll
+        for (int m = 0; m < E; m++) x[m] = act ? a.in[(uint64_t)(p + P * m) * a.ld_in + col] : 0u;
+        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, 0, mask);
+        const uint32_t ncos = 1u << a.b;
+        for (uint32_t cs = 0; cs < ncos; cs++) {
+            uint32_t y[E];
+#pragma unroll
+            for (int t = 0; t < E; t++)
+                y[t] = fmul<F>(x[t], a.scale[(uint64_t)cs * h + rev_dev(p * E + t, a.L)]);
+            tile_dit<F, LOG_E, LOG_P, C, false>(y, lds, p, c, a.tw, 0, mask);
+            if (act) {
+#pragma unroll
+                for (int r = 0; r < E; r++)
+                    a.out[out_row(a, cs, dit_out_index<LOG_E, LOG_P>(p, r)) * w + col] = y[r];
+            }
+        }
+    } else if constexpr (KIND == KIND_A) {
+        const uint32_t n_lo = 1u << a.k2;
+        const uint32_t j_lo = (uint32_t)(tile % n_lo);
+        const uint32_t col = (uint32_t)(tile / n_lo) * C + c;
+        const bool act = valid && col < w;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            x[m] = act ? a.in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + col] : 0u;
+        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, a.k2, mask);
+        if (act) {
+#pragma unroll
+            for (int t = 0; t < E; t++) {
+                const uint32_t pos = (uint32_t)(p * E + t);
+                const uint32_t e = j_lo * rev_dev(pos, a.k1);
+                a.out[(uint64_t)(j_lo + n_lo * pos) * w + col] = fmul<F>(x[t], tw_at<true>(a.tw, e, mask));
+            }
+        }
+    } else if constexpr (KIND == KIND_BC) {
+        const uint32_t n_hi = 1u << a.k1, n_lo = 1u << a.k2;
+        const uint32_t blk = (uint32_t)(tile % n_hi);
+        const uint32_t col = (uint32_t)(tile / n_hi) * C + c;
+        const bool act = valid && col < w;
+        const uint32_t* src = a.out;  // pass A result
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            x[m] = act ? src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * w + col] : 0u;
+        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, a.k1, mask);
+        const uint32_t rblk = rev_dev(blk, a.k1);
+        const uint32_t ncos = 1u << a.b;
+        for (uint32_t cs = 0; cs < ncos; cs++) {
+            uint32_t y[E];
+#pragma unroll
+            for (int t = 0; t < E; t++)
+                y[t] = fmul<F>(x[t], a.scale[(uint64_t)cs * h + rev_dev(n_lo * blk + p * E + t, a.L)]);
+            tile_dit<F, LOG_E, LOG_P, C, false>(y, lds, p, c, a.tw, a.k1, mask);
+            if (act) {
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    const uint32_t j = dit_out_index<LOG_E, LOG_P>(p, r);
+                    a.tmp[((uint64_t)cs * h + n_lo * blk + j) * w + col] =
+                        fmul<F>(y[r], a.tw[j * rblk]);
+                }
+            }
+        }
+    } else {  // KIND_D
+        const uint32_t n_lo = 1u << a.k2;
+        const uint32_t ncos = 1u << a.b;
+        const uint32_t j_lo = (uint32_t)(tile % n_lo);
+        const uint64_t rest = tile / n_lo;
+        const uint32_t cs = (uint32_t)(rest % ncos);
+        const uint32_t col = (uint32_t)(rest / ncos) * C + c;
+        const bool act = valid && col < w;
+#pragma unroll
+        for (int t = 0; t < E; t++)
+            x[t] = act ? a.tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + col] : 0u;
+        tile_dit<F, LOG_E, LOG_P, C, false>(x, lds, p, c, a.tw, a.k2, mask);
+        if (act) {
+#pragma unroll
+            for (int r = 0; r < E; r++) {
+                const uint32_t natural = j_lo + n_lo * dit_out_index<LOG_E, LOG_P>(p, r);
+                a.out[out_row(a, cs, natural) * w + col] = x[r];
+            }
+        }
+    }
+}
+
+// ---- tables ----------------------------------------------------------------
+template <class F>
+__global__ void twiddle_kernel(uint32_t* tw, uint32_t n, uint32_t root) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) tw[e] = fpow<F>(root, e);
+}
+
+template <class F>
+__global__ void coset_scale_kernel(uint32_t* scale, int log_h, int b, uint32_t shift,
+                                   uint32_t root_n, uint32_t h_inv) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = 1ull << (log_h + b);
+    if (i >= n) return;
+    const uint32_t cs = (uint32_t)(i >> log_h), k = (uint32_t)(i & ((1ull << log_h) - 1));
+    const uint32_t shift_c = fmul<F>(shift, fpow<F>(root_n, rev_dev(cs, b)));
+    scale[i] = fmul<F>(fpow<F>(shift_c, k), h_inv);
+}
+
+template <class F>
+static uint32_t root_monty(int log_n) {
+    return to_monty<F>(two_adic_gen_canon<F>(log_n));
+}
+
+hipError_t launch_twiddles(int field, int log_n, uint32_t* tw, hipStream_t stream) {
+    const uint32_t n = 1u << log_n;
+    const unsigned g = (n + 255) / 256;
+    if (field == FIELD_KOALABEAR)
+        twiddle_kernel<KoalaBear><<<g, 256, 0, stream>>>(tw, n, root_monty<KoalaBear>(log_n));
+    else
+        twiddle_kernel<BabyBear><<<g, 256, 0, stream>>>(tw, n, root_monty<BabyBear>(log_n));
+    return hipGetLastError();
+}
+
+template <class F>
+static hipError_t coset_scale_t(int log_h, int b, uint32_t shift, uint32_t* scale, hipStream_t s) {
+    const uint64_t n = 1ull << (log_h + b);
+    const uint32_t h_inv = finv<F>(to_monty<F>((uint32_t)((1ull << log_h) % F::P)));
+    coset_scale_kernel<F><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(scale, log_h, b, shift,
+                                                                       root_monty<F>(log_h + b), h_inv);
+    return hipGetLastError();
+}
+
+hipError_t launch_coset_scale(int field, int log_h, int added_bits, uint32_t shift, uint32_t* scale,
+                              hipStream_t stream) {
+    return field == FIELD_KOALABEAR ? coset_scale_t<KoalaBear>(log_h, added_bits, shift, scale, stream)
+                                    : coset_scale_t<BabyBear>(log_h, added_bits, shift, scale, stream);
+}
+
+// ---- dispatch --------------------------------------------------------------
+int lde_max_log_h() { return 22; }
+
+static const char* const KIND_NAMES[4] = {"lde_single", "lde_pass_a", "lde_pass_bc", "lde_pass_d"};
+
+template <class F, int LOG_E, int LOG_P, int KIND>
+static void run(const LdeArgs& a, hipStream_t s, KernelTimer* kt) {
+    constexpr int G = groups_for(LOG_P);
+    const unsigned grid = (unsigned)((a.ntiles + G - 1) / G);
+    if (kt) kt->begin(KIND_NAMES[KIND], s);
+    lde_kernel<F, LOG_E, LOG_P, KIND><<<grid, threads_for(LOG_P), 0, s>>>(a);
+    if (kt) kt->end(s);
+}
+
+// tile shapes (LOG_E, LOG_P) per log S, with E >= P
+template <class F, int KIND>
+static bool run_logs(int log_s, const LdeArgs& a, hipStream_t s, KernelTimer* kt) {
+    switch (log_s) {
+        case 0: if constexpr (KIND == KIND_SINGLE) { run<F, 0, 0, KIND>(a, s, kt); return true; } break;
+        case 1: if constexpr (KIND == KIND_SINGLE) { run<F, 1, 0, KIND>(a, s, kt); return true; } break;
+        case 2: if constexpr (KIND == KIND_SINGLE) { run<F, 2, 0, KIND>(a, s, kt); return true; } break;
+        case 3: if constexpr (KIND == KIND_SINGLE) { run<F, 3, 0, KIND>(a, s, kt); return true; } break;
+        case 4: if constexpr (KIND == KIND_SINGLE) { run<F, 4, 0, KIND>(a, s, kt); return true; } break;
+        case 5: if constexpr (KIND == KIND_SINGLE) { run<F, 5, 0, KIND>(a, s, kt); return true; } break;
+        case 6: run<F, 3, 3, KIND>(a, s, kt); return true;
+        case 7: run<F, 4, 3, KIND>(a, s, kt); return true;
+        case 8: run<F, 4, 4, KIND>(a, s, kt); return true;
+        case 9: run<F, 5, 4, KIND>(a, s, kt); return true;
+        case 10: run<F, 5, 5, KIND>(a, s, kt); return true;
+        case 11: run<F, 6, 5, KIND>(a, s, kt); return true;
+        default: break;
+    }
+    return false;
+}
+
+template <class F>
+static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t w, int b,
+                              const uint32_t* tw, const uint32_t* scale, uint32_t* tmp, uint32_t* out,
+                              int bitrev, hipStream_t s, KernelTimer* kt) {
+    LdeArgs a{};
+    a.in = in;
+    a.ld_in = ld_in;
+    a.out = out;
+    a.tmp = tmp;
+    a.tw = tw;
+    a.scale = scale;
+    a.w = w;
+    a.L = L;
+    a.b = b;
+    a.bitrev = bitrev;
+    const uint64_t col_blocks = (w + NTT_C - 1) / NTT_C;
+    if (L <= 11) {
+        a.k1 = 0;
+        a.k2 = L;
+        a.ntiles = col_blocks;
+        if (!run_logs<F, KIND_SINGLE>(L, a, s, kt)) return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    a.k1 = (L + 1) / 2;
+    a.k2 = L - a.k1;
+    a.ntiles = col_blocks << a.k2;
+    if (!run_logs<F, KIND_A>(a.k1, a, s, kt)) return hipErrorInvalidValue;
+    a.ntiles = col_blocks << a.k1;
+    if (!run_logs<F, KIND_BC>(a.k2, a, s, kt)) return hipErrorInvalidValue;
+    a.ntiles = (col_blocks << a.k2) << b;
+    if (!run_logs<F, KIND_D>(a.k1, a, s, kt)) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_coset_lde(int field, const uint32_t* in, size_t ld_in, int log_h, uint32_t w,
+                            int added_bits, const uint32_t* tw, const uint32_t* scale, uint32_t* tmp,
+                            uint32_t* out, int bit_reverse, hipStream_t stream, KernelTimer* kt) {
+    if (log_h < 0 || log_h > lde_max_log_h() || w == 0) return hipErrorInvalidValue;
+    return field == FIELD_KOALABEAR
+               ? coset_lde_t<KoalaBear>(in, ld_in, log_h, w, added_bits, tw, scale, tmp, out, bit_reverse, stream, kt)
+               : coset_lde_t<BabyBear>(in, ld_in, log_h, w, added_bits, tw, scale, tmp, out, bit_reverse, stream, kt);
+}
+
+}  // namespace pg

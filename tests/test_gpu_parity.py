@@ -1,0 +1,258 @@
+"""GPU parity: every device result through the C ABI equals the oracle,
+bit for bit, on seeded inputs (LDE buffers, digest layers, roots, openings,
+FRI commitments, betas, final vectors, query answers)."""
+import threading
+
+import numpy as np
+import pytest
+
+import pico_amd
+from pico_amd import BABYBEAR, KOALABEAR
+
+pytestmark = pytest.mark.gpu
+FIELDS = [KOALABEAR, BABYBEAR]
+
+
+def up(ctx, f, a):
+    return ctx.upload(a, field=f)
+
+
+# ------------------------------------------------------------------ inputs --
+@pytest.mark.parametrize("f", FIELDS)
+def test_fill_random_matches_oracle(ctx, oracle, f):
+    m = ctx.random_matrix(f, 0xABCDEF, 1000, 7)
+    assert np.array_equal(m.to_host(f), oracle.random_matrix(f, 0xABCDEF, 1000, 7))
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_upload_download_roundtrip(ctx, oracle, f):
+    a = oracle.random_matrix(f, 3, 33, 5)
+    m = up(ctx, f, a)
+    assert np.array_equal(m.to_host(f), a)
+    assert np.array_equal(m.to_host(), pico_amd.to_monty(f, a))
+    assert np.array_equal(m.to_host(f, row0=7, nrows=4), a[7:11])
+
+
+# --------------------------------------------------------------------- LDE --
+LDE_CASES = [(L, w, b) for L in range(0, 12) for (w, b) in [(1, 1), (5, 2), (16, 1), (33, 3)]]
+LDE_CASES += [(L, w, b) for L in range(12, 17) for (w, b) in [(3, 1), (17, 2), (40, 1)]]
+LDE_CASES += [(18, 2, 3), (20, 3, 1), (22, 1, 1)]
+
+
+@pytest.mark.parametrize("L,w,b", LDE_CASES)
+def test_coset_lde_matches_oracle(ctx, oracle, L, w, b):
+    for f in FIELDS:
+        a = oracle.random_matrix(f, 1000 * L + w, 1 << L, w)
+        m = up(ctx, f, a)
+        dft = pico_amd.Radix2DitParallel(ctx, f)
+        g = pico_amd.GENERATORS[f]
+        out = dft.coset_lde_batch(m, b, g)
+        assert np.array_equal(out.to_host(f), oracle.coset_lde(f, a, b, g)), (f, L, w, b)
+        if L in (3, 9, 13):  # natural order and another shift
+            s = 7
+            out = dft.coset_lde_batch(m, b, s, bit_reversed=False)
+            assert np.array_equal(out.to_host(f), oracle.coset_lde(f, a, b, s, bit_reverse=False))
+
+
+def test_lde_rejects_bad_shapes(ctx):
+    dft = pico_amd.Radix2DitParallel(ctx, KOALABEAR)
+    with pytest.raises(pico_amd.PgError):
+        dft.coset_lde_batch(ctx.matrix(3, 4), 1, 3)  # height not a power of two
+    with pytest.raises(pico_amd.PgError):
+        dft.coset_lde_batch(ctx.matrix(1 << 20, 1), 5, 3)  # 2^25 > two-adicity 24
+
+
+# -------------------------------------------------------------------- MMCS --
+MMCS_SHAPES = [
+    [(8, 1), (8, 7), (4, 8), (2, 9)],
+    [(1024, 8)],
+    [(1024, 64)],
+    [(512, 300)],
+    [(1, 5)],
+    [(64, 3), (256, 0), (256, 16), (1, 4), (64, 5)],
+    [(2048, 16), (1024, 8), (1024, 1), (16, 3), (2, 2)],
+]
+
+
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("shape", MMCS_SHAPES, ids=lambda s: "x".join(f"{h}:{w}" for h, w in s))
+def test_mmcs_commit_open_matches_oracle(ctx, oracle, f, shape):
+    hosts = [oracle.random_matrix(f, 500 + i, h, w) for i, (h, w) in enumerate(shape)]
+    mats = [up(ctx, f, a) for a in hosts]
+    mmcs = pico_amd.MerkleTreeMmcs(ctx, f)
+    root, pd = mmcs.commit(mats)
+    ot = oracle.MerkleTree(f, hosts)
+    assert np.array_equal(root, ot.root)
+    for j in range(pd.log_height + 1):
+        assert np.array_equal(pd.digest_layer(j), ot.layer(j)), j
+    H = ot.max_height
+    idx = np.arange(H) if H <= 64 else np.random.default_rng(1).integers(0, H, 64)
+    rows, paths = mmcs.open_many(idx, pd)
+    dims = [(w, h) for h, w in shape]
+    for q, i in enumerate(idx):
+        orows, opath = ot.open(int(i))
+        assert np.array_equal(rows[q], np.concatenate(orows) if orows else rows[q][:0])
+        assert np.array_equal(paths[q], opath)
+        per = mmcs.open_batch(int(i), pd)
+        assert oracle.mmcs_verify(f, [h for h, _ in shape], [w for _, w in shape], ot.root, int(i),
+                                  per[0], per[1])
+        assert mmcs.verify_batch(root, dims, int(i), per[0], per[1])
+    assert [m.cols for m in mmcs.get_matrices(pd)] == [w for _, w in shape]
+
+
+def test_mmcs_rejects_non_power_of_two(ctx):
+    with pytest.raises(pico_amd.PgError):
+        pico_amd.MerkleTreeMmcs(ctx, KOALABEAR).commit([ctx.matrix(6, 8)])
+
+
+# --------------------------------------------------------------------- PCS --
+def oracle_pcs_commit(oracle, f, hosts, shifts, log_blowup):
+    p = pico_amd.PRIMES[f]
+    g = pico_amd.GENERATORS[f]
+    ldes = []
+    for a, s in zip(hosts, shifts):
+        shift = g * pow(s, p - 2, p) % p
+        ldes.append(oracle.coset_lde(f, a, log_blowup, shift))
+    return ldes, oracle.MerkleTree(f, ldes)
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_pcs_commit_config1(ctx, oracle, f):
+    """The CPU-runnable config: one 2^16 x 64 matrix, log_blowup 1, shift =
+    GENERATOR; LDE buffer and root bit-exact."""
+    a = oracle.random_matrix(f, (1 << 32) | 0, 1 << 16, 64)
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    dom = pcs.natural_domain_for_degree(1 << 16)
+    root, pd = pcs.commit([(dom, up(ctx, f, a))])
+    ldes, ot = oracle_pcs_commit(oracle, f, [a], [1], 1)
+    lde = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd)[0]
+    assert np.array_equal(lde.to_host(f), ldes[0])
+    assert np.array_equal(root, ot.root)
+    # get_evaluations_on_domain: first |Q| committed rows in natural order,
+    # i.e. the evaluations on g<w_Q>
+    q = 1 << 10
+    ev = pcs.get_evaluations_on_domain(pd, 0, pico_amd.TwoAdicMultiplicativeCoset(10, pico_amd.GENERATORS[f]))
+    nat = oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f], bit_reverse=False)
+    assert np.array_equal(ev, nat[::(2 << 16) // q][:q])
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_pcs_commit_mixed_heights_and_shifts(ctx, oracle, f):
+    p = pico_amd.PRIMES[f]
+    shape = [(1 << 12, 20), (1 << 10, 8), (1 << 12, 3), (1 << 6, 9), (1 << 13, 4)]
+    hosts = [oracle.random_matrix(f, 900 + i, h, w) for i, (h, w) in enumerate(shape)]
+    # trace domains (shift 1) and quotient-chunk-like domains (shift g * w^i)
+    shifts = [1, pico_amd.GENERATORS[f] * pico_amd.two_adic_generator(f, 12) % p, 1, 5, 1]
+    for lb in (1, 3):
+        pcs = pico_amd.TwoAdicFriPcs(ctx, f, lb)
+        doms = [pico_amd.TwoAdicMultiplicativeCoset(h.bit_length() - 1, s) for (h, _), s in zip(shape, shifts)]
+        root, pd = pcs.commit([(d, up(ctx, f, a)) for d, a in zip(doms, hosts)])
+        ldes, ot = oracle_pcs_commit(oracle, f, hosts, shifts, lb)
+        assert np.array_equal(root, ot.root), lb
+        got = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd)
+        for g_, e in zip(got, ldes):
+            assert np.array_equal(g_.to_host(f), e)
+
+
+def test_same_job_alone_and_concurrent(ctx, oracle):
+    """Thread safety: identical results when jobs run concurrently."""
+    f = KOALABEAR
+    jobs = [oracle.random_matrix(f, 70 + i, 1 << (10 + i % 4), 8 + 8 * i) for i in range(6)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+
+    def run(a):
+        root, pd = pcs.commit([(pcs.natural_domain_for_degree(a.shape[0]), up(ctx, f, a))])
+        pd.free()
+        return root
+
+    alone = [run(a) for a in jobs]
+    res = [None] * len(jobs)
+
+    def worker(i):
+        res[i] = run(jobs[i])
+
+    ts = [threading.Thread(target=worker, args=(i,)) for i in range(len(jobs))]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    for a, b in zip(alone, res):
+        assert np.array_equal(a, b)
+
+
+def test_admission_budget(oracle):
+    c = pico_amd.Context(0, hbm_budget=1 << 20)
+    try:
+        with pytest.raises(pico_amd.AdmissionError):
+            c.matrix(1 << 20, 4)
+        m = c.matrix(1024, 4)
+        assert c.bytes_in_use >= 1024 * 16
+        m.free()
+    finally:
+        c.close()
+
+
+# --------------------------------------------------------------------- FRI --
+@pytest.mark.parametrize("f", FIELDS)
+def test_fri_fold_matches_oracle(ctx, oracle, f):
+    for ln in (1, 2, 5, 12):
+        v = oracle.random_matrix(f, 60 + ln, 1 << ln, 4)
+        beta = oracle.random_matrix(f, 61, 1, 4)[0]
+        got = pico_amd.fri_fold(ctx, f, up(ctx, f, v), beta)
+        assert np.array_equal(got.to_host(f), oracle.fri_fold(f, v, beta))
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_fri_commit_phase_queries_and_pow(ctx, oracle, f):
+    log_n, log_stop = 12, 3
+    v = oracle.random_matrix(f, 80, 1 << log_n, 4)
+    extra = {9: oracle.random_matrix(f, 81, 1 << 9, 4), 5: oracle.random_matrix(f, 82, 1 << 5, 4)}
+    ch = pico_amd.DuplexChallenger(f)
+    och = oracle.Challenger(f)
+    for c in (ch, och):
+        c.observe([11, 22, 33, 44, 55])
+    inputs = {log_n: up(ctx, f, v)}
+    inputs.update({k: up(ctx, f, a) for k, a in extra.items()})
+    fd = pico_amd.fri_commit_phase(ctx, f, ch, inputs, log_stop, final_mode=1)
+    roots, layers, final, betas = oracle.fri_commit_phase(och, v, log_stop, inputs=extra)
+    assert np.array_equal(fd.commits, roots)
+    assert np.array_equal(fd.betas, betas)
+    assert np.array_equal(fd.final_poly, final)
+    och.observe(final.ravel())
+    # PoW: the device finds the smallest valid witness
+    bits = 8
+    w = pico_amd.fri_grind(ctx, f, ch, bits)
+    for cand in range(w):
+        assert not och.clone().check_witness(bits, cand)
+    assert och.check_witness(bits, w)
+    # query indices from the same transcript, answered from device data
+    idx = [och.sample_bits(log_n) for _ in range(20)]
+    assert [ch.sample_bits(log_n) for _ in range(20)] == idx
+    sib, paths = pico_amd.fri_answer_queries(fd, idx, log_n)
+    for q, i in enumerate(idx):
+        for r in range(log_n - log_stop):
+            ir = i >> r
+            assert np.array_equal(sib[q, r], layers[r][ir ^ 1])
+            leaf = layers[r].reshape(-1, 8)
+            pair = leaf[ir >> 1]
+            assert oracle.mmcs_verify(f, [leaf.shape[0]], [8], roots[r], ir >> 1, [pair], paths[q][r])
+
+
+def test_fri_reference_final_mode(ctx, oracle):
+    """final_mode 0 (reference): folding a low-degree vector to 2^log_blowup
+    leaves a constant, of which one element is observed."""
+    f = KOALABEAR
+    lb, ln = 1, 10
+    cols = [oracle.coset_lde(f, oracle.random_matrix(f, 90 + k, 1 << (ln - lb), 1), lb, 1)[:, 0]
+            for k in range(4)]
+    v = np.stack(cols, axis=1)
+    ch = pico_amd.DuplexChallenger(f)
+    fd = pico_amd.fri_commit_phase(ctx, f, ch, {ln: up(ctx, f, v)}, lb, final_mode=0)
+    assert np.all(fd.final_poly == fd.final_poly[0])
+    och = oracle.Challenger(f)
+    roots, layers, final, betas = oracle.fri_commit_phase(och, v, lb)
+    och.observe(final[0])
+    assert np.array_equal(fd.commits, roots)
+    assert ch.sample() == och.sample()
+    # a random (not low-degree) vector is rejected in reference mode
+    with pytest.raises(pico_amd.PgError):
+        pico_amd.fri_commit_phase(ctx, f, pico_amd.DuplexChallenger(f),
+                                  {ln: up(ctx, f, oracle.random_matrix(f, 99, 1 << ln, 4))}, lb, 0)

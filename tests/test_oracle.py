@@ -1,0 +1,290 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+Each check is independent of the oracle's own code: the reference-held
+Poseidon2 KATs, Python big-int arithmetic, the polynomial definition of the
+LDE, the verify_batch walk, the fold line formula, closed forms (shift = 1
+LDE reproduces its input; folding a low-degree vector stays low degree) and
+planted mistakes that must be caught.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import pyref
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELDS = [0, 1]
+
+
+def kats():
+    with open(os.path.join(HERE, "golden", "poseidon2_kat.json")) as f:
+        return json.load(f)
+
+
+# ----------------------------------------------------------------- field --
+@pytest.mark.parametrize("field,name", [(0, "koalabear"), (1, "babybear")])
+def test_poseidon2_kat(oracle, field, name):
+    k = kats()[name]
+    assert list(oracle.poseidon2(field, k["input"])) == k["output"]
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_field_ops_vs_bigint(oracle, f):
+    p = pyref.P[f]
+    rng = np.random.default_rng(1)
+    L = oracle.lib()
+    for a, b in rng.integers(0, p, size=(200, 2)):
+        a, b = int(a), int(b)
+        assert L.or_add(f, a, b) == (a + b) % p
+        assert L.or_sub(f, a, b) == (a - b) % p
+        assert L.or_mul(f, a, b) == a * b % p
+        if a:
+            assert L.or_inv(f, a) * a % p == 1
+        assert L.or_to_monty(f, a) == (a << 32) % p
+        assert L.or_from_monty(f, (a << 32) % p) == a
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_generators(oracle, f):
+    p = pyref.P[f]
+    g = oracle.generator(f)
+    # primitive root: g^((p-1)/q) != 1 for every prime q | p - 1
+    n, q, primes = p - 1, 2, set()
+    while q * q <= n:
+        while n % q == 0:
+            primes.add(q)
+            n //= q
+        q += 1
+    if n > 1:
+        primes.add(n)
+    assert all(pow(g, (p - 1) // q, p) != 1 for q in primes)
+    adic = oracle.lib().or_two_adicity(f)
+    assert (p - 1) % (1 << adic) == 0 and ((p - 1) >> adic) % 2 == 1
+    for k in range(adic + 1):
+        w = oracle.two_adic_generator(f, k)
+        assert pow(w, 1 << k, p) == 1
+        if k:
+            assert pow(w, 1 << (k - 1), p) == p - 1  # order exactly 2^k
+            assert w * w % p == oracle.two_adic_generator(f, k - 1)
+
+
+def test_koalabear_generator_chain_head(oracle):
+    # the p3 KoalaBear table starts 0x1, 0x7f000000, 0x7e010002, 0x6832fe4a and
+    # ends 0x6ac49f88 = 3^127 (generator chain of two_adic_generator)
+    assert [oracle.two_adic_generator(0, k) for k in range(4)] == [1, 0x7F000000, 0x7E010002, 0x6832FE4A]
+    assert oracle.two_adic_generator(0, 24) == 0x6AC49F88 == pow(3, 127, pyref.P[0])
+    assert oracle.two_adic_generator(1, 27) == 0x1A427A41 == pow(31, 15, pyref.P[1])
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_ext4_mul_inv(oracle, f):
+    p = pyref.P[f]
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        a = [int(x) for x in rng.integers(0, p, 4)]
+        b = [int(x) for x in rng.integers(0, p, 4)]
+        assert list(oracle.ext_mul(f, a, b)) == pyref.ext_mul(f, a, b)
+    a = [int(x) for x in rng.integers(1, p, 4)]
+    assert pyref.ext_inv_by_search_check(f, a, [int(x) for x in oracle.ext_inv(f, a)])
+    # x^4 = W
+    x = [0, 1, 0, 0]
+    x2 = pyref.ext_mul(f, x, x)
+    assert pyref.ext_mul(f, x2, x2) == [pyref.W[f], 0, 0, 0]
+
+
+# --------------------------------------------------------- sponge/compress --
+@pytest.mark.parametrize("f", FIELDS)
+def test_sponge_semantics(oracle, f):
+    p = pyref.P[f]
+    rng = np.random.default_rng(3)
+    assert list(oracle.sponge(f, [])) == [0] * 8
+    for n in [1, 7, 8, 9, 16, 17, 300]:
+        v = [int(x) for x in rng.integers(0, p, n)]
+        st = [0] * 16
+        for off in range(0, n, 8):  # overwrite mode, not add
+            chunk = v[off:off + 8]
+            st[:len(chunk)] = chunk
+            st = [int(x) for x in oracle.poseidon2(f, st)]
+        assert list(oracle.sponge(f, v)) == st[:8]
+    l, r = rng.integers(0, p, 8), rng.integers(0, p, 8)
+    assert list(oracle.compress(f, l, r)) == list(oracle.poseidon2(f, np.concatenate([l, r]))[:8])
+
+
+# --------------------------------------------------------------------- LDE --
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("log_h,w,b", [(0, 2, 1), (1, 3, 1), (2, 1, 2), (3, 3, 3), (4, 2, 1), (5, 1, 2)])
+def test_lde_vs_polynomial_definition(oracle, f, log_h, w, b):
+    p = pyref.P[f]
+    h = 1 << log_h
+    m = oracle.random_matrix(f, 77 + log_h, h, w)
+    for shift in [oracle.generator(f), 5, pow(oracle.two_adic_generator(f, 7), 3, p)]:
+        for br in (True, False):
+            got = oracle.coset_lde(f, m, b, shift, bit_reverse=br)
+            naive = oracle.coset_lde(f, m, b, shift, bit_reverse=br, naive=True)
+            assert np.array_equal(got, naive)
+            for c in range(w):
+                ref = pyref.coset_lde_column(f, [int(x) for x in m[:, c]], b, shift, br)
+                assert [int(x) for x in got[:, c]] == ref
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_lde_closed_forms(oracle, f):
+    # shift = 1: natural row j * 2^b is the input row j (the subgroup <w_h> sits
+    # inside <w_N>); a constant column stays constant; linearity.
+    m = oracle.random_matrix(f, 5, 256, 3)
+    out = oracle.coset_lde(f, m, 2, 1, bit_reverse=False)
+    assert np.array_equal(out[::4], m)
+    const = np.full((64, 2), 12345, np.uint32)
+    assert np.all(oracle.coset_lde(f, const, 3, oracle.generator(f)) == 12345)
+    a, b2 = oracle.random_matrix(f, 6, 64, 2), oracle.random_matrix(f, 7, 64, 2)
+    p = pyref.P[f]
+    s = ((a.astype(np.uint64) + b2) % p).astype(np.uint32)
+    g = oracle.generator(f)
+    la, lb, ls = (oracle.coset_lde(f, x, 1, g) for x in (a, b2, s))
+    assert np.array_equal(((la.astype(np.uint64) + lb) % p).astype(np.uint32), ls)
+
+
+def test_lde_planted_mistake_is_caught(oracle):
+    # a wrong shift (off by one power of the root) must change the LDE
+    f = 0
+    m = oracle.random_matrix(f, 9, 16, 1)
+    g = oracle.generator(f)
+    good = oracle.coset_lde(f, m, 1, g)
+    bad = oracle.coset_lde(f, m, 1, g * oracle.two_adic_generator(f, 5) % pyref.P[f])
+    assert not np.array_equal(good, bad)
+
+
+# -------------------------------------------------------------------- MMCS --
+SHAPES = [[(8, 1), (8, 7), (4, 8), (2, 9)], [(16, 300)], [(1, 5)], [(4, 3), (8, 0), (8, 2), (1, 4)],
+          [(8, 8), (4, 8), (4, 1), (2, 3), (1, 2)]]
+
+
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_mmcs_vs_definition(oracle, f, shape):
+    mats = [oracle.random_matrix(f, 100 + i, h, w) for i, (h, w) in enumerate(shape)]
+    tree = oracle.MerkleTree(f, mats)
+    rows = [[[int(x) for x in r] for r in m] for m in mats]
+    root, layers = pyref.merkle_root(f, rows, oracle.sponge, oracle.compress)
+    assert [int(x) for x in tree.root] == [int(x) for x in root]
+    heights = [h for h, _ in shape]
+    widths = [w for _, w in shape]
+    for idx in range(tree.max_height):
+        opened, path = tree.open(idx)
+        for k, m in enumerate(mats):
+            assert np.array_equal(opened[k], m[idx >> (tree.log_max - (m.shape[0].bit_length() - 1))])
+        assert pyref.verify_batch(f, heights, widths, tree.root, idx, opened, path, oracle.sponge,
+                                  oracle.compress)
+        assert oracle.mmcs_verify(f, heights, widths, tree.root, idx, opened, path)
+        # planted mistakes: a flipped opened value or sibling must not verify
+        bad = [o.copy() for o in opened]
+        nz = [k for k, o in enumerate(bad) if o.size]
+        bad[nz[0]][0] ^= 1
+        assert not oracle.mmcs_verify(f, heights, widths, tree.root, idx, bad, path)
+        if len(path):
+            bp = path.copy()
+            bp[0, 0] ^= 1
+            assert not oracle.mmcs_verify(f, heights, widths, tree.root, idx, opened, bp)
+
+
+# --------------------------------------------------------------------- FRI --
+@pytest.mark.parametrize("f", FIELDS)
+def test_fri_fold_vs_line(oracle, f):
+    p = pyref.P[f]
+    n = 32
+    v = oracle.random_matrix(f, 11, n, 4)
+    beta = [int(x) for x in oracle.random_matrix(f, 12, 1, 4)[0]]
+    out = oracle.fri_fold(f, v, beta)
+    wn = pyref.root_of_unity(f, 5)
+    for i in range(n // 2):
+        x = pow(wn, pyref.rev(i, 4), p)
+        exp = pyref.fold_pair(f, [int(t) for t in v[2 * i]], [int(t) for t in v[2 * i + 1]], beta, x)
+        assert [int(t) for t in out[i]] == exp
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_fri_fold_keeps_low_degree(oracle, f):
+    # v = bit-reversed evaluations on <w_n> of an ext4 polynomial of degree < n/4
+    # (each coordinate a base LDE with shift 1).  Folding halves the degree
+    # bound; after log(n/4)+1 folds... every fold of a degree < d polynomial is
+    # degree < d/2, so its iDFT's top half vanishes, and folding to length
+    # 2^log_b leaves a constant when d = n / 2^log_b.
+    n, lb = 256, 2
+    coeff_cols = [oracle.random_matrix(f, 20 + k, n >> lb, 1) for k in range(4)]
+    cols = [oracle.coset_lde(f, c, lb, 1, bit_reverse=True)[:, 0] for c in coeff_cols]
+    v = np.stack(cols, axis=1)
+    betas = oracle.random_matrix(f, 30, 8, 4)
+    cur, r = v, 0
+    while cur.shape[0] > (1 << lb):
+        cur = oracle.fri_fold(f, cur, betas[r])
+        r += 1
+        # natural-order evaluations of the folded vector: degree < len / 2^lb
+        nat = np.empty_like(cur)
+        ln = cur.shape[0].bit_length() - 1
+        for i in range(cur.shape[0]):
+            nat[i] = cur[pyref.rev(i, ln)]
+        for k in range(4):
+            co = pyref.interpolate_coeffs(f, [int(x) for x in nat[:, k]])
+            assert all(c == 0 for c in co[cur.shape[0] >> lb:])
+    assert np.all(cur == cur[0])
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_challenger_vs_duplex_rules(oracle, f):
+    ch = oracle.Challenger(f)
+    py = pyref.PyChallenger(f, oracle.poseidon2)
+    rng = np.random.default_rng(4)
+    p = pyref.P[f]
+    for step in range(60):
+        op = rng.integers(0, 4)
+        if op == 0:
+            v = int(rng.integers(0, p))
+            ch.observe(v)
+            py.observe(v)
+        elif op == 1:
+            assert ch.sample() == py.sample()
+        elif op == 2:
+            assert list(ch.sample_ext()) == py.sample_ext()
+        else:
+            assert ch.sample_bits(10) == py.sample_bits(10)
+    # PoW check_witness = observe then sample_bits == 0
+    c2 = ch.clone()
+    ok = c2.check_witness(4, 17)
+    py.observe(17)
+    assert ok == (py.sample_bits(4) == 0)
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_fri_commit_phase_vs_loop(oracle, f):
+    log_n, log_stop = 9, 2
+    v = oracle.random_matrix(f, 40, 1 << log_n, 4)
+    extra = {6: oracle.random_matrix(f, 41, 1 << 6, 4)}
+    ch = oracle.Challenger(f)
+    ch.observe([1, 2, 3])
+    py = ch.clone()
+    roots, layers, final, betas = oracle.fri_commit_phase(ch, v, log_stop, inputs=extra)
+    cur = v
+    p = pyref.P[f]
+    for r in range(log_n - log_stop):
+        assert np.array_equal(layers[r], cur)
+        leaves = cur.reshape(-1, 8)
+        t = oracle.MerkleTree(f, [leaves])
+        assert np.array_equal(t.root, roots[r])
+        py.observe(t.root)
+        beta = py.sample_ext()
+        assert np.array_equal(beta, betas[r])
+        cur = oracle.fri_fold(f, cur, beta)
+        if cur.shape[0].bit_length() - 1 in extra:
+            cur = ((cur.astype(np.uint64) + extra[cur.shape[0].bit_length() - 1]) % p).astype(np.uint32)
+    assert np.array_equal(cur, final)
+
+
+# ------------------------------------------------------------------ inputs --
+@pytest.mark.parametrize("f", FIELDS)
+def test_seeded_inputs(oracle, f):
+    m = oracle.random_matrix(f, 0xC0FFEE, 4, 16).ravel()
+    assert [int(x) for x in m] == [pyref.splitmix_element(f, 0xC0FFEE, i) for i in range(64)]
+    big = oracle.random_matrix(f, 1, 1 << 12, 16)
+    assert big.max() < pyref.P[f]
