@@ -20,13 +20,15 @@ METRIC = "KoalaBear LDE+Merkle commit cells/s at 2^20x256; p50/p99 job latency"
 
 # Poseidon2-W16 KoalaBear work per permutation: 616 modmul + 1416 modadd
 # (8 full rounds x 16 S-boxes x 2 mul; 20 partial rounds x (2 + 16); 9
-# external layers x 72 adds + 128 + 20 x 32).  Priced in VALU issue slots of
-# one lane: a Montgomery modmul = 3 quarter-rate 32-bit multiplies (12) + 2
-# full-rate ops; a modadd = 3 full-rate ops.
-SLOTS_PER_PERM = 616 * 14 + 1416 * 3
-# MI355X VALU issue peak, 32-bit lane ops: 256 CU x 4 SIMD x 16 lanes x
-# 2.4 GHz = 39.3 T/s (the published 157.3 TFLOP/s FP32 vector figure counts
-# an FMA as 2 flops and packed-FP32 dual issue as 2 lanes).
+# external layers x 72 adds + 128 + 20 x 32), priced in 32-bit VALU
+# lane-instructions: a Montgomery modmul = 3 multiplies (v_mad_u64_u32,
+# v_mul_lo_u32, v_mad_u64_u32) + subtract + min = 5; a modadd = add +
+# subtract + min = 3.  (On gfx950 the integer multiplies issue at the full
+# VALU rate: the measured leaf-hash rate is only reachable that way.)
+OPS_PER_PERM = 616 * 5 + 1416 * 3
+# MI355X VALU issue peak, 32-bit lane-instructions: 256 CU x 4 SIMD x 16
+# lanes x 2.4 GHz = 39.3 T/s (the published 157.3 TFLOP/s FP32 vector figure
+# counts an FMA as 2 flops and packed-FP32 dual issue as 2 lanes).
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
 
@@ -96,11 +98,11 @@ def main():
 
     import pico_amd
 
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only; every library call is synchronous
-        sync = lambda: None  # noqa: E731
+    # Every C-ABI call returns after its device work has completed, so the
+    # timed region is bracketed by completed device work on both sides.  torch
+    # is deliberately not imported: its wheel bundles a second HIP runtime,
+    # and two runtimes in one process do not share the device.
+    sync = pico_amd.device_synchronize
 
     f = pico_amd.KOALABEAR
     h, w, lb = 1 << args.log_h, args.width, args.log_blowup
@@ -149,7 +151,7 @@ def main():
     roofline = None
     if dominant in ("leaf_hash_w8", "leaf_hash"):
         avg_s = kern[dominant]["avg_ms"] / 1e3
-        achieved = perms_leaf * SLOTS_PER_PERM / avg_s / 1e12
+        achieved = perms_leaf * OPS_PER_PERM / avg_s / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                     "kernel": dominant, "perms_per_launch": perms_leaf,

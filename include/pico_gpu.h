@@ -73,6 +73,9 @@ int pg_version(void);
 /* hbm_budget_bytes = 0 means "all free device memory". */
 int pg_create(int device, size_t hbm_budget_bytes, pg_ctx **out);
 void pg_destroy(pg_ctx *ctx);
+/* wait for all work on the device (timing brackets; not needed for
+ * correctness, every call below completes its own work) */
+int pg_device_synchronize(int device);
 size_t pg_bytes_in_use(const pg_ctx *ctx);
 /* Per-kernel timing with HIP events on the launch stream (adds one host sync
  * per launch while enabled).  Families: lde_single, lde_pass_a, lde_pass_bc,

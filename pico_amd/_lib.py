@@ -50,6 +50,7 @@ SIGNATURES = {
     "pg_create": (_i, [_i, _sz, _pp]),
     "pg_destroy": (None, [_P]),
     "pg_bytes_in_use": (_sz, [_P]),
+    "pg_device_synchronize": (_i, [_i]),
     "pg_profile_enable": (_i, [_P, _i]),
     "pg_profile_reset": (_i, [_P]),
     "pg_profile_get": (_i, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

@@ -36,6 +36,10 @@ def from_monty(field, a):
     return ((np.asarray(a, dtype=np.uint64) * rinv) % np.uint64(p)).astype(np.uint32)
 
 
+def device_synchronize(device=0):
+    check(lib().pg_device_synchronize(device))
+
+
 def two_adic_generator(field, log_n):
     p = PRIMES[field]
     return pow(GENERATORS[field], (p - 1) >> log_n, p)
@@ -423,4 +427,5 @@ __all__ = [
     "Radix2DitParallel", "MerkleTreeMmcs", "ProverData", "TwoAdicFriPcs",
     "TwoAdicMultiplicativeCoset", "DuplexChallenger", "fri_fold", "fri_commit_phase",
     "fri_grind", "fri_answer_queries", "to_monty", "from_monty", "two_adic_generator", "PgError",
+    "device_synchronize",
 ]

@@ -86,6 +86,11 @@ struct pg_ctx {
     std::map<std::tuple<int, int, int, uint32_t>, std::shared_ptr<DevMem>> scales;
     std::vector<std::tuple<int, int, int, uint32_t>> scale_order;  // insertion order for eviction
     size_t scale_bytes = 0;
+    // freed device blocks kept for reuse, keyed by size
+    std::mutex cache_mu;
+    std::multimap<size_t, void*> cache;
+    size_t cached_bytes = 0;
+    size_t cache_limit = 0;
     // per-kernel-family event timings (profiling mode only)
     std::atomic<bool> profiling{false};
     std::map<std::string, std::pair<double, uint64_t>> prof;
@@ -135,6 +140,22 @@ static std::unique_ptr<EventTimer> make_timer(pg_ctx* ctx) {
         HIP_TRY(te_);                       \
     } while (0)
 
+// Device memory: live bytes are charged against the budget; freed blocks of
+// 1 MiB or more are kept in a size-keyed cache (bounded) and handed out again
+// for the same size, so the repeated multi-GiB LDE / scratch / digest buffers
+// of a commit stream do not go through hipMalloc / hipFree every time.
+// Callers free a block only after the work using it has completed.
+static constexpr size_t CACHE_MIN_BLOCK = 1ull << 20;
+
+static void cache_trim(pg_ctx* ctx, size_t keep_bytes) {  // ctx->cache_mu held
+    while (ctx->cached_bytes > keep_bytes && !ctx->cache.empty()) {
+        auto it = ctx->cache.begin();
+        (void)hipFree(it->second);
+        ctx->cached_bytes -= it->first;
+        ctx->cache.erase(it);
+    }
+}
+
 static int ctx_alloc(pg_ctx* ctx, size_t bytes, void** out) {
     *out = nullptr;
     if (bytes == 0) bytes = 4;
@@ -144,7 +165,25 @@ static int ctx_alloc(pg_ctx* ctx, size_t bytes, void** out) {
         return fail(PG_E_ADMISSION, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu in use)",
                     bytes, prev, ctx->budget);
     }
+    if (bytes >= CACHE_MIN_BLOCK) {
+        std::lock_guard<std::mutex> g(ctx->cache_mu);
+        auto it = ctx->cache.find(bytes);
+        if (it != ctx->cache.end()) {
+            *out = it->second;
+            ctx->cached_bytes -= bytes;
+            ctx->cache.erase(it);
+            return PG_OK;
+        }
+    }
     hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {  // release cached blocks and retry once
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> g(ctx->cache_mu);
+            cache_trim(ctx, 0);
+        }
+        e = hipMalloc(out, bytes);
+    }
     if (e != hipSuccess) {
         ctx->in_use.fetch_sub(bytes);
         (void)hipGetLastError();
@@ -156,9 +195,16 @@ static int ctx_alloc(pg_ctx* ctx, size_t bytes, void** out) {
 
 static void ctx_free(pg_ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
-    (void)hipFree(p);
     if (bytes == 0) bytes = 4;
     ctx->in_use.fetch_sub(bytes);
+    if (bytes >= CACHE_MIN_BLOCK && bytes <= ctx->cache_limit) {
+        std::lock_guard<std::mutex> g(ctx->cache_mu);
+        ctx->cache.emplace(bytes, p);
+        ctx->cached_bytes += bytes;
+        cache_trim(ctx, ctx->cache_limit);
+        return;
+    }
+    (void)hipFree(p);
 }
 
 static std::shared_ptr<DevMem> make_devmem(pg_ctx* ctx, size_t bytes, int* rc) {
@@ -202,6 +248,7 @@ extern "C" int pg_create(int device, size_t budget, pg_ctx** out) {
         budget = free_b;
     }
     c->budget = budget;
+    c->cache_limit = std::min<size_t>(budget / 4, 16ull << 30);
     for (int i = 0; i < 4; i++) {
         hipStream_t s;
         HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -217,11 +264,21 @@ extern "C" void pg_destroy(pg_ctx* ctx) {
     for (auto s : ctx->streams) (void)hipStreamSynchronize(s);
     ctx->twiddles.clear();
     ctx->scales.clear();
+    {
+        std::lock_guard<std::mutex> g(ctx->cache_mu);
+        cache_trim(ctx, 0);
+    }
     for (auto s : ctx->streams) (void)hipStreamDestroy(s);
     delete ctx;
 }
 
 extern "C" size_t pg_bytes_in_use(const pg_ctx* ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+extern "C" int pg_device_synchronize(int device) {
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceSynchronize());
+    return PG_OK;
+}
 
 extern "C" int pg_profile_enable(pg_ctx* ctx, int enable) {
     if (!ctx) return fail(PG_E_INVALID, "null context");

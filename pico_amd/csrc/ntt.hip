@@ -188,9 +188,19 @@ struct LdeArgs {
 
 enum { KIND_SINGLE = 0, KIND_A = 1, KIND_BC = 2, KIND_D = 3 };
 
-constexpr int NTT_C = 16;  // columns per tile
-constexpr int threads_for(int log_p) { return NTT_C * (1 << log_p) >= 256 ? NTT_C * (1 << log_p) : 256; }
-constexpr int groups_for(int log_p) { return threads_for(log_p) / (NTT_C * (1 << log_p)); }
+// Columns per tile: 16 (64-byte row runs), 8 for the largest tile when it
+// also parks coefficients in LDS (exchange + coefficient buffers <= 128 KiB).
+constexpr int cols_for(int log_e, int log_p, int kind) {
+    return (log_e + log_p >= 11 && (kind == KIND_BC || kind == KIND_SINGLE)) ? 8 : 16;
+}
+constexpr int threads_for(int log_e, int log_p, int kind) {
+    return cols_for(log_e, log_p, kind) * (1 << log_p) >= 256 ? cols_for(log_e, log_p, kind) * (1 << log_p) : 256;
+}
+constexpr int groups_for(int log_e, int log_p, int kind) {
+    return threads_for(log_e, log_p, kind) / (cols_for(log_e, log_p, kind) * (1 << log_p));
+}
+// kinds that keep the inverse-DFT output while looping over cosets
+constexpr bool parks_coefficients(int kind) { return kind == KIND_BC || kind == KIND_SINGLE; }
 
 __device__ __forceinline__ uint64_t out_row(const LdeArgs& a, uint32_t cs, uint32_t natural) {
     // committed (bit-reversed) or natural row of coset block cs, natural index j
@@ -198,43 +208,93 @@ __device__ __forceinline__ uint64_t out_row(const LdeArgs& a, uint32_t cs, uint3
                     : (uint64_t)rev_dev(cs, a.b) + ((uint64_t)natural << a.b);
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Renumber them so every XCD runs one contiguous range of logical
+// workgroups: neighbouring column blocks of the same rows, which share
+// 128-byte lines, then hit the same L2.
+__device__ __forceinline__ uint32_t xcd_logical_block() {
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    const uint32_t x = b & 7u, y = b >> 3, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + y;
+}
+
 template <class F, int LOG_E, int LOG_P, int KIND>
-__global__ void __launch_bounds__(threads_for(LOG_P)) lde_kernel(LdeArgs a) {
-    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P, C = NTT_C, G = groups_for(LOG_P);
+__global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND)) lde_kernel(LdeArgs a) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P;
+    constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
+    constexpr int NT = threads_for(LOG_E, LOG_P, KIND);
+    constexpr bool PARK = parks_coefficients(KIND) && E >= 16;
     __shared__ uint32_t lds_all[LOG_P > 0 ? G * S * C : 1];
+    __shared__ uint32_t lds_coef[PARK ? E * NT : 1];
     const int tid = threadIdx.x;
     const int c = tid % C, p = (tid / C) % P, g = tid / (C * P);
     uint32_t* lds = lds_all + (LOG_P > 0 ? g * S * C : 0);
-    const uint64_t tile = (uint64_t)blockIdx.x * G + g;
+    const uint64_t tile = (uint64_t)xcd_logical_block() * G + g;
     const bool valid = tile < a.ntiles;
     const uint32_t h = 1u << a.L, mask = h - 1;
     const uint32_t w = a.w;
+    const uint32_t ncb = (w + C - 1) / C;
+    const uint32_t col = (uint32_t)(tile % ncb) * C + c;  // column block fastest
+    const uint64_t rest = tile / ncb;
+    const bool act = valid && col < w;
     uint32_t x[E];
 
-    if constexpr (KIND == KIND_SINGLE) {
-        const uint32_t col = (uint32_t)tile * C + c;
-        const bool act = valid && col < w;
+    if constexpr (KIND == KIND_SINGLE || KIND == KIND_BC) {
+        // SINGLE: the whole column is one tile.  BC: contiguous block `blk`
+        // of n_lo rows of the pass-A output.
+        const uint32_t n_lo = 1u << a.k2;
+        const uint32_t blk = KIND == KIND_BC ? (uint32_t)rest : 0u;
+        const uint32_t* src = KIND == KIND_BC ? a.out : a.in;
+        const uint64_t ld = KIND == KIND_BC ? (uint64_t)w : a.ld_in;
 #pragma unroll
-        for (int m = 0; m < E; m++) x[m] = act ? a.in[(uint64_t)(p + P * m) * a.ld_in + col] : 0u;
-        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, 0, mask);
+        for (int m = 0; m < E; m++)
+            x[m] = act ? src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * ld + col] : 0u;
+        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, a.k1, mask);
+        // The coefficients are reused for every coset: large tiles park them
+        // in a thread-private LDS slice (VGPRs would spill), small ones keep a
+        // register copy.
+        uint32_t keep[PARK ? 1 : E];
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            if constexpr (PARK)
+                lds_coef[t * NT + tid] = x[t];
+            else
+                keep[t] = x[t];
+        }
+        const uint32_t rblk = rev_dev(blk, a.k1);
         const uint32_t ncos = 1u << a.b;
         for (uint32_t cs = 0; cs < ncos; cs++) {
-            uint32_t y[E];
+            const uint32_t* sc = a.scale + (uint64_t)cs * h;
+            // Opaque copy of the twiddle pointer: keeps the (coset-invariant)
+            // twiddle loads inside the loop instead of hoisting ~64 of them
+            // into VGPRs for all cosets, which spills.
+            const uint32_t* twl = a.tw;
+            asm volatile("" : "+s"(twl));
+            // Likewise re-derive the lane coordinates per coset so the 64-bit
+            // gather/store addresses are not all precomputed outside the loop.
+            int pl = p;
+            uint32_t coll = col;
+            if constexpr (LOG_P > 0) asm volatile("" : "+v"(pl), "+v"(coll));
 #pragma unroll
-            for (int t = 0; t < E; t++)
-                y[t] = fmul<F>(x[t], a.scale[(uint64_t)cs * h + rev_dev(p * E + t, a.L)]);
-            tile_dit<F, LOG_E, LOG_P, C, false>(y, lds, p, c, a.tw, 0, mask);
+            for (int t = 0; t < E; t++) {
+                const uint32_t v = PARK ? lds_coef[t * NT + tid] : keep[PARK ? 0 : t];
+                x[t] = fmul<F>(v, sc[rev_dev(n_lo * blk + pl * E + t, a.L)]);
+            }
+            tile_dit<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask);
             if (act) {
 #pragma unroll
-                for (int r = 0; r < E; r++)
-                    a.out[out_row(a, cs, dit_out_index<LOG_E, LOG_P>(p, r)) * w + col] = y[r];
+                for (int r = 0; r < E; r++) {
+                    const uint32_t j = dit_out_index<LOG_E, LOG_P>(pl, r);
+                    if constexpr (KIND == KIND_BC)
+                        a.tmp[((uint64_t)cs * h + n_lo * blk + j) * w + coll] = fmul<F>(x[r], twl[j * rblk]);
+                    else
+                        a.out[out_row(a, cs, j) * w + coll] = x[r];
+                }
             }
         }
     } else if constexpr (KIND == KIND_A) {
         const uint32_t n_lo = 1u << a.k2;
-        const uint32_t j_lo = (uint32_t)(tile % n_lo);
-        const uint32_t col = (uint32_t)(tile / n_lo) * C + c;
-        const bool act = valid && col < w;
+        const uint32_t j_lo = (uint32_t)rest;
 #pragma unroll
         for (int m = 0; m < E; m++)
             x[m] = act ? a.in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + col] : 0u;
@@ -247,41 +307,10 @@ __global__ void __launch_bounds__(threads_for(LOG_P)) lde_kernel(LdeArgs a) {
                 a.out[(uint64_t)(j_lo + n_lo * pos) * w + col] = fmul<F>(x[t], tw_at<true>(a.tw, e, mask));
             }
         }
-    } else if constexpr (KIND == KIND_BC) {
-        const uint32_t n_hi = 1u << a.k1, n_lo = 1u << a.k2;
-        const uint32_t blk = (uint32_t)(tile % n_hi);
-        const uint32_t col = (uint32_t)(tile / n_hi) * C + c;
-        const bool act = valid && col < w;
-        const uint32_t* src = a.out;  // pass A result
-#pragma unroll
-        for (int m = 0; m < E; m++)
-            x[m] = act ? src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * w + col] : 0u;
-        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, a.k1, mask);
-        const uint32_t rblk = rev_dev(blk, a.k1);
-        const uint32_t ncos = 1u << a.b;
-        for (uint32_t cs = 0; cs < ncos; cs++) {
-            uint32_t y[E];
-#pragma unroll
-            for (int t = 0; t < E; t++)
-                y[t] = fmul<F>(x[t], a.scale[(uint64_t)cs * h + rev_dev(n_lo * blk + p * E + t, a.L)]);
-            tile_dit<F, LOG_E, LOG_P, C, false>(y, lds, p, c, a.tw, a.k1, mask);
-            if (act) {
-#pragma unroll
-                for (int r = 0; r < E; r++) {
-                    const uint32_t j = dit_out_index<LOG_E, LOG_P>(p, r);
-                    a.tmp[((uint64_t)cs * h + n_lo * blk + j) * w + col] =
-                        fmul<F>(y[r], a.tw[j * rblk]);
-                }
-            }
-        }
     } else {  // KIND_D
         const uint32_t n_lo = 1u << a.k2;
-        const uint32_t ncos = 1u << a.b;
-        const uint32_t j_lo = (uint32_t)(tile % n_lo);
-        const uint64_t rest = tile / n_lo;
-        const uint32_t cs = (uint32_t)(rest % ncos);
-        const uint32_t col = (uint32_t)(rest / ncos) * C + c;
-        const bool act = valid && col < w;
+        const uint32_t j_lo = (uint32_t)(rest % n_lo);
+        const uint32_t cs = (uint32_t)(rest / n_lo);
 #pragma unroll
         for (int t = 0; t < E; t++)
             x[t] = act ? a.tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + col] : 0u;
@@ -351,10 +380,13 @@ static const char* const KIND_NAMES[4] = {"lde_single", "lde_pass_a", "lde_pass_
 
 template <class F, int LOG_E, int LOG_P, int KIND>
 static void run(const LdeArgs& a, hipStream_t s, KernelTimer* kt) {
-    constexpr int G = groups_for(LOG_P);
-    const unsigned grid = (unsigned)((a.ntiles + G - 1) / G);
+    constexpr int G = groups_for(LOG_E, LOG_P, KIND);
+    constexpr int C = cols_for(LOG_E, LOG_P, KIND);
+    LdeArgs b = a;
+    b.ntiles = a.ntiles * ((a.w + C - 1) / C);  // callers count row tiles
+    const unsigned grid = (unsigned)((b.ntiles + G - 1) / G);
     if (kt) kt->begin(KIND_NAMES[KIND], s);
-    lde_kernel<F, LOG_E, LOG_P, KIND><<<grid, threads_for(LOG_P), 0, s>>>(a);
+    lde_kernel<F, LOG_E, LOG_P, KIND><<<grid, threads_for(LOG_E, LOG_P, KIND), 0, s>>>(b);
     if (kt) kt->end(s);
 }
 
@@ -394,21 +426,20 @@ static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t 
     a.L = L;
     a.b = b;
     a.bitrev = bitrev;
-    const uint64_t col_blocks = (w + NTT_C - 1) / NTT_C;
     if (L <= 11) {
         a.k1 = 0;
         a.k2 = L;
-        a.ntiles = col_blocks;
+        a.ntiles = 1;
         if (!run_logs<F, KIND_SINGLE>(L, a, s, kt)) return hipErrorInvalidValue;
         return hipGetLastError();
     }
     a.k1 = (L + 1) / 2;
     a.k2 = L - a.k1;
-    a.ntiles = col_blocks << a.k2;
+    a.ntiles = 1ull << a.k2;
     if (!run_logs<F, KIND_A>(a.k1, a, s, kt)) return hipErrorInvalidValue;
-    a.ntiles = col_blocks << a.k1;
+    a.ntiles = 1ull << a.k1;
     if (!run_logs<F, KIND_BC>(a.k2, a, s, kt)) return hipErrorInvalidValue;
-    a.ntiles = (col_blocks << a.k2) << b;
+    a.ntiles = (1ull << a.k2) << b;
     if (!run_logs<F, KIND_D>(a.k1, a, s, kt)) return hipErrorInvalidValue;
     return hipGetLastError();
 }
