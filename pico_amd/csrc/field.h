@@ -68,9 +68,25 @@ PG_HD uint32_t fmul(uint32_t a, uint32_t b) {
     uint32_t s = u - F::P;
     return s < u ? s : u;
 }
+// Montgomery product without the final subtraction.  For a < 2p, b < p the
+// result is below (2p^2 + 2^32 p) / 2^32 < 2p; for a, b < p below 1.5p.
 template <class F>
-PG_HD uint32_t fhalve(uint32_t a) {  // a / 2: works on any representation
-    return (a & 1u) ? (uint32_t)(((uint64_t)a + F::P) >> 1) : (a >> 1);
+PG_HD uint32_t fmul_lazy(uint32_t a, uint32_t b) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * F::MU;
+    return (uint32_t)((t + (uint64_t)m * F::P) >> 32);
+}
+// [0, 2p) -> [0, p)
+template <class F>
+PG_HD uint32_t freduce(uint32_t u) {
+    uint32_t s = u - F::P;
+    return s < u ? s : u;
+}
+// a / 2 for a < p (linear, so any representation): (a >> 1) + (a odd) (p+1)/2
+template <class F>
+PG_HD uint32_t fhalve(uint32_t a) {
+    const uint32_t odd = 0u - (a & 1u);
+    return (a >> 1) + (odd & ((F::P + 1) >> 1));
 }
 template <class F>
 PG_HD uint32_t to_monty(uint32_t a) {

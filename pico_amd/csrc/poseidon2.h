@@ -25,8 +25,51 @@ template <class F>
 struct P2Const {
     uint32_t ext_rc[8][16];
     uint32_t int_rc[F::ROUNDS_P];
+    // pre_rc[r] = M_E^{-1} ext_rc[r]: adding it before an external layer is
+    // the same as adding ext_rc[r] after it (M_E is linear and invertible)
+    uint32_t pre_rc[8][16];
     uint32_t diag[16];
 };
+
+// External-layer matrix: out_{4b+i} = sum_j M4[i][j] (s_{4b+j} + sum_b' s_{4b'+j}),
+// i.e. block (b, b') is 2 M4 on the diagonal and M4 elsewhere.
+constexpr uint32_t M4_COEF[4][4] = {{2, 3, 1, 1}, {1, 2, 3, 1}, {1, 1, 2, 3}, {3, 1, 1, 2}};
+
+// Inverse of the 16 x 16 external matrix mod p by Gauss-Jordan elimination
+// (compile time; the matrix is MDS, so every pivot search succeeds).
+constexpr void ext_matrix_inverse(uint32_t p, uint64_t (&inv)[16][16]) {
+    uint64_t a[16][16] = {};
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) {
+            a[r][c] = M4_COEF[r % 4][c % 4] * ((r / 4 == c / 4) ? 2u : 1u);
+            inv[r][c] = r == c;
+        }
+    for (int col = 0; col < 16; col++) {
+        int piv = col;
+        while (a[piv][col] % p == 0) piv++;
+        for (int c = 0; c < 16; c++) {
+            uint64_t t = a[col][c];
+            a[col][c] = a[piv][c];
+            a[piv][c] = t;
+            t = inv[col][c];
+            inv[col][c] = inv[piv][c];
+            inv[piv][c] = t;
+        }
+        const uint64_t pinv = cpow_canon(p, (uint32_t)(a[col][col] % p), p - 2);
+        for (int c = 0; c < 16; c++) {
+            a[col][c] = a[col][c] * pinv % p;
+            inv[col][c] = inv[col][c] * pinv % p;
+        }
+        for (int r = 0; r < 16; r++) {
+            if (r == col) continue;
+            const uint64_t f = a[r][col] % p;
+            for (int c = 0; c < 16; c++) {
+                a[r][c] = (a[r][c] + (p - f) * a[col][c]) % p;
+                inv[r][c] = (inv[r][c] + (p - f) * inv[col][c]) % p;
+            }
+        }
+    }
+}
 
 template <class F>
 constexpr P2Const<F> make_p2const() {
@@ -38,6 +81,16 @@ constexpr P2Const<F> make_p2const() {
             k.ext_rc[4 + r][i] = cmonty<F>(RC16_RAW[4 + F::ROUNDS_P + r][i] % p);
         }
     for (int r = 0; r < F::ROUNDS_P; r++) k.int_rc[r] = cmonty<F>(RC16_RAW[4 + r][0] % p);
+    uint64_t inv[16][16] = {};
+    ext_matrix_inverse(p, inv);
+    for (int r = 0; r < 8; r++) {
+        const int row = r < 4 ? r : 4 + F::ROUNDS_P + (r - 4);
+        for (int i = 0; i < 16; i++) {
+            uint64_t acc = 0;
+            for (int j = 0; j < 16; j++) acc = (acc + inv[i][j] * (RC16_RAW[row][j] % p)) % p;
+            k.pre_rc[r][i] = cmonty<F>((uint32_t)acc);
+        }
+    }
     // canonical diagonal values; KoalaBear consts.rs:136-154, BabyBear :110-127
     uint32_t d[16] = {};
     if (F::P == KoalaBear::P) {
@@ -62,24 +115,30 @@ struct P2 {
 
 template <class F>
 PG_HD uint32_t sbox(uint32_t x) {
-    uint32_t x2 = fmul<F>(x, x);
+    // x < p.  fmul_lazy leaves a product below 1.5p; a full product with one
+    // such factor and one factor below p is below 1.75p before its final
+    // subtraction, so it still reduces to [0, p).
     if constexpr (F::SBOX == 3) {
-        return fmul<F>(x2, x);
+        return fmul<F>(fmul_lazy<F>(x, x), x);
     } else {
-        uint32_t x3 = fmul<F>(x2, x);
-        uint32_t x4 = fmul<F>(x2, x2);
-        return fmul<F>(x4, x3);
+        const uint32_t x2 = fmul<F>(x, x);
+        const uint32_t x3 = fmul<F>(x2, x);
+        return fmul<F>(fmul_lazy<F>(x2, x2), x3);
     }
 }
 
-// y = M4 x with M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]]
+// y = M4 x with M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] in 11 field
+// additions: with s = a+b+c+d,
+//   y0 = (s + b) + (a + b),  y1 = (s + b) + 2c,
+//   y2 = (s + d) + (c + d),  y3 = (s + d) + 2a.
 template <class F>
 PG_HD void m4(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
-    uint32_t ab = fadd<F>(a, b), cd = fadd<F>(c, d), s = fadd<F>(ab, cd);
-    uint32_t y0 = fadd<F>(fadd<F>(s, ab), b);           // 2a+3b+ c+ d
-    uint32_t y1 = fadd<F>(fadd<F>(s, b), fadd<F>(c, c)); //  a+2b+3c+ d
-    uint32_t y2 = fadd<F>(fadd<F>(s, cd), d);           //  a+ b+2c+3d
-    uint32_t y3 = fadd<F>(fadd<F>(s, d), fadd<F>(a, a)); // 3a+ b+ c+2d
+    const uint32_t ab = fadd<F>(a, b), cd = fadd<F>(c, d), s = fadd<F>(ab, cd);
+    const uint32_t sb = fadd<F>(s, b), sd = fadd<F>(s, d);
+    const uint32_t y0 = fadd<F>(sb, ab);
+    const uint32_t y1 = fadd<F>(sb, fadd<F>(c, c));
+    const uint32_t y2 = fadd<F>(sd, cd);
+    const uint32_t y3 = fadd<F>(sd, fadd<F>(a, a));
     a = y0;
     b = y1;
     c = y2;
@@ -97,17 +156,112 @@ PG_HD void external_layer(uint32_t (&s)[16]) {
     for (int i = 0; i < 16; i++) s[i] = fadd<F>(s[i], t[i & 3]);
 }
 
+// x mod p for x < 70 p (KoalaBear): q = x >> 31 <= 69 and x - q p lies in
+// [0, 2^31 + 69 (2^24 - 1)), below 2p, so one conditional subtraction ends it.
+template <class F>
+PG_HD uint32_t reduce_wide(uint64_t x) {
+    static_assert(F::P == KoalaBear::P, "bound derived for KoalaBear");
+    const uint32_t q = (uint32_t)(x >> 31);
+    return freduce<F>((uint32_t)x - q * F::P);
+}
+
+// KoalaBear external layer with unreduced 64-bit sums (one instruction each on
+// gfx950).  Inputs may be below 2p (an element plus a pre_rc constant); every
+// output combines the 16 inputs with coefficients summing to 35, so it stays
+// below 70p and is reduced once.
+template <class F>
+PG_HD void external_layer_wide(uint32_t (&s)[16]) {
+    uint64_t y[16];
+#pragma unroll
+    for (int b = 0; b < 16; b += 4) {
+        const uint64_t a = s[b], bb = s[b + 1], c = s[b + 2], d = s[b + 3];
+        const uint64_t ab = a + bb, cd = c + d, t = ab + cd, sb = t + bb, sd = t + d;
+        y[b] = sb + ab;
+        y[b + 1] = sb + (c + c);
+        y[b + 2] = sd + cd;
+        y[b + 3] = sd + (a + a);
+    }
+    uint64_t col[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) col[i] = (y[i] + y[4 + i]) + (y[8 + i] + y[12 + i]);
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = reduce_wide<F>(y[i] + col[i & 3]);
+}
+
+// Sum of 16 reduced elements.  Pairs are added in 32 bits (< 2p), the rest in
+// 64 bits (< 16p < 2^35); then q = T >> 31 <= 15 and T - q p lies in
+// [0, 2^31 + 15 (2^31 - p)), below 2p for KoalaBear (one conditional
+// subtraction) and below 3p for BabyBear (two).
+template <class F>
+PG_HD uint32_t sum16(const uint32_t (&s)[16]) {
+    uint32_t l1[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) l1[i] = s[2 * i] + s[2 * i + 1];
+    uint64_t l2[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) l2[i] = (uint64_t)l1[2 * i] + l1[2 * i + 1];
+    const uint64_t t = (l2[0] + l2[1]) + (l2[2] + l2[3]);
+    const uint32_t q = (uint32_t)(t >> 31);
+    uint32_t r = (uint32_t)t - q * F::P;
+    r = freduce<F>(r);
+    if constexpr ((1ull << 31) + 15ull * ((1ull << 31) - F::P) >= 2ull * F::P) r = freduce<F>(r);
+    return r;
+}
+
+// s_i <- s_i * diag_i + S.  Both fields' diagonals start with
+// -2, 1, 2, 1/2, 3, 4, -1/2 (consts.rs:110-154); those entries use
+// additions and a halving instead of a Montgomery product.
 template <class F>
 PG_HD void internal_layer(uint32_t (&s)[16]) {
-    uint32_t sum = s[0];
+    const uint32_t sum = sum16<F>(s);
+    s[0] = fsub<F>(sum, fadd<F>(s[0], s[0]));
+    s[1] = fadd<F>(sum, s[1]);
+    s[2] = fadd<F>(sum, fadd<F>(s[2], s[2]));
+    s[3] = fadd<F>(sum, fhalve<F>(s[3]));
+    s[6] = fsub<F>(sum, fhalve<F>(s[6]));
 #pragma unroll
-    for (int i = 1; i < 16; i++) sum = fadd<F>(sum, s[i]);
-#pragma unroll
-    for (int i = 0; i < 16; i++) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
+    for (int i = 4; i < 16; i++)
+        if (i != 6) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
 }
 
 template <class F>
 PG_HD void permute(uint32_t (&s)[16]) {
+    if constexpr (F::P == KoalaBear::P) {
+        // Same rounds.  The constants of a full round are added (as pre_rc,
+        // unreduced: the result stays below 2p) before the external layer
+        // that precedes the round, so the layer's single reduction covers them.
+#pragma unroll
+        for (int i = 0; i < 16; i++) s[i] += P2<F>::K.pre_rc[0][i];
+        external_layer_wide<F>(s);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                s[i] = sbox<F>(s[i]);
+                if (r < 3) s[i] += P2<F>::K.pre_rc[r + 1][i];
+            }
+            external_layer_wide<F>(s);
+        }
+        s[0] = fadd<F>(s[0], P2<F>::K.int_rc[0]);
+#pragma unroll
+        for (int r = 0; r < F::ROUNDS_P; r++) {
+            s[0] = sbox<F>(s[0]);
+            internal_layer<F>(s);
+            if (r + 1 < F::ROUNDS_P) s[0] = fadd<F>(s[0], P2<F>::K.int_rc[r + 1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) s[i] = fadd<F>(s[i], P2<F>::K.ext_rc[4][i]);
+#pragma unroll
+        for (int r = 4; r < 8; r++) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                s[i] = sbox<F>(s[i]);
+                if (r < 7) s[i] += P2<F>::K.pre_rc[r + 1][i];
+            }
+            external_layer_wide<F>(s);
+        }
+        return;
+    }
     external_layer<F>(s);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
