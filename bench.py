@@ -18,18 +18,26 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KoalaBear LDE+Merkle commit cells/s at 2^20x256; p50/p99 job latency"
 
-# Poseidon2-W16 KoalaBear work per permutation: 616 modmul + 1416 modadd
-# (8 full rounds x 16 S-boxes x 2 mul; 20 partial rounds x (2 + 16); 9
-# external layers x 72 adds + 128 + 20 x 32), priced in 32-bit VALU
-# lane-instructions: a Montgomery modmul = 3 multiplies (v_mad_u64_u32,
-# v_mul_lo_u32, v_mad_u64_u32) + subtract + min = 5; a modadd = add +
-# subtract + min = 3.  (On gfx950 the integer multiplies issue at the full
-# VALU rate: the measured leaf-hash rate is only reachable that way.)
-OPS_PER_PERM = 616 * 5 + 1416 * 3
+# Poseidon2-W16 KoalaBear work per permutation, textbook Montgomery form:
+# 616 modmul + 1416 modadd (8 full rounds x 16 S-boxes x 2 mul; 20 partial
+# rounds x (2 + 16); 9 external layers x 72 adds + 128 + 20 x 32) = 5 + 3
+# VALU lane-instructions each -> 7328.  Reported beside the count of the
+# built kernel (pico_amd/_build/isa_stats.json, loop body per sponge chunk).
+TEXTBOOK_OPS_PER_PERM = 616 * 5 + 1416 * 3
 # MI355X VALU issue peak, 32-bit lane-instructions: 256 CU x 4 SIMD x 16
 # lanes x 2.4 GHz = 39.3 T/s (the published 157.3 TFLOP/s FP32 vector figure
-# counts an FMA as 2 flops and packed-FP32 dual issue as 2 lanes).
+# counts an FMA as 2 flops and packed-FP32 dual issue as 2 lanes).  gfx950
+# issues the 32-bit integer multiplies at this full rate.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def isa_valu_per_chunk(field="koalabear"):
+    path = os.path.join(ROOT, "pico_amd", "_build", "isa_stats.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[field]["leaf_hash_w8_valu_per_chunk"]
+    except (OSError, KeyError, ValueError):
+        return None
 HBM_PEAK_GBS = 8000.0
 
 
@@ -151,11 +159,16 @@ def main():
     roofline = None
     if dominant in ("leaf_hash_w8", "leaf_hash"):
         avg_s = kern[dominant]["avg_ms"] / 1e3
-        achieved = perms_leaf * OPS_PER_PERM / avg_s / 1e12
+        per_perm = isa_valu_per_chunk() or TEXTBOOK_OPS_PER_PERM
+        achieved = perms_leaf * per_perm / avg_s / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                     "kernel": dominant, "perms_per_launch": perms_leaf,
-                    "perm_rate_per_s": perms_leaf / avg_s}
+                    "perm_rate_per_s": perms_leaf / avg_s,
+                    "valu_instr_per_perm": per_perm,
+                    "textbook_instr_per_perm": TEXTBOOK_OPS_PER_PERM,
+                    "perm_rate_vs_textbook_at_peak": round(perms_leaf / avg_s * TEXTBOOK_OPS_PER_PERM
+                                                           / (VALU_PEAK_TOPS * 1e12), 4)}
     # HBM view of the NTT passes: algorithmic bytes per launch / duration
     by = 4 * w
     pass_bytes = {"lde_pass_a": 2 * h * by, "lde_pass_bc": h * by + n_rows * by,
