@@ -147,6 +147,50 @@ int pg_pcs_commit(pg_ctx *ctx, int field, const pg_buf *const *evals,
  * returned in natural order (rows * width words). */
 int pg_pcs_evaluations(pg_ctx *ctx, const pg_tree *tree, size_t k, size_t rows, uint32_t *host);
 
+/* ---- batched commits ------------------------------------------------------
+ * Many independent single-matrix Pcs::commit jobs (the reference proves the
+ * shards of a batch in parallel rayon workers, vm/src/instances/machine/
+ * riscv.rs:235-309) as one launch sequence: each LDE pass once per distinct
+ * log-height, one leaf-hash launch, one launch per Merkle level, one root
+ * gather.  create allocates every output and uploads all descriptors;
+ * use_graph != 0 captures the sequence as a hipGraph.  run executes it (it
+ * may run repeatedly while no tree has been taken) and returns njobs x 8 root
+ * words.  The input buffers must stay alive until the batch is freed.
+ * take_tree hands job j's prover data to the caller (free with pg_tree_free). */
+typedef struct pg_batch pg_batch;
+int pg_batch_create(pg_ctx *ctx, int field, const pg_buf *const *evals,
+                    const uint32_t *domain_shifts_mont, const uint32_t *log_blowups, size_t njobs,
+                    int use_graph, pg_batch **out);
+int pg_batch_run(pg_ctx *ctx, pg_batch *batch, uint32_t *roots_out);
+int pg_batch_take_tree(pg_batch *batch, size_t job, pg_tree **out);
+size_t pg_batch_num_launches(const pg_batch *batch);
+void pg_batch_free(pg_batch *batch);
+
+/* ---- commit service -------------------------------------------------------
+ * The GPU counterpart of the reference's emulator -> bounded channel -> prover
+ * pipeline (vm/src/instances/machine/riscv.rs:93-129, shards proven in
+ * parallel at :235): shard commit jobs are submitted as they are produced,
+ * admitted by HBM footprint (LDE + scratch + digests + trace) against the
+ * service budget (0 = the context's budget), run `workers` at a time on
+ * their own streams, and keep their LDE and Merkle tree on the device for
+ * openings until released.  Jobs submitted while others run are admitted as
+ * memory frees and overlap with them on the device.
+ * submit takes ownership of `trace`; submit_synthetic fills the trace on the
+ * device from a seed (pg_fill_random's stream).  wait blocks until the job
+ * is done and returns its root; times (may be null) = submit, start and done
+ * times in seconds (steady clock). */
+typedef struct pg_service pg_service;
+int pg_service_create(pg_ctx *ctx, size_t budget_bytes, int workers, pg_service **out);
+int pg_service_submit(pg_service *svc, int field, pg_buf *trace, uint32_t log_blowup, uint64_t *job_id);
+int pg_service_submit_synthetic(pg_service *svc, int field, uint64_t seed, size_t rows, size_t cols,
+                                uint32_t log_blowup, uint64_t *job_id);
+int pg_service_wait(pg_service *svc, uint64_t job_id, uint32_t root[8], double times[3]);
+int pg_service_open(pg_service *svc, uint64_t job_id, const uint64_t *indices, size_t n_idx,
+                    uint32_t *rows_out, uint32_t *paths_out);
+int pg_service_release(pg_service *svc, uint64_t job_id);
+int pg_service_stats(pg_service *svc, size_t *reserved_bytes, size_t *pending_jobs, size_t *peak_running);
+void pg_service_destroy(pg_service *svc);
+
 /* ---- FRI ----------------------------------------------------------------- */
 /* out = fold_even_odd(v, beta): v is n x 4 (ext4, bit-reversed order), out is
  * (n/2) x 4. */

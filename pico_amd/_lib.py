@@ -77,6 +77,19 @@ SIGNATURES = {
     "pg_tree_free": (None, [_P]),
     "pg_pcs_commit": (_i, [_P, _i, _P, _P, _sz, _u32, _P, _pp]),
     "pg_pcs_evaluations": (_i, [_P, _P, _sz, _sz, _P]),
+    "pg_batch_create": (_i, [_P, _i, _P, _P, _P, _sz, _i, _pp]),
+    "pg_batch_run": (_i, [_P, _P, _P]),
+    "pg_batch_take_tree": (_i, [_P, _sz, _pp]),
+    "pg_batch_num_launches": (_sz, [_P]),
+    "pg_batch_free": (None, [_P]),
+    "pg_service_create": (_i, [_P, _sz, _i, _pp]),
+    "pg_service_submit": (_i, [_P, _i, _P, _u32, _P]),
+    "pg_service_submit_synthetic": (_i, [_P, _i, _u64, _sz, _sz, _u32, _P]),
+    "pg_service_wait": (_i, [_P, _u64, _P, _P]),
+    "pg_service_open": (_i, [_P, _u64, _P, _sz, _P, _P]),
+    "pg_service_release": (_i, [_P, _u64]),
+    "pg_service_stats": (_i, [_P, _P, _P, _P]),
+    "pg_service_destroy": (None, [_P]),
     "pg_fri_fold": (_i, [_P, _i, _P, _P, _pp]),
     "pg_fri_commit_phase": (_i, [_P, _i, _P, _P, _i, _i, _i, _P, _P, _P, _P, _pp]),
     "pg_fri_grind": (_i, [_P, _i, _P, _i, _P]),

@@ -306,6 +306,106 @@ class TwoAdicFriPcs:
         return from_monty(self.field, out)
 
 
+class CommitBatch:
+    """Many single-matrix Pcs::commit jobs as one launch sequence (optionally a
+    captured hipGraph).  jobs: [(DeviceMatrix, log_blowup)] or
+    [(DeviceMatrix, log_blowup, domain_shift)].  The inputs must stay alive
+    while the batch exists."""
+
+    def __init__(self, ctx, field, jobs, use_graph=True):
+        self.ctx, self.field, self.n = ctx, field, len(jobs)
+        self._inputs = [j[0] for j in jobs]
+        arr = (ctypes.c_void_p * self.n)(*[j[0].handle for j in jobs])
+        lbs = np.ascontiguousarray([j[1] for j in jobs], dtype=np.uint32)
+        shifts = None
+        if any(len(j) > 2 for j in jobs):
+            shifts = np.ascontiguousarray(to_monty(field, [j[2] if len(j) > 2 else 1 for j in jobs]))
+        h = ctypes.c_void_p()
+        check(lib().pg_batch_create(ctx.handle, field, arr, None if shifts is None else _ptr(shifts),
+                                    _ptr(lbs), self.n, int(use_graph), ctypes.byref(h)))
+        self.handle = h
+        self.num_launches = lib().pg_batch_num_launches(h)
+
+    def run(self):
+        roots = np.zeros((self.n, 8), np.uint32)
+        check(lib().pg_batch_run(self.ctx.handle, self.handle, _ptr(roots)))
+        return from_monty(self.field, roots)
+
+    def take(self, j):
+        h = ctypes.c_void_p()
+        check(lib().pg_batch_take_tree(self.handle, j, ctypes.byref(h)))
+        return ProverData(self.ctx, self.field, h)
+
+    def free(self):
+        if self.handle:
+            lib().pg_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CommitService:
+    """Device-resident commit service (pg_service): submit shard commit jobs as
+    they arrive, wait for roots, open from the resident prover data, release."""
+
+    def __init__(self, ctx, budget=0, workers=3):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().pg_service_create(ctx.handle, budget, workers, ctypes.byref(h)))
+        self.handle = h
+
+    def submit(self, field, trace, log_blowup):
+        """Hands `trace` (a DeviceMatrix) to the service; returns a job id."""
+        jid = ctypes.c_uint64()
+        check(lib().pg_service_submit(self.handle, field, trace.handle, log_blowup, ctypes.byref(jid)))
+        trace._owned = False
+        trace.handle = None
+        return jid.value
+
+    def submit_synthetic(self, field, seed, rows, cols, log_blowup):
+        jid = ctypes.c_uint64()
+        check(lib().pg_service_submit_synthetic(self.handle, field, seed, rows, cols, log_blowup,
+                                                ctypes.byref(jid)))
+        return jid.value
+
+    def wait(self, job_id, field):
+        """(root canonical[8], (t_submit, t_start, t_done))"""
+        root = np.zeros(8, np.uint32)
+        times = np.zeros(3, np.float64)
+        check(lib().pg_service_wait(self.handle, job_id, _ptr(root), _ptr(times)))
+        return from_monty(field, root), tuple(times)
+
+    def open(self, job_id, field, indices, total_width, log_height):
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        rows = np.zeros((idx.size, total_width), np.uint32)
+        paths = np.zeros((idx.size, max(log_height, 1), 8), np.uint32)
+        check(lib().pg_service_open(self.handle, job_id, _ptr(idx), idx.size, _ptr(rows), _ptr(paths)))
+        return from_monty(field, rows), from_monty(field, paths[:, :log_height])
+
+    def release(self, job_id):
+        check(lib().pg_service_release(self.handle, job_id))
+
+    def stats(self):
+        r, p, k = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().pg_service_stats(self.handle, ctypes.byref(r), ctypes.byref(p), ctypes.byref(k)))
+        return {"reserved_bytes": r.value, "pending": p.value, "peak_running": k.value}
+
+    def close(self):
+        if self.handle:
+            lib().pg_service_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ----------------------------------------------------------- challenger --
 class DuplexChallenger:
     """DuplexChallenger<F, Poseidon2, 16, 8>; values are canonical here."""
@@ -427,5 +527,5 @@ __all__ = [
     "Radix2DitParallel", "MerkleTreeMmcs", "ProverData", "TwoAdicFriPcs",
     "TwoAdicMultiplicativeCoset", "DuplexChallenger", "fri_fold", "fri_commit_phase",
     "fri_grind", "fri_answer_queries", "to_monty", "from_monty", "two_adic_generator", "PgError",
-    "device_synchronize",
+    "device_synchronize", "CommitBatch", "CommitService",
 ]

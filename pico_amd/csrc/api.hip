@@ -14,10 +14,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdarg>
 #include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <deque>
+#include <thread>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -555,9 +560,17 @@ extern "C" void pg_tree_free(pg_tree* t) {
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const std::vector<pg_buf*>& mats,
-                                 const std::vector<bool>& owned, uint32_t root[8], pg_tree** out) {
-    *out = nullptr;
+static void tree_discard(pg_tree* x) {  // release only what the tree allocated itself
+    for (size_t k = 0; k < x->owned.size(); k++) x->owned[k] = false;
+    pg_tree_free(x);
+}
+using TreePtr = std::unique_ptr<pg_tree, void (*)(pg_tree*)>;
+
+// Allocates a tree's digest layers and device metadata (matrix descriptors
+// in original and height-sorted order, height shifts, column offsets, layer
+// offsets) without hashing anything.  The tree owns none of `mats` yet.
+static int tree_prepare(pg_ctx* ctx, hipStream_t s, int field, const std::vector<pg_buf*>& mats, TreePtr* out,
+                        std::vector<size_t>* order_out, MatRef** d_sorted_out) {
     const size_t n = mats.size();
     if (n == 0) return fail(PG_E_INVALID, "no matrices to commit");
     size_t H = 0;
@@ -566,11 +579,7 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
         if (!is_pow2(m->rows)) return fail(PG_E_INVALID, "height %zu is not a power of two", m->rows);
         H = std::max(H, m->rows);
     }
-    std::unique_ptr<pg_tree, void (*)(pg_tree*)> t(new pg_tree, [](pg_tree* x) {
-        // on failure: release only what the tree allocated itself
-        for (size_t k = 0; k < x->owned.size(); k++) x->owned[k] = false;
-        pg_tree_free(x);
-    });
+    TreePtr t(new pg_tree, tree_discard);
     t->ctx = ctx;
     t->field = field;
     t->H = H;
@@ -629,6 +638,21 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
         PG_TRY(ctx_alloc(ctx, t->digest_bytes, &p));
         t->digests = (uint32_t*)p;
     }
+    *order_out = std::move(order);
+    *d_sorted_out = d_sorted;
+    *out = std::move(t);
+    return PG_OK;
+}
+
+static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const std::vector<pg_buf*>& mats,
+                                 const std::vector<bool>& owned, uint32_t root[8], pg_tree** out) {
+    *out = nullptr;
+    const size_t n = mats.size();
+    TreePtr t(nullptr, tree_discard);
+    std::vector<size_t> order;
+    MatRef* d_sorted = nullptr;
+    PG_TRY(tree_prepare(ctx, s, field, mats, &t, &order, &d_sorted));
+    const size_t H = t->H;
     // layer 0: every matrix of height H
     size_t g0 = 0, g1 = 0;
     uint32_t wsum = 0;
@@ -848,6 +872,543 @@ extern "C" int pg_pcs_evaluations(pg_ctx* ctx, const pg_tree* t, size_t k, size_
     for (size_t j = 0; j < rows; j++)
         memcpy(host + j * m->cols, tmp.data() + (size_t)bit_rev((uint32_t)j, q) * m->cols, m->cols * 4);
     return PG_OK;
+}
+
+// ----------------------------------------------------------------- batch --
+// Many independent single-matrix PCS commits as one launch sequence: each
+// LDE pass runs once per distinct log-height over all jobs of that height,
+// the leaf layer runs once over every job's rows, each Merkle level once over
+// every job that has it, and a final kernel gathers the roots.  All outputs
+// and descriptors are allocated up front; the sequence can be captured as a
+// hipGraph and replayed.
+struct pg_batch {
+    pg_ctx* ctx = nullptr;
+    int field = 0;
+    size_t njobs = 0;
+    std::vector<pg_tree*> trees;
+    std::vector<bool> taken;
+    std::vector<std::shared_ptr<DevMem>> keep;  // tables, scratch, descriptors
+    uint32_t* d_roots = nullptr;
+    std::vector<std::function<hipError_t(hipStream_t)>> launches;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" void pg_batch_free(pg_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (size_t j = 0; j < b->trees.size(); j++)
+        if (b->trees[j] && !b->taken[j]) pg_tree_free(b->trees[j]);
+    b->keep.clear();
+    delete b;
+}
+
+// host staging for one device descriptor block
+struct Staging {
+    std::vector<uint8_t> bytes;
+    size_t put(const void* p, size_t n) {  // p == nullptr reserves n zero bytes
+        const size_t off = align_up(bytes.size(), 256);
+        bytes.resize(off + n, 0);
+        if (n && p) memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* evals, const uint32_t* dshift,
+                               const uint32_t* log_blowups, size_t njobs, int use_graph, pg_batch** out) {
+    if (!ctx || !evals || !log_blowups || !out || !field_ok(field) || njobs == 0)
+        return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    std::unique_ptr<pg_batch, void (*)(pg_batch*)> b(new pg_batch, pg_batch_free);
+    b->ctx = ctx;
+    b->field = field;
+    b->njobs = njobs;
+    b->trees.assign(njobs, nullptr);
+    b->taken.assign(njobs, false);
+    const uint32_t g = gen_monty(field);
+    std::vector<int> L(njobs), lb(njobs);
+    std::vector<uint64_t> N(njobs), W(njobs), tmp_off(njobs, 0);
+    uint64_t tmp_words = 0;
+    for (size_t j = 0; j < njobs; j++) {
+        const pg_buf* m = evals[j];
+        if (!m || !is_pow2(m->rows) || m->cols == 0) return fail(PG_E_INVALID, "job %zu: bad matrix", j);
+        L[j] = log2_of(m->rows);
+        lb[j] = (int)log_blowups[j];
+        if (L[j] > lde_max_log_h()) return fail(PG_E_UNSUPPORTED, "job %zu: log height %d", j, L[j]);
+        if (L[j] + lb[j] > two_adicity(field) || lb[j] > 8) return fail(PG_E_INVALID, "job %zu: blowup", j);
+        N[j] = m->rows << lb[j];
+        W[j] = m->cols;
+        if (L[j] > 11) {
+            tmp_off[j] = tmp_words;
+            tmp_words += N[j] * W[j];
+        }
+    }
+    // outputs: one LDE buffer and one tree per job
+    std::vector<uint32_t*> digests(njobs);
+    for (size_t j = 0; j < njobs; j++) {
+        pg_buf* o;
+        PG_TRY(buf_new(ctx, N[j], W[j], &o));
+        TreePtr t(nullptr, tree_discard);
+        std::vector<size_t> order;
+        MatRef* ds;
+        const int rc = tree_prepare(ctx, l.s, field, {o}, &t, &order, &ds);
+        if (rc != PG_OK) {
+            pg_buf_free(o);
+            return rc;
+        }
+        t->owned = {true};
+        digests[j] = t->digests;
+        b->trees[j] = t.release();
+    }
+    std::shared_ptr<DevMem> scratch;
+    if (tmp_words) {
+        int rc;
+        scratch = make_devmem(ctx, tmp_words * 4, &rc);
+        if (!scratch) return rc;
+        b->keep.push_back(scratch);
+    }
+    // descriptors
+    Staging st;
+    struct PassRec {
+        int kind, log_s, njobs, max_b;
+        size_t off_args, off_prefix;
+        uint64_t total;
+    };
+    std::vector<PassRec> passes;
+    std::map<int, std::vector<size_t>> by_L;
+    for (size_t j = 0; j < njobs; j++) by_L[L[j]].push_back(j);
+    for (auto& kv : by_L) {
+        const int Lg = kv.first;
+        int k1, k2;
+        lde_split(Lg, &k1, &k2);
+        std::shared_ptr<DevMem> tw;
+        PG_TRY(get_twiddles(ctx, l.s, field, Lg, &tw));
+        b->keep.push_back(tw);
+        std::vector<LdeArgs> args;
+        for (size_t j : kv.second) {
+            uint32_t shift = g;
+            if (dshift) {
+                if (dshift[j] == 0) return fail(PG_E_INVALID, "job %zu: domain shift is zero", j);
+                shift = field == PG_KOALABEAR ? fmul<KoalaBear>(g, finv<KoalaBear>(dshift[j]))
+                                              : fmul<BabyBear>(g, finv<BabyBear>(dshift[j]));
+            }
+            std::shared_ptr<DevMem> sc;
+            PG_TRY(get_scale(ctx, l.s, field, Lg, lb[j], shift, &sc));
+            b->keep.push_back(sc);
+            LdeArgs a{};
+            a.in = evals[j]->ptr;
+            a.ld_in = evals[j]->cols;
+            a.out = b->trees[j]->mats[0]->ptr;
+            a.tmp = scratch && Lg > 11 ? (uint32_t*)scratch->ptr + tmp_off[j] : nullptr;
+            a.tw = (const uint32_t*)tw->ptr;
+            a.scale = (const uint32_t*)sc->ptr;
+            a.w = (uint32_t)W[j];
+            a.L = Lg;
+            a.k1 = k1;
+            a.k2 = k2;
+            a.b = lb[j];
+            a.bitrev = 1;
+            args.push_back(a);
+        }
+        const int kinds3[3] = {KIND_A, KIND_BC, KIND_D};
+        for (int i = 0; i < (Lg <= 11 ? 1 : 3); i++) {
+            const int kind = Lg <= 11 ? KIND_SINGLE : kinds3[i];
+            const int log_s = (kind == KIND_BC || kind == KIND_SINGLE) ? k2 : k1;
+            const int C = lde_pass_cols(kind, log_s);
+            std::vector<uint64_t> prefix(1, 0);
+            for (auto& a : args)
+                prefix.push_back(prefix.back() + lde_pass_row_tiles(kind, Lg, a.b) * ((a.w + C - 1) / C));
+            int max_b = 0;
+            for (auto& a : args) max_b = std::max(max_b, a.b);
+            PassRec pr{kind, log_s, (int)args.size(), max_b, st.put(args.data(), args.size() * sizeof(LdeArgs)),
+                       st.put(prefix.data(), prefix.size() * 8), prefix.back()};
+            passes.push_back(pr);
+        }
+    }
+    // Merkle: leaf layer, then levels over the jobs (tallest first) that have them
+    std::vector<size_t> tall(njobs);
+    for (size_t j = 0; j < njobs; j++) tall[j] = j;
+    std::stable_sort(tall.begin(), tall.end(), [&](size_t a, size_t c) { return N[a] > N[c]; });
+    std::vector<LeafJob> leaves;
+    std::vector<uint64_t> leaf_prefix(1, 0);
+    for (size_t j : tall) {
+        leaves.push_back(LeafJob{b->trees[j]->mats[0]->ptr, W[j], (uint32_t)W[j], 0, digests[j]});
+        leaf_prefix.push_back(leaf_prefix.back() + N[j]);
+    }
+    const size_t off_leaf = st.put(leaves.data(), leaves.size() * sizeof(LeafJob));
+    const size_t off_leaf_prefix = st.put(leaf_prefix.data(), leaf_prefix.size() * 8);
+    struct LevelRec {
+        size_t off_jobs, off_prefix;
+        int njobs;
+        uint64_t total;
+    };
+    std::vector<LevelRec> levels;
+    const int max_log = log2_of(N[tall[0]]);
+    for (int lv = 1; lv <= max_log; lv++) {
+        std::vector<CompressJob> cj;
+        std::vector<uint64_t> pf(1, 0);
+        for (size_t j : tall) {
+            const int lj = log2_of(N[j]);
+            if (lj < lv) break;
+            uint64_t off_prev = 0;
+            for (int t = 0; t < lv - 1; t++) off_prev += N[j] >> t;
+            const uint64_t off_next = off_prev + (N[j] >> (lv - 1));
+            cj.push_back(CompressJob{digests[j] + 8 * off_prev, digests[j] + 8 * off_next});
+            pf.push_back(pf.back() + (N[j] >> lv));
+        }
+        levels.push_back(LevelRec{st.put(cj.data(), cj.size() * sizeof(CompressJob)), st.put(pf.data(), pf.size() * 8),
+                                  (int)cj.size(), pf.back()});
+    }
+    std::vector<const uint32_t*> root_src(njobs);
+    for (size_t j = 0; j < njobs; j++) root_src[j] = digests[j] + 8 * (2 * N[j] - 2);
+    const size_t off_root_src = st.put(root_src.data(), njobs * sizeof(void*));
+    const size_t off_roots = st.put(nullptr, njobs * 32);
+    int rc;
+    auto desc = make_devmem(ctx, st.bytes.size(), &rc);
+    if (!desc) return rc;
+    b->keep.push_back(desc);
+    uint8_t* dd = (uint8_t*)desc->ptr;
+    HIP_TRY(hipMemcpyAsync(dd, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, l.s));
+    HIP_TRY(hipStreamSynchronize(l.s));
+    b->d_roots = (uint32_t*)(dd + off_roots);
+    // launch list
+    for (const auto& pr : passes) {
+        const LdeArgs* jobs = (const LdeArgs*)(dd + pr.off_args);
+        const uint64_t* pf = (const uint64_t*)(dd + pr.off_prefix);
+        b->launches.push_back([=](hipStream_t s) {
+            return launch_lde_pass_batch(field, pr.kind, pr.log_s, jobs, pf, pr.njobs, pr.total, pr.max_b, s);
+        });
+    }
+    {
+        const LeafJob* lj = (const LeafJob*)(dd + off_leaf);
+        const uint64_t* lp = (const uint64_t*)(dd + off_leaf_prefix);
+        const uint64_t total = leaf_prefix.back();
+        const int nj = (int)njobs;
+        b->launches.push_back([=](hipStream_t s) { return launch_leaf_hash_batch(field, lj, lp, nj, total, s); });
+    }
+    for (const auto& lv : levels) {
+        const CompressJob* cj = (const CompressJob*)(dd + lv.off_jobs);
+        const uint64_t* pf = (const uint64_t*)(dd + lv.off_prefix);
+        b->launches.push_back([=](hipStream_t s) { return launch_compress_batch(field, cj, pf, lv.njobs, lv.total, s); });
+    }
+    {
+        const uint32_t* const* rs = (const uint32_t* const*)(dd + off_root_src);
+        uint32_t* dst = b->d_roots;
+        const int nj = (int)njobs;
+        b->launches.push_back([=](hipStream_t s) { return launch_gather_roots(rs, dst, nj, s); });
+    }
+    if (use_graph) {
+        HIP_TRY(hipStreamBeginCapture(l.s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (auto& fn : b->launches)
+            if (e == hipSuccess) e = fn(l.s);
+        hipError_t e2 = hipStreamEndCapture(l.s, &b->graph);
+        HIP_TRY(e);
+        HIP_TRY(e2);
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    *out = b.release();
+    return PG_OK;
+}
+
+extern "C" int pg_batch_run(pg_ctx* ctx, pg_batch* b, uint32_t* roots_out) {
+    if (!ctx || !b || b->ctx != ctx) return fail(PG_E_INVALID, "bad argument");
+    for (size_t j = 0; j < b->njobs; j++)  // its buffers now belong to the caller
+        if (b->taken[j]) return fail(PG_E_INVALID, "tree %zu was taken; the batch cannot run again", j);
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    if (b->exec) {
+        HIP_TRY(hipGraphLaunch(b->exec, l.s));
+    } else {
+        for (auto& fn : b->launches) HIP_TRY(fn(l.s));
+    }
+    std::vector<uint32_t> roots(b->njobs * 8);
+    HIP_TRY(hipMemcpyAsync(roots.data(), b->d_roots, roots.size() * 4, hipMemcpyDeviceToHost, l.s));
+    HIP_TRY(hipStreamSynchronize(l.s));
+    for (size_t j = 0; j < b->njobs; j++)
+        if (b->trees[j]) memcpy(b->trees[j]->root, &roots[8 * j], 32);
+    if (roots_out) memcpy(roots_out, roots.data(), roots.size() * 4);
+    return PG_OK;
+}
+
+extern "C" int pg_batch_take_tree(pg_batch* b, size_t j, pg_tree** out) {
+    if (!b || !out || j >= b->njobs) return fail(PG_E_INVALID, "bad argument");
+    if (b->taken[j]) return fail(PG_E_INVALID, "tree %zu already taken", j);
+    b->taken[j] = true;
+    *out = b->trees[j];
+    return PG_OK;
+}
+
+extern "C" size_t pg_batch_num_launches(const pg_batch* b) { return b ? b->launches.size() : 0; }
+
+// --------------------------------------------------------------- service --
+// Device-resident commit service.  Jobs (one trace each) queue on submit; a
+// job is admitted when its HBM footprint fits the service budget minus what
+// admitted-but-unreleased jobs hold (first fit, with a bound on how often the
+// oldest job may be passed over).  Worker threads, each with its own stream,
+// run admitted jobs concurrently, so jobs arriving while others are on the
+// GPU overlap with them.  Prover data stays on the device for openings until
+// the job is released.
+struct SvcJob {
+    uint64_t id = 0;
+    int field = 0;
+    uint64_t seed = 0;
+    pg_buf* trace = nullptr;  // owned input, or null for a seeded synthetic trace
+    size_t rows = 0, cols = 0;
+    int lb = 1;
+    size_t footprint = 0;
+    int skipped = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    int state = 0;  // 0 pending, 1 running, 2 done
+    int rc = PG_OK;
+    std::string err;
+    pg_tree* tree = nullptr;
+    uint32_t root[8] = {0};
+};
+
+struct pg_service {
+    pg_ctx* ctx = nullptr;
+    size_t budget = 0, reserved = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> pending;
+    std::map<uint64_t, std::unique_ptr<SvcJob>> jobs;
+    std::vector<std::thread> workers;
+    std::vector<hipStream_t> streams;
+    bool stop = false;
+    uint64_t next_id = 1;
+    size_t running = 0, peak_running = 0;
+};
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static size_t job_footprint(size_t rows, size_t cols, int lb, bool owns_trace) {
+    const size_t N = rows << lb;
+    const size_t lde = N * cols * 4, digests = 2 * N * 32;
+    const size_t tmp = log2_of(rows) > 11 ? lde : 0;
+    return lde + digests + tmp + (owns_trace ? rows * cols * 4 : 0);
+}
+
+static void run_svc_job(pg_service* sv, SvcJob* j, hipStream_t s) {
+    pg_ctx* ctx = sv->ctx;
+    (void)hipSetDevice(ctx->device);
+    pg_buf* tr = j->trace;
+    int rc = PG_OK;
+    if (!tr) {
+        rc = buf_new(ctx, j->rows, j->cols, &tr);
+        if (rc == PG_OK) {
+            hipError_t e = launch_fill_random(j->field, j->seed, tr->ptr, j->rows * j->cols, 1, s);
+            if (e != hipSuccess) rc = fail(PG_E_DEVICE, "fill: %s", hipGetErrorString(e));
+        }
+    }
+    pg_buf* lde = nullptr;
+    std::shared_ptr<DevMem> scratch;
+    if (rc == PG_OK) rc = lde_on_stream(ctx, s, j->field, tr, j->lb, gen_monty(j->field), 1, &lde, &scratch);
+    scratch.reset();
+    if (tr) pg_buf_free(tr);
+    j->trace = nullptr;
+    if (rc == PG_OK) {
+        rc = mmcs_commit_on_stream(ctx, s, j->field, {lde}, {true}, j->root, &j->tree);
+        if (rc != PG_OK) pg_buf_free(lde);
+    }
+    j->rc = rc;
+    if (rc != PG_OK) j->err = g_err;
+}
+
+static void svc_worker(pg_service* sv, int wi) {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        SvcJob* pick = nullptr;
+        while (!sv->stop) {
+            // first fit over the queue; once the oldest job has been passed
+            // over 8 times nothing else is admitted until it fits
+            bool rescan = false;
+            for (auto it = sv->pending.begin(); it != sv->pending.end(); ++it) {
+                SvcJob* j = sv->jobs[*it].get();
+                if (j->footprint > sv->budget) {  // can never fit: fail it
+                    j->rc = fail(PG_E_ADMISSION, "job needs %zu bytes, budget %zu", j->footprint, sv->budget);
+                    j->err = g_err;
+                    j->state = 2;
+                    j->t_done = now_s();
+                    j->footprint = 0;
+                    sv->pending.erase(it);
+                    sv->cv_done.notify_all();
+                    rescan = true;
+                    break;
+                }
+                if (sv->reserved + j->footprint <= sv->budget) {
+                    pick = j;
+                    sv->pending.erase(it);
+                    break;
+                }
+                if (it == sv->pending.begin() && ++j->skipped > 8) break;
+            }
+            if (pick) break;
+            if (!rescan) sv->cv_work.wait(lk);
+        }
+        if (!pick) return;  // stopping
+        sv->reserved += pick->footprint;
+        pick->state = 1;
+        pick->t_start = now_s();
+        sv->running++;
+        sv->peak_running = std::max(sv->peak_running, sv->running);
+        lk.unlock();
+        run_svc_job(sv, pick, sv->streams[wi]);
+        lk.lock();
+        sv->running--;
+        pick->state = 2;
+        pick->t_done = now_s();
+        if (pick->rc != PG_OK) {  // nothing stays resident for a failed job
+            sv->reserved -= pick->footprint;
+            pick->footprint = 0;
+            sv->cv_work.notify_all();
+        }
+        sv->cv_done.notify_all();
+    }
+}
+
+extern "C" int pg_service_create(pg_ctx* ctx, size_t budget, int workers, pg_service** out) {
+    if (!ctx || !out || workers < 1 || workers > 16) return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    PG_TRY(bind_device(ctx));
+    std::unique_ptr<pg_service> sv(new pg_service);
+    sv->ctx = ctx;
+    sv->budget = budget ? budget : ctx->budget;
+    for (int i = 0; i < workers; i++) {
+        hipStream_t s;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        sv->streams.push_back(s);
+    }
+    pg_service* p = sv.get();
+    for (int i = 0; i < workers; i++) sv->workers.emplace_back(svc_worker, p, i);
+    *out = sv.release();
+    return PG_OK;
+}
+
+static int svc_enqueue(pg_service* sv, std::unique_ptr<SvcJob> j, uint64_t* id_out) {
+    std::lock_guard<std::mutex> g(sv->mu);
+    j->id = sv->next_id++;
+    j->t_submit = now_s();
+    *id_out = j->id;
+    sv->pending.push_back(j->id);
+    sv->jobs[j->id] = std::move(j);
+    sv->cv_work.notify_one();
+    return PG_OK;
+}
+
+extern "C" int pg_service_submit(pg_service* sv, int field, pg_buf* trace, uint32_t log_blowup, uint64_t* id_out) {
+    if (!sv || !trace || !id_out || !field_ok(field) || !is_pow2(trace->rows) || trace->cols == 0)
+        return fail(PG_E_INVALID, "bad argument");
+    if (log2_of(trace->rows) + (int)log_blowup > two_adicity(field)) return fail(PG_E_INVALID, "LDE too large");
+    std::unique_ptr<SvcJob> j(new SvcJob);
+    j->field = field;
+    j->trace = trace;  // the service owns it from here
+    j->rows = trace->rows;
+    j->cols = trace->cols;
+    j->lb = (int)log_blowup;
+    j->footprint = job_footprint(j->rows, j->cols, j->lb, true);
+    return svc_enqueue(sv, std::move(j), id_out);
+}
+
+extern "C" int pg_service_submit_synthetic(pg_service* sv, int field, uint64_t seed, size_t rows, size_t cols,
+                                           uint32_t log_blowup, uint64_t* id_out) {
+    if (!sv || !id_out || !field_ok(field) || !is_pow2(rows) || cols == 0) return fail(PG_E_INVALID, "bad argument");
+    if (log2_of(rows) + (int)log_blowup > two_adicity(field)) return fail(PG_E_INVALID, "LDE too large");
+    std::unique_ptr<SvcJob> j(new SvcJob);
+    j->field = field;
+    j->seed = seed;
+    j->rows = rows;
+    j->cols = cols;
+    j->lb = (int)log_blowup;
+    j->footprint = job_footprint(rows, cols, j->lb, true);
+    return svc_enqueue(sv, std::move(j), id_out);
+}
+
+extern "C" int pg_service_wait(pg_service* sv, uint64_t id, uint32_t root[8], double times[3]) {
+    if (!sv) return fail(PG_E_INVALID, "null service");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->jobs.find(id);
+    if (it == sv->jobs.end()) return fail(PG_E_INVALID, "unknown job %llu", (unsigned long long)id);
+    SvcJob* j = it->second.get();
+    sv->cv_done.wait(lk, [&] { return j->state == 2; });
+    if (times) {
+        times[0] = j->t_submit;
+        times[1] = j->t_start;
+        times[2] = j->t_done;
+    }
+    if (j->rc != PG_OK) {
+        g_err = j->err;
+        return j->rc;
+    }
+    if (root) memcpy(root, j->root, 32);
+    return PG_OK;
+}
+
+extern "C" int pg_service_open(pg_service* sv, uint64_t id, const uint64_t* idx, size_t nq, uint32_t* rows_out,
+                               uint32_t* paths_out) {
+    if (!sv) return fail(PG_E_INVALID, "null service");
+    pg_tree* t = nullptr;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        auto it = sv->jobs.find(id);
+        if (it == sv->jobs.end() || it->second->state != 2 || !it->second->tree)
+            return fail(PG_E_INVALID, "job %llu has no resident prover data", (unsigned long long)id);
+        t = it->second->tree;
+    }
+    return pg_mmcs_open_batch(sv->ctx, t, idx, nq, rows_out, paths_out);
+}
+
+extern "C" int pg_service_release(pg_service* sv, uint64_t id) {
+    if (!sv) return fail(PG_E_INVALID, "null service");
+    std::unique_ptr<SvcJob> j;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        auto it = sv->jobs.find(id);
+        if (it == sv->jobs.end()) return fail(PG_E_INVALID, "unknown job %llu", (unsigned long long)id);
+        if (it->second->state != 2) return fail(PG_E_INVALID, "job %llu is not finished", (unsigned long long)id);
+        j = std::move(it->second);
+        sv->jobs.erase(it);
+    }
+    if (j->tree) pg_tree_free(j->tree);
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->reserved -= j->footprint;
+    sv->cv_work.notify_all();
+    return PG_OK;
+}
+
+extern "C" int pg_service_stats(pg_service* sv, size_t* reserved, size_t* pending, size_t* peak_running) {
+    if (!sv) return fail(PG_E_INVALID, "null service");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (reserved) *reserved = sv->reserved;
+    if (pending) *pending = sv->pending.size();
+    if (peak_running) *peak_running = sv->peak_running;
+    return PG_OK;
+}
+
+extern "C" void pg_service_destroy(pg_service* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    for (auto& t : sv->workers) t.join();
+    (void)hipSetDevice(sv->ctx->device);
+    for (auto& kv : sv->jobs) {
+        if (kv.second->tree) pg_tree_free(kv.second->tree);
+        if (kv.second->trace) pg_buf_free(kv.second->trace);
+    }
+    for (auto s : sv->streams) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    delete sv;
 }
 
 // ------------------------------------------------------------ challenger --

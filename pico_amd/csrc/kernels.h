@@ -44,7 +44,54 @@ hipError_t launch_open_gather(const MatRef* mats, const uint32_t* shift, const u
                               int nq, uint32_t* rows_out, uint32_t* paths_out,
                               hipStream_t stream);
 
+// ---- batched Merkle (merkle.hip): many single-matrix commits per launch ---
+// Jobs are found by binary search of the thread's global row / node index in
+// a prefix array (njobs + 1 entries, prefix[0] = 0).
+struct LeafJob {
+    const uint32_t* data;  // rows x width, row stride `stride`
+    uint64_t stride;
+    uint32_t width;
+    uint32_t pad_;
+    uint32_t* digests;  // rows x 8
+};
+struct CompressJob {
+    const uint32_t* prev;  // 2 n x 8
+    uint32_t* next;        // n x 8
+};
+hipError_t launch_leaf_hash_batch(int field, const LeafJob* jobs, const uint64_t* row_prefix, int njobs,
+                                  uint64_t total_rows, hipStream_t stream);
+hipError_t launch_compress_batch(int field, const CompressJob* jobs, const uint64_t* node_prefix,
+                                 int njobs, uint64_t total_nodes, hipStream_t stream);
+// dst[8 j + k] = src[j][k] for j < n (root digests of n trees)
+hipError_t launch_gather_roots(const uint32_t* const* src, uint32_t* dst, int n, hipStream_t stream);
+
 // ---- NTT / LDE (ntt.hip) --------------------------------------------------
+// One LDE job as the kernels see it (see ntt.hip for the pass structure).
+struct LdeArgs {
+    const uint32_t* in;    // h x w input, row stride ld_in
+    uint64_t ld_in;
+    uint32_t* out;         // N x w output (also pass-A scratch)
+    uint32_t* tmp;         // N x w scratch (multi-pass only)
+    const uint32_t* tw;    // w_h^e, e < h
+    const uint32_t* scale; // (shift_c)^k / h, [c h + k]
+    uint64_t ntiles;       // tiles of this job in the current pass
+    uint32_t w;
+    int L, k1, k2, b, bitrev;
+};
+enum LdeKind { KIND_SINGLE = 0, KIND_A = 1, KIND_BC = 2, KIND_D = 3 };
+// Split of a size-2^L transform: L <= 11 is one pass (k1 = 0, k2 = L);
+// otherwise k1 = ceil(L/2) strided bits, k2 = L - k1 block bits.
+void lde_split(int L, int* k1, int* k2);
+// Columns per tile of the kernel that runs pass `kind` with tile size 2^log_s.
+int lde_pass_cols(int kind, int log_s);
+// Row tiles (before the column-block factor) of pass `kind` for one job.
+uint64_t lde_pass_row_tiles(int kind, int L, int b);
+// One pass over a device table of jobs (all with the same L); prefix holds
+// each job's first tile (row tiles x column blocks), prefix[njobs] = total;
+// max_b = the largest added_bits among the jobs.
+hipError_t launch_lde_pass_batch(int field, int kind, int log_s, const LdeArgs* jobs,
+                                 const uint64_t* prefix, int njobs, uint64_t total_tiles, int max_b,
+                                 hipStream_t stream, KernelTimer* kt = nullptr);
 // tw[e] = w_{2^log_n}^e, e < 2^log_n (Montgomery)
 hipError_t launch_twiddles(int field, int log_n, uint32_t* tw, hipStream_t stream);
 // scale[c*h + k] = (shift * w_N^{rev_b(c)})^k / h for c < 2^b, k < h

@@ -130,7 +130,112 @@ __global__ void open_gather_kernel(const MatRef* mats, const uint32_t* shift,
     }
 }
 
+// ---- batched single-matrix commits --------------------------------------
+__device__ __forceinline__ int find_job(const uint64_t* __restrict__ prefix, int njobs, uint64_t i) {
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Leaf layer of many jobs: thread -> (job, row); rows of any width.
+template <class F>
+__global__ void __launch_bounds__(256) leaf_hash_batch_kernel(const LeafJob* __restrict__ jobs,
+                                                              const uint64_t* __restrict__ prefix,
+                                                              int njobs, uint64_t total) {
+    const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total) return;
+    const int j = find_job(prefix, njobs, gi);
+    const LeafJob jb = jobs[j];
+    const uint64_t row = gi - prefix[j];
+    const uint32_t* rp = jb.data + row * jb.stride;
+    uint32_t st[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) st[t] = 0;
+    const uint32_t full = jb.width / 8, tail = jb.width % 8;
+    if ((jb.width & 3u) == 0 && (jb.stride & 3u) == 0) {  // 16-byte aligned rows
+        const uint4* r4 = reinterpret_cast<const uint4*>(rp);
+        for (uint32_t c = 0; c < full; c++) {
+            const uint4 a = r4[2 * c], b = r4[2 * c + 1];
+            st[0] = a.x; st[1] = a.y; st[2] = a.z; st[3] = a.w;
+            st[4] = b.x; st[5] = b.y; st[6] = b.z; st[7] = b.w;
+            permute<F>(st);
+        }
+    } else {
+        for (uint32_t c = 0; c < full; c++) {
+#pragma unroll
+            for (int t = 0; t < 8; t++) st[t] = rp[8 * c + t];
+            permute<F>(st);
+        }
+    }
+    if (tail) {  // overwrite only the first `tail` state words
+        const uint32_t* tp = rp + 8 * full;
+#pragma unroll
+        for (int t = 0; t < 7; t++)
+            if ((uint32_t)t < tail) st[t] = tp[t];
+        permute<F>(st);
+    }
+    uint4* o = reinterpret_cast<uint4*>(jb.digests + 8 * row);
+    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
+    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) compress_batch_kernel(const CompressJob* __restrict__ jobs,
+                                                             const uint64_t* __restrict__ prefix,
+                                                             int njobs, uint64_t total) {
+    const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total) return;
+    const int j = find_job(prefix, njobs, gi);
+    const CompressJob jb = jobs[j];
+    const uint64_t i = gi - prefix[j];
+    const uint4* p = reinterpret_cast<const uint4*>(jb.prev + 16 * i);
+    uint32_t st[16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = p[q];
+        st[4 * q] = v.x; st[4 * q + 1] = v.y; st[4 * q + 2] = v.z; st[4 * q + 3] = v.w;
+    }
+    permute<F>(st);
+    uint4* o = reinterpret_cast<uint4*>(jb.next + 8 * i);
+    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
+    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+__global__ void gather_roots_kernel(const uint32_t* const* __restrict__ src, uint32_t* __restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * 8) dst[i] = src[i >> 3][i & 7];
+}
+
 static unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+hipError_t launch_gather_roots(const uint32_t* const* src, uint32_t* dst, int n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    gather_roots_kernel<<<grid_for((uint64_t)n * 8, 256), 256, 0, stream>>>(src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_hash_batch(int field, const LeafJob* jobs, const uint64_t* prefix, int njobs,
+                                  uint64_t total, hipStream_t stream) {
+    if (total == 0) return hipSuccess;
+    if (field == FIELD_KOALABEAR)
+        leaf_hash_batch_kernel<KoalaBear><<<grid_for(total, 256), 256, 0, stream>>>(jobs, prefix, njobs, total);
+    else
+        leaf_hash_batch_kernel<BabyBear><<<grid_for(total, 256), 256, 0, stream>>>(jobs, prefix, njobs, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_compress_batch(int field, const CompressJob* jobs, const uint64_t* prefix, int njobs,
+                                 uint64_t total, hipStream_t stream) {
+    if (total == 0) return hipSuccess;
+    if (field == FIELD_KOALABEAR)
+        compress_batch_kernel<KoalaBear><<<grid_for(total, 256), 256, 0, stream>>>(jobs, prefix, njobs, total);
+    else
+        compress_batch_kernel<BabyBear><<<grid_for(total, 256), 256, 0, stream>>>(jobs, prefix, njobs, total);
+    return hipGetLastError();
+}
 
 hipError_t launch_leaf_hash(int field, const MatRef* mats, int n, uint32_t total_width,
                             size_t rows, uint32_t* digests, hipStream_t stream) {

@@ -174,20 +174,6 @@ __device__ __forceinline__ uint32_t dit_out_index(int p, int r) {
     return (uint32_t)(p * (E / P) + r / P + E * (r % P));
 }
 
-struct LdeArgs {
-    const uint32_t* in;   // h x w input, row stride ld_in
-    uint64_t ld_in;
-    uint32_t* out;        // N x w output (also pass-A scratch)
-    uint32_t* tmp;        // N x w scratch (multi-pass only)
-    const uint32_t* tw;   // w_h^e, e < h
-    const uint32_t* scale;// (shift_c)^k / h, [c h + k]
-    uint64_t ntiles;
-    uint32_t w;
-    int L, k1, k2, b, bitrev;
-};
-
-enum { KIND_SINGLE = 0, KIND_A = 1, KIND_BC = 2, KIND_D = 3 };
-
 // Columns per tile: 16 (64-byte row runs), 8 for the largest tile when it
 // also parks coefficients in LDS (exchange + coefficient buffers <= 128 KiB).
 constexpr int cols_for(int log_e, int log_p, int kind) {
@@ -218,8 +204,11 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
     return x * q + (x < r ? x : r) + y;
 }
 
+// njobs == 0: one job, `a0`, tiles counted from 0.  Otherwise the tile's job
+// is found in the device table: prefix[j] <= tile < prefix[j + 1].
 template <class F, int LOG_E, int LOG_P, int KIND>
-__global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND)) lde_kernel(LdeArgs a) {
+__global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
+    lde_kernel(LdeArgs a0, const LdeArgs* __restrict__ jobs, const uint64_t* __restrict__ prefix, int njobs) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P;
     constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
     constexpr int NT = threads_for(LOG_E, LOG_P, KIND);
@@ -229,7 +218,23 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND)) lde_kernel(Ld
     const int tid = threadIdx.x;
     const int c = tid % C, p = (tid / C) % P, g = tid / (C * P);
     uint32_t* lds = lds_all + (LOG_P > 0 ? g * S * C : 0);
-    const uint64_t tile = (uint64_t)xcd_logical_block() * G + g;
+    uint64_t tile = (uint64_t)xcd_logical_block() * G + g;
+    LdeArgs a = a0;
+    if (njobs > 0) {
+        int lo = 0, hi = njobs;
+        const uint64_t total = prefix[njobs];
+        const uint64_t t = tile < total ? tile : total - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (prefix[mid] <= t) lo = mid; else hi = mid;
+        }
+        // a wave spans one tile group when a group has >= 64 threads: the job
+        // is then wave-uniform and its fields can live in scalar registers
+        if constexpr (C * P >= 64) lo = __builtin_amdgcn_readfirstlane(lo);
+        a = jobs[lo];
+        a.ntiles = tile < total ? prefix[lo + 1] - prefix[lo] : 0;
+        tile -= prefix[lo];
+    }
     const bool valid = tile < a.ntiles;
     const uint32_t h = 1u << a.L, mask = h - 1;
     const uint32_t w = a.w;
@@ -262,14 +267,19 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND)) lde_kernel(Ld
                 keep[t] = x[t];
         }
         const uint32_t rblk = rev_dev(blk, a.k1);
-        const uint32_t ncos = 1u << a.b;
+        // Every tile group of a workgroup must run the same number of
+        // barriers: batched launches loop over the pass-wide maximum coset
+        // count (a0.b) and mask the cosets a job does not have.
+        const uint32_t ncos_job = 1u << a.b;
+        const uint32_t ncos = njobs > 0 ? (1u << a0.b) : ncos_job;
         for (uint32_t cs = 0; cs < ncos; cs++) {
-            const uint32_t* sc = a.scale + (uint64_t)cs * h;
+            const bool live = cs < ncos_job;
+            const uint32_t* sc = a.scale + (uint64_t)(live ? cs : 0u) * h;
             // Opaque copy of the twiddle pointer: keeps the (coset-invariant)
             // twiddle loads inside the loop instead of hoisting ~64 of them
             // into VGPRs for all cosets, which spills.
             const uint32_t* twl = a.tw;
-            asm volatile("" : "+s"(twl));
+            asm volatile("" : "+v"(twl));
             // Likewise re-derive the lane coordinates per coset so the 64-bit
             // gather/store addresses are not all precomputed outside the loop.
             int pl = p;
@@ -281,7 +291,7 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND)) lde_kernel(Ld
                 x[t] = fmul<F>(v, sc[rev_dev(n_lo * blk + pl * E + t, a.L)]);
             }
             tile_dit<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask);
-            if (act) {
+            if (act && live) {
 #pragma unroll
                 for (int r = 0; r < E; r++) {
                     const uint32_t j = dit_out_index<LOG_E, LOG_P>(pl, r);
@@ -378,21 +388,37 @@ int lde_max_log_h() { return 22; }
 
 static const char* const KIND_NAMES[4] = {"lde_single", "lde_pass_a", "lde_pass_bc", "lde_pass_d"};
 
+// A pass launch: one job by value (njobs == 0, a0.ntiles = row tiles) or a
+// device table of jobs with tile prefix sums.
+struct PassLaunch {
+    LdeArgs a0;
+    const LdeArgs* jobs;
+    const uint64_t* prefix;
+    int njobs;
+    uint64_t total_tiles;
+};
+
 template <class F, int LOG_E, int LOG_P, int KIND>
-static void run(const LdeArgs& a, hipStream_t s, KernelTimer* kt) {
+static void run(const PassLaunch& pl, hipStream_t s, KernelTimer* kt) {
     constexpr int G = groups_for(LOG_E, LOG_P, KIND);
     constexpr int C = cols_for(LOG_E, LOG_P, KIND);
-    LdeArgs b = a;
-    b.ntiles = a.ntiles * ((a.w + C - 1) / C);  // callers count row tiles
-    const unsigned grid = (unsigned)((b.ntiles + G - 1) / G);
+    LdeArgs a = pl.a0;
+    uint64_t total = pl.total_tiles;
+    if (pl.njobs == 0) {
+        a.ntiles = a.ntiles * ((a.w + C - 1) / C);  // row tiles x column blocks
+        total = a.ntiles;
+    }
+    if (total == 0) return;
+    const unsigned grid = (unsigned)((total + G - 1) / G);
     if (kt) kt->begin(KIND_NAMES[KIND], s);
-    lde_kernel<F, LOG_E, LOG_P, KIND><<<grid, threads_for(LOG_E, LOG_P, KIND), 0, s>>>(b);
+    lde_kernel<F, LOG_E, LOG_P, KIND><<<grid, threads_for(LOG_E, LOG_P, KIND), 0, s>>>(a, pl.jobs, pl.prefix,
+                                                                                        pl.njobs);
     if (kt) kt->end(s);
 }
 
 // tile shapes (LOG_E, LOG_P) per log S, with E >= P
 template <class F, int KIND>
-static bool run_logs(int log_s, const LdeArgs& a, hipStream_t s, KernelTimer* kt) {
+static bool run_logs(int log_s, const PassLaunch& a, hipStream_t s, KernelTimer* kt) {
     switch (log_s) {
         case 0: if constexpr (KIND == KIND_SINGLE) { run<F, 0, 0, KIND>(a, s, kt); return true; } break;
         case 1: if constexpr (KIND == KIND_SINGLE) { run<F, 1, 0, KIND>(a, s, kt); return true; } break;
@@ -411,6 +437,47 @@ static bool run_logs(int log_s, const LdeArgs& a, hipStream_t s, KernelTimer* kt
     return false;
 }
 
+void lde_split(int L, int* k1, int* k2) {
+    *k1 = L <= 11 ? 0 : (L + 1) / 2;
+    *k2 = L - *k1;
+}
+
+int lde_pass_cols(int kind, int log_s) {
+    return (log_s >= 11 && (kind == KIND_BC || kind == KIND_SINGLE)) ? 8 : 16;
+}
+
+uint64_t lde_pass_row_tiles(int kind, int L, int b) {
+    int k1, k2;
+    lde_split(L, &k1, &k2);
+    switch (kind) {
+        case KIND_SINGLE: return 1;
+        case KIND_A: return 1ull << k2;
+        case KIND_BC: return 1ull << k1;
+        default: return (1ull << k2) << b;
+    }
+}
+
+template <class F>
+static bool pass_t(int kind, int log_s, const PassLaunch& pl, hipStream_t s, KernelTimer* kt) {
+    switch (kind) {
+        case KIND_SINGLE: return run_logs<F, KIND_SINGLE>(log_s, pl, s, kt);
+        case KIND_A: return run_logs<F, KIND_A>(log_s, pl, s, kt);
+        case KIND_BC: return run_logs<F, KIND_BC>(log_s, pl, s, kt);
+        default: return run_logs<F, KIND_D>(log_s, pl, s, kt);
+    }
+}
+
+hipError_t launch_lde_pass_batch(int field, int kind, int log_s, const LdeArgs* jobs, const uint64_t* prefix,
+                                 int njobs, uint64_t total_tiles, int max_b, hipStream_t s, KernelTimer* kt) {
+    if (njobs <= 0) return hipErrorInvalidValue;
+    LdeArgs a0{};
+    a0.b = max_b;  // coset-loop bound shared by every tile group
+    const PassLaunch pl{a0, jobs, prefix, njobs, total_tiles};
+    const bool ok = field == FIELD_KOALABEAR ? pass_t<KoalaBear>(kind, log_s, pl, s, kt)
+                                             : pass_t<BabyBear>(kind, log_s, pl, s, kt);
+    return ok ? hipGetLastError() : hipErrorInvalidValue;
+}
+
 template <class F>
 static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t w, int b,
                               const uint32_t* tw, const uint32_t* scale, uint32_t* tmp, uint32_t* out,
@@ -426,21 +493,15 @@ static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t 
     a.L = L;
     a.b = b;
     a.bitrev = bitrev;
-    if (L <= 11) {
-        a.k1 = 0;
-        a.k2 = L;
-        a.ntiles = 1;
-        if (!run_logs<F, KIND_SINGLE>(L, a, s, kt)) return hipErrorInvalidValue;
-        return hipGetLastError();
+    lde_split(L, &a.k1, &a.k2);
+    const int kinds[3] = {KIND_A, KIND_BC, KIND_D};
+    for (int i = 0; i < (L <= 11 ? 1 : 3); i++) {
+        const int kind = L <= 11 ? KIND_SINGLE : kinds[i];
+        a.ntiles = lde_pass_row_tiles(kind, L, b);
+        const int log_s = kind == KIND_BC || kind == KIND_SINGLE ? a.k2 : a.k1;
+        const PassLaunch pl{a, nullptr, nullptr, 0, 0};
+        if (!pass_t<F>(kind, log_s, pl, s, kt)) return hipErrorInvalidValue;
     }
-    a.k1 = (L + 1) / 2;
-    a.k2 = L - a.k1;
-    a.ntiles = 1ull << a.k2;
-    if (!run_logs<F, KIND_A>(a.k1, a, s, kt)) return hipErrorInvalidValue;
-    a.ntiles = 1ull << a.k1;
-    if (!run_logs<F, KIND_BC>(a.k2, a, s, kt)) return hipErrorInvalidValue;
-    a.ntiles = (1ull << a.k2) << b;
-    if (!run_logs<F, KIND_D>(a.k1, a, s, kt)) return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
