@@ -96,6 +96,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=32)
     ap.add_argument("--cpu-leaf-rows", type=int, default=1 << 15)
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the shard+FRI, batched-jobs and served-stream sections")
+    ap.add_argument("--served-jobs", type=int, default=32)
+    ap.add_argument("--rates", type=str, default="5,20,50,200")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -203,6 +207,16 @@ def main():
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
         "root": [int(x) for x in root],
     }
+    trace.free()
+    if not args.no_workloads:
+        from pico_amd import workloads
+        workloads.shard_commit_fri(ctx)  # warm tables
+        out["shard_commit_fri"] = workloads.shard_commit_fri(ctx)
+        out["batched_jobs"] = workloads.batched_jobs(ctx)
+        jobs = workloads.served_jobs(n=args.served_jobs)
+        served = {"jobs": [list(j) for j in jobs], "sequential": workloads.served_sequential(ctx, jobs)}
+        served["service"] = [workloads.served_stream(ctx, float(r), jobs) for r in args.rates.split(",")]
+        out["served_stream"] = served
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.log_h, w, lb, args.cpu_cols, args.cpu_leaf_rows)
     ctx.close()

@@ -118,6 +118,19 @@ __device__ __forceinline__ uint32_t tw_at(const uint32_t* __restrict__ tw, uint3
     return tw[INV ? ((0u - e) & mask) : e];
 }
 
+// LDS word of (tile position, column).  A wave's lanes are C columns x
+// (64 / C) values of p; positions strided by E p would all map to the same
+// banks (E C is a multiple of 64), so C extra words follow every E positions:
+// lanes with different p then land C banks apart.
+template <int LOG_E, int C>
+__device__ __forceinline__ int lds_at(int pos, int c) {
+    return pos * C + c + (pos >> LOG_E) * C;
+}
+template <int LOG_E, int LOG_P, int C>
+constexpr int lds_words() {
+    return ((1 << (LOG_E + LOG_P)) + (1 << LOG_P)) * C;
+}
+
 // Tile DIF.  In: x[m] = tile position p + P m.  Out: x[t] = tile position
 // p E + t, holding DFT output index rev_S(p E + t).
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
@@ -132,12 +145,12 @@ __device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* ld
             x[m] = fmul<F>(x[m], tw_at<INV>(tw, e, mask));
         }
 #pragma unroll
-        for (int m = 0; m < E; m++) lds[(p + P * m) * C + c] = x[m];
+        for (int m = 0; m < E; m++) lds[lds_at<LOG_E, C>(p + P * m, c)] = x[m];
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < E / P; u++)
 #pragma unroll
-            for (int i = 0; i < P; i++) x[u * P + i] = lds[(P * (p * (E / P) + u) + i) * C + c];
+            for (int i = 0; i < P; i++) x[u * P + i] = lds[lds_at<LOG_E, C>(P * (p * (E / P) + u) + i, c)];
 #pragma unroll
         for (int u = 0; u < E / P; u++) reg_dif<F, LOG_P, INV>(&x[u * P]);
         __syncthreads();
@@ -156,12 +169,12 @@ __device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* ld
 #pragma unroll
         for (int j = 1; j < E; j++) x[j] = fmul<F>(x[j], tw_at<INV>(tw, ((uint32_t)j * rp) << tw_sh, mask));
 #pragma unroll
-        for (int j = 0; j < E; j++) lds[(E * p + j) * C + c] = x[j];
+        for (int j = 0; j < E; j++) lds[lds_at<LOG_E, C>(E * p + j, c)] = x[j];
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < E / P; u++)
 #pragma unroll
-            for (int a = 0; a < P; a++) x[u * P + a] = lds[(p * (E / P) + u + E * a) * C + c];
+            for (int a = 0; a < P; a++) x[u * P + a] = lds[lds_at<LOG_E, C>(p * (E / P) + u + E * a, c)];
 #pragma unroll
         for (int u = 0; u < E / P; u++) reg_dit<F, LOG_P, INV>(&x[u * P]);
         __syncthreads();
@@ -213,11 +226,12 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
     constexpr int NT = threads_for(LOG_E, LOG_P, KIND);
     constexpr bool PARK = parks_coefficients(KIND) && E >= 16;
-    __shared__ uint32_t lds_all[LOG_P > 0 ? G * S * C : 1];
+    constexpr int LW = lds_words<LOG_E, LOG_P, C>();  // exchange words per tile group
+    __shared__ uint32_t lds_all[LOG_P > 0 ? G * LW : 1];
     __shared__ uint32_t lds_coef[PARK ? E * NT : 1];
     const int tid = threadIdx.x;
     const int c = tid % C, p = (tid / C) % P, g = tid / (C * P);
-    uint32_t* lds = lds_all + (LOG_P > 0 ? g * S * C : 0);
+    uint32_t* lds = lds_all + (LOG_P > 0 ? g * LW : 0);
     uint64_t tile = (uint64_t)xcd_logical_block() * G + g;
     LdeArgs a = a0;
     if (njobs > 0) {

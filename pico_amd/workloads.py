@@ -1,0 +1,239 @@
+"""Job streams for the shard-commit workloads, drawn deterministically.
+
+Shapes come from SplitMix64 seeded with (workload_id << 32) | 0; traces are
+the device-side seeded inputs of pg_fill_random.  Each workload is run on the
+device through the C ABI and reports wall time from the host clock.
+
+  shard_commit_fri   one shard: 14 matrices with log-heights in 14..22 and
+                     widths in {8,16,32,64,128,256,300}, one mixed-height
+                     commit at blowup 2; a quotient-domain commit at blowup 8;
+                     FRI on a 2^22 ext4 vector down to 2^6 with a commit per
+                     round, 16-bit proof of work and 100 query openings
+  batched_jobs       64 single-matrix commits, log-heights 10..18, widths
+                     8..512, blowup 2/4/8: one launch sequence (hipGraph) vs
+                     the same jobs committed one by one
+  served_stream      Poisson arrivals of commit jobs (log-heights 16..22,
+                     widths 16..512, KoalaBear/BabyBear, blowup 2 or 8), 50
+                     openings per job, HBM-budgeted service vs sequential
+"""
+import math
+import threading
+import time
+
+import numpy as np
+
+from . import core
+
+GOLDEN = 0x9E3779B97F4A7C15
+MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & MASK
+
+    def next(self):
+        self.s = (self.s + GOLDEN) & MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+    def uniform_int(self, lo, hi):  # inclusive
+        return lo + self.next() % (hi - lo + 1)
+
+    def uniform01(self):
+        return (self.next() >> 11) / float(1 << 53)
+
+
+def percentile(xs, q):
+    s = sorted(xs)
+    if not s:
+        return None
+    k = (len(s) - 1) * q / 100.0
+    lo, hi = math.floor(k), math.ceil(k)
+    return s[lo] + (s[hi] - s[lo]) * (k - lo)
+
+
+# ------------------------------------------------------------------ shard --
+def shard_shapes(seed_base=3 << 32, n=14, min_log=14, max_log=22):
+    r = SplitMix64(seed_base)
+    widths = [8, 16, 32, 64, 128, 256, 300]
+    return [(r.uniform_int(min_log, max_log), widths[r.next() % len(widths)]) for _ in range(n)]
+
+
+def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_bits=16,
+                     quotient_log_h=19, quotient_width=16):
+    """One shard's commits and FRI on the device; returns a timing dict."""
+    f = core.KOALABEAR
+    shapes = shapes or shard_shapes()
+    mats = [ctx.random_matrix(f, (3 << 32) | (100 + i), 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    q = ctx.random_matrix(f, (3 << 32) | 200, 1 << quotient_log_h, quotient_width)
+    v = ctx.random_matrix(f, (3 << 32) | 300, 1 << log_fri, 4)
+    core.device_synchronize()
+    cells = sum((1 << L) * w for L, w in shapes)
+    t = {}
+    t0 = time.perf_counter()
+    pcs = core.TwoAdicFriPcs(ctx, f, 1)
+    root, main = pcs.commit([(pcs.natural_domain_for_degree(m.rows), m) for m in mats])
+    t1 = time.perf_counter()
+    qpcs = core.TwoAdicFriPcs(ctx, f, 3)
+    p = core.PRIMES[f]
+    qshift = core.GENERATORS[f] * core.two_adic_generator(f, quotient_log_h + 3) % p
+    qroot, quot = qpcs.commit([(core.TwoAdicMultiplicativeCoset(quotient_log_h, qshift), q)])
+    t2 = time.perf_counter()
+    ch = core.DuplexChallenger(f)
+    ch.observe(root)
+    ch.observe(qroot)
+    fri = core.fri_commit_phase(ctx, f, ch, {log_fri: v}, log_stop, final_mode=1)
+    t3 = time.perf_counter()
+    witness = core.fri_grind(ctx, f, ch, pow_bits)
+    t4 = time.perf_counter()
+    idx = [ch.sample_bits(log_fri) for _ in range(queries)]
+    sib, paths = core.fri_answer_queries(fri, idx, log_fri)
+    mm = core.MerkleTreeMmcs(ctx, f)
+    main_idx = [i >> (log_fri - main.log_height) if main.log_height <= log_fri else i for i in idx]
+    mm.open_many(main_idx, main)
+    mm.open_many([i >> max(0, log_fri - quot.log_height) for i in idx], quot)
+    t5 = time.perf_counter()
+    t.update({
+        "cells": cells, "matrices": len(shapes), "shapes": shapes,
+        "main_commit_ms": (t1 - t0) * 1e3, "quotient_commit_ms": (t2 - t1) * 1e3,
+        "fri_commit_phase_ms": (t3 - t2) * 1e3, "pow_ms": (t4 - t3) * 1e3,
+        "queries_ms": (t5 - t4) * 1e3, "total_ms": (t5 - t0) * 1e3,
+        "fri_rounds": fri.num_rounds, "pow_witness": witness,
+    })
+    main.free()
+    quot.free()
+    fri.free()
+    return t
+
+
+# ---------------------------------------------------------------- batched --
+def batched_shapes(seed_base=4 << 32, n=64):
+    r = SplitMix64(seed_base)
+    return [(r.uniform_int(10, 18), r.uniform_int(8, 512), r.uniform_int(1, 3)) for _ in range(n)]
+
+
+def batched_jobs(ctx, shapes=None, reps=3):
+    """Batched (graph replay) vs one-by-one commits of the same jobs."""
+    f = core.KOALABEAR
+    shapes = shapes or batched_shapes()
+    mats = [ctx.random_matrix(f, (4 << 32) | (1000 + i), 1 << L, w) for i, (L, w, _) in enumerate(shapes)]
+    cells = sum((1 << L) * w for L, w, _ in shapes)
+
+    def one_by_one():
+        roots = []
+        for m, (L, w, lb) in zip(mats, shapes):
+            pcs = core.TwoAdicFriPcs(ctx, f, lb)
+            r, pd = pcs.commit([(pcs.natural_domain_for_degree(m.rows), m)])
+            pd.free()
+            roots.append(r)
+        return roots
+
+    seq_roots = one_by_one()  # warm
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one_by_one()
+    t_seq = (time.perf_counter() - t0) / reps
+    batch = core.CommitBatch(ctx, f, [(m, lb) for m, (_, _, lb) in zip(mats, shapes)], use_graph=True)
+    roots = batch.run()  # warm
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        roots = batch.run()
+    t_bat = (time.perf_counter() - t0) / reps
+    same = all(np.array_equal(a, b) for a, b in zip(roots, seq_roots))
+    out = {"jobs": len(shapes), "cells": cells, "launches": batch.num_launches,
+           "one_by_one_ms": t_seq * 1e3, "batched_ms": t_bat * 1e3,
+           "one_by_one_cells_per_s": cells / t_seq, "batched_cells_per_s": cells / t_bat,
+           "speedup": t_seq / t_bat, "roots_identical": bool(same)}
+    batch.free()
+    return out
+
+
+# ----------------------------------------------------------------- served --
+def served_jobs(seed_base=5 << 32, n=40):
+    """(field, log_h, width, log_blowup) per job.  A KoalaBear LDE has at most
+    2^24 rows (two-adicity 24), so KoalaBear jobs with log_h + b > 24 use b = 1."""
+    r = SplitMix64(seed_base)
+    jobs = []
+    for _ in range(n):
+        L = r.uniform_int(16, 22)
+        w = r.uniform_int(16, 512)
+        f = core.KOALABEAR if r.next() & 1 else core.BABYBEAR
+        b = 1 if r.next() & 1 else 3
+        if f == core.KOALABEAR and L + b > core.TWO_ADICITY[f]:
+            b = 1
+        jobs.append((f, L, w, b))
+    return jobs
+
+
+def served_stream(ctx, rate, jobs, budget=200 * 10 ** 9, workers=3, openings=50, seed=5):
+    """Poisson arrivals at `rate` jobs/s into the commit service; each job is
+    finished when its 50 openings have been answered.  Returns throughput and
+    latency percentiles (arrival -> last opening)."""
+    svc = core.CommitService(ctx, budget=budget, workers=workers)
+    rng = SplitMix64((5 << 32) | seed)
+    arrivals, t = [], 0.0
+    for _ in jobs:
+        t += -math.log(1.0 - rng.uniform01()) / rate
+        arrivals.append(t)
+    lat = [None] * len(jobs)
+    ids = [None] * len(jobs)
+    ready = threading.Semaphore(0)
+    start = time.perf_counter()
+
+    def submitter():
+        for k, (f, L, w, b) in enumerate(jobs):
+            delay = start + arrivals[k] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids[k] = svc.submit_synthetic(f, (5 << 32) | (10000 + k), 1 << L, w, b)
+            ready.release()
+
+    def finisher(k):
+        f, L, w, b = jobs[k]
+        svc.wait(ids[k], f)
+        q = np.random.default_rng(k).integers(0, 1 << (L + b), openings)
+        svc.open(ids[k], f, q, w, L + b)
+        lat[k] = time.perf_counter() - (start + arrivals[k])
+        svc.release(ids[k])
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    fins = []
+    for k in range(len(jobs)):
+        ready.acquire()
+        ft = threading.Thread(target=finisher, args=(k,))
+        ft.start()
+        fins.append(ft)
+    th.join()
+    for ft in fins:
+        ft.join()
+    wall = time.perf_counter() - start
+    stats = svc.stats()
+    svc.close()
+    cells = sum((1 << L) * w for _, L, w, _ in jobs)
+    return {"rate": rate, "jobs": len(jobs), "wall_s": wall, "jobs_per_s": len(jobs) / wall,
+            "cells_per_s": cells / wall, "p50_ms": percentile(lat, 50) * 1e3,
+            "p99_ms": percentile(lat, 99) * 1e3, "peak_running": stats["peak_running"]}
+
+
+def served_sequential(ctx, jobs, openings=50):
+    """The plain path: each job committed, opened 50 times and released before
+    the next starts (all jobs present at time 0)."""
+    start = time.perf_counter()
+    lat = []
+    for k, (f, L, w, b) in enumerate(jobs):
+        m = ctx.random_matrix(f, (5 << 32) | (10000 + k), 1 << L, w)
+        pcs = core.TwoAdicFriPcs(ctx, f, b)
+        _, pd = pcs.commit([(pcs.natural_domain_for_degree(1 << L), m)])
+        m.free()
+        q = np.random.default_rng(k).integers(0, 1 << (L + b), openings)
+        core.MerkleTreeMmcs(ctx, f).open_many(q, pd)
+        pd.free()
+        lat.append(time.perf_counter() - start)
+    wall = time.perf_counter() - start
+    cells = sum((1 << L) * w for _, L, w, _ in jobs)
+    return {"jobs": len(jobs), "wall_s": wall, "jobs_per_s": len(jobs) / wall, "cells_per_s": cells / wall,
+            "p50_ms": percentile(lat, 50) * 1e3, "p99_ms": percentile(lat, 99) * 1e3}
