@@ -91,11 +91,21 @@ struct pg_ctx {
     std::map<std::tuple<int, int, int, uint32_t>, std::shared_ptr<DevMem>> scales;
     std::vector<std::tuple<int, int, int, uint32_t>> scale_order;  // insertion order for eviction
     size_t scale_bytes = 0;
-    // freed device blocks kept for reuse, keyed by size
-    std::mutex cache_mu;
-    std::multimap<size_t, void*> cache;
-    size_t cached_bytes = 0;
-    size_t cache_limit = 0;
+    // device memory pool: segments from hipMalloc, carved into blocks
+    struct PoolSeg {
+        char* base;
+        size_t size;
+    };
+    struct FreeBlk {
+        size_t size;
+        int seg;
+    };
+    std::mutex pool_mu;
+    std::vector<PoolSeg> segs;                       // base == nullptr once released
+    std::map<char*, FreeBlk> free_by_addr;           // free blocks, address order
+    std::multimap<size_t, char*> free_by_size;       // the same blocks, size order
+    std::map<char*, std::pair<size_t, int>> live;    // block -> (size, segment)
+    size_t pool_bytes = 0;                           // sum of live segment sizes
     // per-kernel-family event timings (profiling mode only)
     std::atomic<bool> profiling{false};
     std::map<std::string, std::pair<double, uint64_t>> prof;
@@ -145,71 +155,140 @@ static std::unique_ptr<EventTimer> make_timer(pg_ctx* ctx) {
         HIP_TRY(te_);                       \
     } while (0)
 
-// Device memory: live bytes are charged against the budget; freed blocks of
-// 1 MiB or more are kept in a size-keyed cache (bounded) and handed out again
-// for the same size, so the repeated multi-GiB LDE / scratch / digest buffers
-// of a commit stream do not go through hipMalloc / hipFree every time.
-// Callers free a block only after the work using it has completed.
-static constexpr size_t CACHE_MIN_BLOCK = 1ull << 20;
+// Device memory: live bytes are charged against the budget.  Blocks are
+// carved from large hipMalloc segments (at least POOL_SEG_MIN) by best fit;
+// a freed block is merged with free neighbours of the same segment and
+// reused for any later request it can hold.  hipMalloc / hipFree of
+// multi-GiB buffers costs tens of ms per GiB on this device, so a stream of
+// commits of varying shapes would otherwise spend most of its time there.
+// Wholly free segments go back to the driver only when a new segment does
+// not fit, or when the context is destroyed.  Callers free a block only
+// after the work using it has completed.
+static constexpr size_t POOL_ALIGN = 512;
+static constexpr size_t POOL_SEG_MIN = 1ull << 30;
+static constexpr size_t POOL_SEG_ROUND = 2ull << 20;
 
-static void cache_trim(pg_ctx* ctx, size_t keep_bytes) {  // ctx->cache_mu held
-    while (ctx->cached_bytes > keep_bytes && !ctx->cache.empty()) {
-        auto it = ctx->cache.begin();
-        (void)hipFree(it->second);
-        ctx->cached_bytes -= it->first;
-        ctx->cache.erase(it);
+static size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+static void pool_erase_size(pg_ctx* c, size_t n, char* p) {  // pool_mu held
+    auto r = c->free_by_size.equal_range(n);
+    for (auto it = r.first; it != r.second; ++it)
+        if (it->second == p) {
+            c->free_by_size.erase(it);
+            return;
+        }
+}
+
+static void pool_put_free(pg_ctx* c, char* p, size_t n, int seg) {  // pool_mu held
+    auto next = c->free_by_addr.find(p + n);
+    if (next != c->free_by_addr.end() && next->second.seg == seg) {
+        n += next->second.size;
+        pool_erase_size(c, next->second.size, next->first);
+        c->free_by_addr.erase(next);
     }
+    auto after = c->free_by_addr.upper_bound(p);
+    if (after != c->free_by_addr.begin()) {
+        auto prev = std::prev(after);
+        if (prev->second.seg == seg && prev->first + prev->second.size == p) {
+            pool_erase_size(c, prev->second.size, prev->first);
+            p = prev->first;
+            n += prev->second.size;
+            c->free_by_addr.erase(prev);
+        }
+    }
+    c->free_by_addr.emplace(p, pg_ctx::FreeBlk{n, seg});
+    c->free_by_size.emplace(n, p);
+}
+
+static char* pool_take(pg_ctx* c, size_t n) {  // pool_mu held; n aligned
+    auto it = c->free_by_size.lower_bound(n);
+    if (it == c->free_by_size.end()) return nullptr;
+    char* p = it->second;
+    const size_t have = it->first;
+    c->free_by_size.erase(it);
+    auto fa = c->free_by_addr.find(p);
+    const int seg = fa->second.seg;
+    c->free_by_addr.erase(fa);
+    if (have > n) {
+        c->free_by_addr.emplace(p + n, pg_ctx::FreeBlk{have - n, seg});
+        c->free_by_size.emplace(have - n, p + n);
+    }
+    c->live.emplace(p, std::make_pair(n, seg));
+    return p;
+}
+
+static void pool_release_free_segments(pg_ctx* c) {  // pool_mu held
+    for (size_t k = 0; k < c->segs.size(); k++) {
+        auto& sg = c->segs[k];
+        if (!sg.base) continue;
+        auto it = c->free_by_addr.find(sg.base);
+        if (it == c->free_by_addr.end() || it->second.seg != (int)k || it->second.size != sg.size) continue;
+        pool_erase_size(c, sg.size, sg.base);
+        c->free_by_addr.erase(it);
+        (void)hipFree(sg.base);
+        c->pool_bytes -= sg.size;
+        sg.base = nullptr;
+        sg.size = 0;
+    }
+}
+
+static hipError_t pool_grow(pg_ctx* c, size_t n) {  // pool_mu held
+    const size_t need = round_up(n, POOL_SEG_ROUND);
+    size_t sz = std::max(need, c->budget ? std::min(POOL_SEG_MIN, round_up(c->budget, POOL_SEG_ROUND)) : POOL_SEG_MIN);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess && need < sz) {  // the minimum segment may not fit where n does
+        (void)hipGetLastError();
+        sz = need;
+        e = hipMalloc(&p, sz);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    c->segs.push_back({(char*)p, sz});
+    c->pool_bytes += sz;
+    pool_put_free(c, (char*)p, sz, (int)c->segs.size() - 1);
+    return hipSuccess;
 }
 
 static int ctx_alloc(pg_ctx* ctx, size_t bytes, void** out) {
     *out = nullptr;
-    if (bytes == 0) bytes = 4;
-    size_t prev = ctx->in_use.fetch_add(bytes);
-    if (ctx->budget && prev + bytes > ctx->budget) {
-        ctx->in_use.fetch_sub(bytes);
+    const size_t n = round_up(bytes ? bytes : 1, POOL_ALIGN);
+    size_t prev = ctx->in_use.fetch_add(n);
+    if (ctx->budget && prev + n > ctx->budget) {
+        ctx->in_use.fetch_sub(n);
         return fail(PG_E_ADMISSION, "allocation of %zu bytes exceeds the HBM budget (%zu of %zu in use)",
                     bytes, prev, ctx->budget);
     }
-    if (bytes >= CACHE_MIN_BLOCK) {
-        std::lock_guard<std::mutex> g(ctx->cache_mu);
-        auto it = ctx->cache.find(bytes);
-        if (it != ctx->cache.end()) {
-            *out = it->second;
-            ctx->cached_bytes -= bytes;
-            ctx->cache.erase(it);
-            return PG_OK;
+    std::lock_guard<std::mutex> g(ctx->pool_mu);
+    char* p = pool_take(ctx, n);
+    if (!p) {
+        hipError_t e = pool_grow(ctx, n);
+        if (e != hipSuccess) {  // hand wholly free segments back and retry once
+            pool_release_free_segments(ctx);
+            e = pool_grow(ctx, n);
         }
-    }
-    hipError_t e = hipMalloc(out, bytes);
-    if (e != hipSuccess) {  // release cached blocks and retry once
-        (void)hipGetLastError();
-        {
-            std::lock_guard<std::mutex> g(ctx->cache_mu);
-            cache_trim(ctx, 0);
+        if (e != hipSuccess) {
+            ctx->in_use.fetch_sub(n);
+            return fail(PG_E_OOM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
         }
-        e = hipMalloc(out, bytes);
+        p = pool_take(ctx, n);
     }
-    if (e != hipSuccess) {
-        ctx->in_use.fetch_sub(bytes);
-        (void)hipGetLastError();
-        *out = nullptr;
-        return fail(PG_E_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    }
+    *out = p;
     return PG_OK;
 }
 
-static void ctx_free(pg_ctx* ctx, void* p, size_t bytes) {
-    if (!p) return;
-    if (bytes == 0) bytes = 4;
-    ctx->in_use.fetch_sub(bytes);
-    if (bytes >= CACHE_MIN_BLOCK && bytes <= ctx->cache_limit) {
-        std::lock_guard<std::mutex> g(ctx->cache_mu);
-        ctx->cache.emplace(bytes, p);
-        ctx->cached_bytes += bytes;
-        cache_trim(ctx, ctx->cache_limit);
-        return;
-    }
-    (void)hipFree(p);
+static void ctx_free(pg_ctx* ctx, void* ptr, size_t) {
+    if (!ptr) return;
+    std::lock_guard<std::mutex> g(ctx->pool_mu);
+    auto it = ctx->live.find((char*)ptr);
+    if (it == ctx->live.end()) return;
+    const size_t n = it->second.first;
+    const int seg = it->second.second;
+    ctx->live.erase(it);
+    ctx->in_use.fetch_sub(n);
+    pool_put_free(ctx, (char*)ptr, n, seg);
 }
 
 static std::shared_ptr<DevMem> make_devmem(pg_ctx* ctx, size_t bytes, int* rc) {
@@ -253,7 +332,6 @@ extern "C" int pg_create(int device, size_t budget, pg_ctx** out) {
         budget = free_b;
     }
     c->budget = budget;
-    c->cache_limit = std::min<size_t>(budget / 4, 16ull << 30);
     for (int i = 0; i < 4; i++) {
         hipStream_t s;
         HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -270,8 +348,10 @@ extern "C" void pg_destroy(pg_ctx* ctx) {
     ctx->twiddles.clear();
     ctx->scales.clear();
     {
-        std::lock_guard<std::mutex> g(ctx->cache_mu);
-        cache_trim(ctx, 0);
+        std::lock_guard<std::mutex> g(ctx->pool_mu);
+        for (auto& sg : ctx->segs)
+            if (sg.base) (void)hipFree(sg.base);
+        ctx->segs.clear();
     }
     for (auto s : ctx->streams) (void)hipStreamDestroy(s);
     delete ctx;
@@ -508,8 +588,8 @@ static int lde_on_stream(pg_ctx* ctx, hipStream_t s, int field, const pg_buf* in
     }
     auto kt = make_timer(ctx);
     hipError_t e = launch_coset_lde(field, in->ptr, in->cols, L, (uint32_t)in->cols, (int)b,
-                                    (const uint32_t*)tw->ptr, (const uint32_t*)sc->ptr, tmp, o->ptr, bitrev, s,
-                                    kt.get());
+                                    (const uint32_t*)tw->ptr, (const uint32_t*)sc->ptr, tmp, o->ptr, in->cols,
+                                    bitrev, s, kt.get());
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // tables and scratch stay referenced until done
     if (e != hipSuccess) {
         pg_buf_free(o);
@@ -1003,6 +1083,7 @@ extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* eval
             a.in = evals[j]->ptr;
             a.ld_in = evals[j]->cols;
             a.out = b->trees[j]->mats[0]->ptr;
+            a.ld_out = W[j];
             a.tmp = scratch && Lg > 11 ? (uint32_t*)scratch->ptr + tmp_off[j] : nullptr;
             a.tw = (const uint32_t*)tw->ptr;
             a.scale = (const uint32_t*)sc->ptr;

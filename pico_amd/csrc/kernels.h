@@ -8,6 +8,16 @@
 
 namespace pg {
 
+// Global-memory pointers.  A pointer that reaches a kernel inside a struct or
+// a device table is generic to the compiler, which then emits FLAT accesses:
+// those pay the LDS-aperture check and count against the LDS wait counter, so
+// every LDS exchange would also wait for outstanding HBM loads.  Kernels cast
+// such pointers to address space 1 once.
+#define GLOBAL __attribute__((address_space(1)))
+typedef GLOBAL const uint32_t* gcptr;
+typedef GLOBAL uint32_t* gptr;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // copyable across address spaces
+
 // Optional per-launch timing hook: begin() before a kernel launch and end()
 // after it, on the launch stream (the implementation records HIP events).
 struct KernelTimer {
@@ -70,7 +80,8 @@ hipError_t launch_gather_roots(const uint32_t* const* src, uint32_t* dst, int n,
 struct LdeArgs {
     const uint32_t* in;    // h x w input, row stride ld_in
     uint64_t ld_in;
-    uint32_t* out;         // N x w output (also pass-A scratch)
+    uint32_t* out;         // N x w output, row stride ld_out (also pass-A scratch)
+    uint64_t ld_out;
     uint32_t* tmp;         // N x w scratch (multi-pass only)
     const uint32_t* tw;    // w_h^e, e < h
     const uint32_t* scale; // (shift_c)^k / h, [c h + k]
@@ -97,13 +108,14 @@ hipError_t launch_twiddles(int field, int log_n, uint32_t* tw, hipStream_t strea
 // scale[c*h + k] = (shift * w_N^{rev_b(c)})^k / h for c < 2^b, k < h
 hipError_t launch_coset_scale(int field, int log_h, int added_bits, uint32_t shift,
                               uint32_t* scale, hipStream_t stream);
-// Coset LDE of an h x w matrix (row stride ld_in) into out (N x w, stride w).
-// tmp must hold N*w elements when log_h > 11 (unused otherwise).  The
-// output rows are bit-reversed (committed order) when bit_reverse != 0.
+// Coset LDE of an h x w matrix (row stride ld_in) into out (N x w, row
+// stride ld_out >= w, so a column slice of a wider matrix can be written).
+// tmp must hold N*w elements (stride w) when log_h > 11 (unused otherwise).
+// The output rows are bit-reversed (committed order) when bit_reverse != 0.
 hipError_t launch_coset_lde(int field, const uint32_t* in, size_t ld_in, int log_h, uint32_t w,
                             int added_bits, const uint32_t* tw, const uint32_t* scale,
-                            uint32_t* tmp, uint32_t* out, int bit_reverse, hipStream_t stream,
-                            KernelTimer* kt = nullptr);
+                            uint32_t* tmp, uint32_t* out, size_t ld_out, int bit_reverse,
+                            hipStream_t stream, KernelTimer* kt = nullptr);
 int lde_max_log_h();
 
 // ---- FRI (fri.hip) ----------------------------------------------------------

@@ -24,7 +24,7 @@ __device__ __forceinline__ void absorb_rows(uint32_t (&st)[16], const MatRef* ma
     if (total == 0) return;
     int k = 0;
     uint32_t j = 0;
-    const uint32_t* base = mats[0].data + row * mats[0].stride;
+    gcptr base = (gcptr)mats[0].data + row * mats[0].stride;
     uint32_t wk = mats[0].width;
     for (uint32_t off = 0; off < total; off += 8) {
         uint32_t cnt = total - off < 8u ? total - off : 8u;
@@ -34,7 +34,7 @@ __device__ __forceinline__ void absorb_rows(uint32_t (&st)[16], const MatRef* ma
                 while (j >= wk && k + 1 < n) {
                     k++;
                     j = 0;
-                    base = mats[k].data + row * mats[k].stride;
+                    base = (gcptr)mats[k].data + row * mats[k].stride;
                     wk = mats[k].width;
                 }
                 st[t] = base[j];
@@ -150,15 +150,15 @@ __global__ void __launch_bounds__(256) leaf_hash_batch_kernel(const LeafJob* __r
     const int j = find_job(prefix, njobs, gi);
     const LeafJob jb = jobs[j];
     const uint64_t row = gi - prefix[j];
-    const uint32_t* rp = jb.data + row * jb.stride;
+    const gcptr rp = (gcptr)jb.data + row * jb.stride;
     uint32_t st[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) st[t] = 0;
     const uint32_t full = jb.width / 8, tail = jb.width % 8;
     if ((jb.width & 3u) == 0 && (jb.stride & 3u) == 0) {  // 16-byte aligned rows
-        const uint4* r4 = reinterpret_cast<const uint4*>(rp);
+        const GLOBAL u32x4* r4 = reinterpret_cast<const GLOBAL u32x4*>(rp);
         for (uint32_t c = 0; c < full; c++) {
-            const uint4 a = r4[2 * c], b = r4[2 * c + 1];
+            const u32x4 a = r4[2 * c], b = r4[2 * c + 1];
             st[0] = a.x; st[1] = a.y; st[2] = a.z; st[3] = a.w;
             st[4] = b.x; st[5] = b.y; st[6] = b.z; st[7] = b.w;
             permute<F>(st);
@@ -171,15 +171,15 @@ __global__ void __launch_bounds__(256) leaf_hash_batch_kernel(const LeafJob* __r
         }
     }
     if (tail) {  // overwrite only the first `tail` state words
-        const uint32_t* tp = rp + 8 * full;
+        const gcptr tp = rp + 8 * full;
 #pragma unroll
         for (int t = 0; t < 7; t++)
             if ((uint32_t)t < tail) st[t] = tp[t];
         permute<F>(st);
     }
-    uint4* o = reinterpret_cast<uint4*>(jb.digests + 8 * row);
-    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
-    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+    GLOBAL u32x4* o = reinterpret_cast<GLOBAL u32x4*>((gptr)jb.digests + 8 * row);
+    o[0] = u32x4{st[0], st[1], st[2], st[3]};
+    o[1] = u32x4{st[4], st[5], st[6], st[7]};
 }
 
 template <class F>
@@ -191,17 +191,17 @@ __global__ void __launch_bounds__(256) compress_batch_kernel(const CompressJob* 
     const int j = find_job(prefix, njobs, gi);
     const CompressJob jb = jobs[j];
     const uint64_t i = gi - prefix[j];
-    const uint4* p = reinterpret_cast<const uint4*>(jb.prev + 16 * i);
+    const GLOBAL u32x4* p = reinterpret_cast<const GLOBAL u32x4*>((gcptr)jb.prev + 16 * i);
     uint32_t st[16];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const uint4 v = p[q];
+        const u32x4 v = p[q];
         st[4 * q] = v.x; st[4 * q + 1] = v.y; st[4 * q + 2] = v.z; st[4 * q + 3] = v.w;
     }
     permute<F>(st);
-    uint4* o = reinterpret_cast<uint4*>(jb.next + 8 * i);
-    o[0] = make_uint4(st[0], st[1], st[2], st[3]);
-    o[1] = make_uint4(st[4], st[5], st[6], st[7]);
+    GLOBAL u32x4* o = reinterpret_cast<GLOBAL u32x4*>((gptr)jb.next + 8 * i);
+    o[0] = u32x4{st[0], st[1], st[2], st[3]};
+    o[1] = u32x4{st[4], st[5], st[6], st[7]};
 }
 
 __global__ void gather_roots_kernel(const uint32_t* const* __restrict__ src, uint32_t* __restrict__ dst, int n) {

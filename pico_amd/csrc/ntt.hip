@@ -28,8 +28,10 @@
 // In-tile transform of size S = E * P: each thread owns E elements of one
 // column in VGPRs; P threads share a column.  Stage 1 is a size-E register
 // DFT, then a twiddle, one exchange through LDS, and stage 2 is E/P register
-// DFTs of size P.  Lanes map to adjacent columns so every global access is a
-// run of 16 consecutive words of one row.
+// DFTs of size P.  For P > E (2^11 tiles) a third round is added so each
+// thread still holds only E values (tile_dif3 / tile_dit3).  Lanes map to
+// adjacent columns so every global access is a run of 8-16 consecutive
+// words of one row.
 #include "kernels.h"
 #include "field.h"
 
@@ -114,7 +116,7 @@ __device__ __forceinline__ uint32_t rev_dev(uint32_t x, int bits) {
 
 // w_h^{+-e} from the table of w_h^e, e < h (mask = h - 1)
 template <bool INV>
-__device__ __forceinline__ uint32_t tw_at(const uint32_t* __restrict__ tw, uint32_t e, uint32_t mask) {
+__device__ __forceinline__ uint32_t tw_at(gcptr __restrict__ tw, uint32_t e, uint32_t mask) {
     return tw[INV ? ((0u - e) & mask) : e];
 }
 
@@ -135,8 +137,9 @@ constexpr int lds_words() {
 // p E + t, holding DFT output index rev_S(p E + t).
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                         const uint32_t* __restrict__ tw, int tw_sh, uint32_t mask) {
+                                         gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    static_assert(LOG_P <= LOG_E, "two-level tile needs P <= E");
     reg_dif<F, LOG_E, INV>(x);
     if constexpr (LOG_P > 0) {
 #pragma unroll
@@ -161,8 +164,9 @@ __device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* ld
 // Out: x[u P + a] = natural output index (p E/P + u) + E a.
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                         const uint32_t* __restrict__ tw, int tw_sh, uint32_t mask) {
+                                         gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    static_assert(LOG_P <= LOG_E, "two-level tile needs P <= E");
     reg_dit<F, LOG_E, INV>(x);
     if constexpr (LOG_P > 0) {
         const uint32_t rp = rev_dev((uint32_t)p, LOG_P);
@@ -181,16 +185,150 @@ __device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* ld
     }
 }
 
+// Three-level tiles (LOG_P > LOG_E): S = E P with P = E R threads per column
+// (R = P / E <= E), so each thread keeps only E values and a column of a
+// large tile needs no more than E VGPRs of data.  Position n of the tile
+// splits as n = n1 P + n2 R + n3 (n1, n2 < E, n3 < R), frequency k as
+// k = k1 + E k2 + E^2 k3; with w = w_S,
+//   X[k] = sum_n3 w_R^{n3 k3} w_P^{n3 k2} sum_n2 w_E^{n2 k2} w_S^{(n2 R + n3) k1}
+//          sum_n1 w_E^{n1 k1} x[n]
+// i.e. three rounds of register DFTs (size E, E, R) with a twiddle and an LDS
+// exchange between rounds.  Same contracts as the two-level tiles.
+template <class F, int LOG_E, int LOG_P, int C, bool INV>
+__device__ __forceinline__ void tile_dif3(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
+                                          gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
+    constexpr int LOG_G = LOG_E - LOG_R, G = 1 << LOG_G;
+    // round 1: thread p owns positions p + P m (n1 = m, n2 R + n3 = p)
+    reg_dif<F, LOG_E, INV>(x);  // x[t]: k1 = rev_E(t)
+#pragma unroll
+    for (int t = 1; t < E; t++) {
+        const uint32_t e = ((uint32_t)p * (uint32_t)crev(t, LOG_E)) << tw_sh;
+        x[t] = fmul<F>(x[t], tw_at<INV>(tw, e, mask));
+    }
+#pragma unroll
+    for (int t = 0; t < E; t++) lds[lds_at<LOG_E, C>(crev(t, LOG_E) * P + p, c)] = x[t];
+    __syncthreads();
+    // round 2: thread p -> (k1, n3) = (p / R, p % R), values over n2
+    const int k1 = p >> LOG_R, n3 = p & (R - 1);
+#pragma unroll
+    for (int m = 0; m < E; m++) x[m] = lds[lds_at<LOG_E, C>(k1 * P + m * R + n3, c)];
+    reg_dif<F, LOG_E, INV>(x);  // x[t]: k2 = rev_E(t)
+#pragma unroll
+    for (int t = 1; t < E; t++) {
+        const uint32_t e = ((uint32_t)n3 * (uint32_t)crev(t, LOG_E)) << (tw_sh + LOG_E);
+        x[t] = fmul<F>(x[t], tw_at<INV>(tw, e, mask));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < E; t++) lds[lds_at<LOG_E, C>((k1 * E + crev(t, LOG_E)) * R + n3, c)] = x[t];
+    __syncthreads();
+    // round 3: thread p -> G pairs (k1, k2) = (rev_E(p / R), rev_E((p % R) G + g)),
+    // values over n3; output position p E + g R + t holds k3 = rev_R(t)
+    const uint32_t r1 = rev_dev((uint32_t)(p >> LOG_R), LOG_E);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t r2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
+#pragma unroll
+        for (int i = 0; i < R; i++) x[g * R + i] = lds[lds_at<LOG_E, C>((int)((r1 * E + r2) * R) + i, c)];
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) reg_dif<F, LOG_R, INV>(&x[g * R]);
+    __syncthreads();
+}
+
+// DIT counterpart: in, x[t] = tile position p E + t (bit-reversed input
+// order); out, x[r] = natural output index p + P r.
+template <class F, int LOG_E, int LOG_P, int C, bool INV>
+__device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
+                                          gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
+    constexpr int LOG_G = LOG_E - LOG_R, G = 1 << LOG_G;
+    // round A: groups g of R values over n3 (bit-reversed), n1 = rev_E(p / R),
+    // n2 = rev_E((p % R) G + g)
+    const uint32_t n1 = rev_dev((uint32_t)(p >> LOG_R), LOG_E);
+#pragma unroll
+    for (int g = 0; g < G; g++) reg_dit<F, LOG_R, INV>(&x[g * R]);  // x[g R + i]: k3 = i
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t n2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            if (i > 0) x[g * R + i] = fmul<F>(x[g * R + i], tw_at<INV>(tw, (n2 * (uint32_t)i) << (tw_sh + LOG_E), mask));
+            lds[lds_at<LOG_E, C>((int)((n1 * R + i) * E + n2), c)] = x[g * R + i];
+        }
+    }
+    __syncthreads();
+    // round B: thread p -> (n1, k3) = (p / R, p % R), values over n2 (bit-reversed)
+    const int m1 = p >> LOG_R, k3 = p & (R - 1);
+#pragma unroll
+    for (int i = 0; i < E; i++) x[i] = lds[lds_at<LOG_E, C>((m1 * R + k3) * E + crev(i, LOG_E), c)];
+    reg_dit<F, LOG_E, INV>(x);  // x[i]: k2 = i
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const uint32_t kr = (uint32_t)(k3 + R * i);
+        if (m1 > 0 || i > 0) x[i] = fmul<F>(x[i], tw_at<INV>(tw, ((uint32_t)m1 * kr) << tw_sh, mask));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; i++) lds[lds_at<LOG_E, C>((k3 + R * i) * E + m1, c)] = x[i];
+    __syncthreads();
+    // round C: thread p -> kr = p, values over n1 (bit-reversed)
+#pragma unroll
+    for (int i = 0; i < E; i++) x[i] = lds[lds_at<LOG_E, C>(p * E + crev(i, LOG_E), c)];
+    reg_dit<F, LOG_E, INV>(x);  // x[i] = X[p + P i]
+    __syncthreads();
+}
+
+// tile transform of either kind, by shape
+template <class F, int LOG_E, int LOG_P, int C, bool INV>
+__device__ __forceinline__ void tile_dif_any(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
+                                             gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+    if constexpr (LOG_P > LOG_E)
+        tile_dif3<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+    else
+        tile_dif<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+}
+template <class F, int LOG_E, int LOG_P, int C, bool INV>
+__device__ __forceinline__ void tile_dit_any(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
+                                             gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+    if constexpr (LOG_P > LOG_E)
+        tile_dit3<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+    else
+        tile_dit<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+}
+
 template <int LOG_E, int LOG_P>
 __device__ __forceinline__ uint32_t dit_out_index(int p, int r) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
-    return (uint32_t)(p * (E / P) + r / P + E * (r % P));
+    if constexpr (LOG_P > LOG_E)
+        return (uint32_t)(p + P * r);  // three-level tile
+    else
+        return (uint32_t)(p * (E / P) + r / P + E * (r % P));
 }
 
-// Columns per tile: 16 (64-byte row runs), 8 for the largest tile when it
-// also parks coefficients in LDS (exchange + coefficient buffers <= 128 KiB).
+// Tile shape per transform size 2^log_s and pass: E = 2^log_e values per
+// thread, P = 2^(log_s - log_e) threads per column; three-level when P > E.
+// Measured on MI355X (2^20 x 256 and 2^22 x 128 commits): at 2^10 the
+// three-level 16 x 64 tile beats 32 x 32 in every pass (more waves per SIMD:
+// 55-67 VGPRs against 97-169); at 2^11 the coset-looping passes (BC, SINGLE)
+// take 16 x 128 (64 x 32 needs 257 VGPRs there, one wave per SIMD) while the
+// strided passes keep 64 x 32, which measures faster for them.
+constexpr int shape_log_e(int log_s, int kind) {
+    return log_s <= 5 ? log_s
+           : log_s == 6 ? 3
+           : log_s <= 8 ? 4
+           : log_s == 9 ? 5
+           : log_s == 10 ? 4
+           : (kind == KIND_BC || kind == KIND_SINGLE) ? 4 : 6;
+}
+
+// Columns per tile: 16 (64-byte row runs) up to 1024 threads per workgroup;
+// 8 for the two-level 2^11 tile when it also parks coefficients in LDS
+// (exchange + coefficient buffers <= 128 KiB).
 constexpr int cols_for(int log_e, int log_p, int kind) {
-    return (log_e + log_p >= 11 && (kind == KIND_BC || kind == KIND_SINGLE)) ? 8 : 16;
+    return log_p > log_e ? (1024 >> log_p < 16 ? 1024 >> log_p : 16)
+           : (log_e + log_p >= 11 && (kind == KIND_BC || kind == KIND_SINGLE)) ? 8 : 16;
 }
 constexpr int threads_for(int log_e, int log_p, int kind) {
     return cols_for(log_e, log_p, kind) * (1 << log_p) >= 256 ? cols_for(log_e, log_p, kind) * (1 << log_p) : 256;
@@ -225,7 +363,7 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P;
     constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
     constexpr int NT = threads_for(LOG_E, LOG_P, KIND);
-    constexpr bool PARK = parks_coefficients(KIND) && E >= 16;
+    constexpr bool PARK = parks_coefficients(KIND) && E >= 32;
     constexpr int LW = lds_words<LOG_E, LOG_P, C>();  // exchange words per tile group
     __shared__ uint32_t lds_all[LOG_P > 0 ? G * LW : 1];
     __shared__ uint32_t lds_coef[PARK ? E * NT : 1];
@@ -256,6 +394,8 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     const uint32_t col = (uint32_t)(tile % ncb) * C + c;  // column block fastest
     const uint64_t rest = tile / ncb;
     const bool act = valid && col < w;
+    const gcptr g_in = (gcptr)a.in, g_tw = (gcptr)a.tw, g_scale = (gcptr)a.scale;
+    const gptr g_out = (gptr)a.out, g_tmp = (gptr)a.tmp;
     uint32_t x[E];
 
     if constexpr (KIND == KIND_SINGLE || KIND == KIND_BC) {
@@ -263,12 +403,12 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         // of n_lo rows of the pass-A output.
         const uint32_t n_lo = 1u << a.k2;
         const uint32_t blk = KIND == KIND_BC ? (uint32_t)rest : 0u;
-        const uint32_t* src = KIND == KIND_BC ? a.out : a.in;
-        const uint64_t ld = KIND == KIND_BC ? (uint64_t)w : a.ld_in;
+        const gcptr src = KIND == KIND_BC ? (gcptr)g_out : g_in;
+        const uint64_t ld = KIND == KIND_BC ? a.ld_out : a.ld_in;
 #pragma unroll
         for (int m = 0; m < E; m++)
             x[m] = act ? src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * ld + col] : 0u;
-        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, a.k1, mask);
+        tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k1, mask);
         // The coefficients are reused for every coset: large tiles park them
         // in a thread-private LDS slice (VGPRs would spill), small ones keep a
         // register copy.
@@ -288,11 +428,11 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t ncos = njobs > 0 ? (1u << a0.b) : ncos_job;
         for (uint32_t cs = 0; cs < ncos; cs++) {
             const bool live = cs < ncos_job;
-            const uint32_t* sc = a.scale + (uint64_t)(live ? cs : 0u) * h;
+            const gcptr sc = g_scale + (uint64_t)(live ? cs : 0u) * h;
             // Opaque copy of the twiddle pointer: keeps the (coset-invariant)
             // twiddle loads inside the loop instead of hoisting ~64 of them
             // into VGPRs for all cosets, which spills.
-            const uint32_t* twl = a.tw;
+            gcptr twl = g_tw;
             asm volatile("" : "+v"(twl));
             // Likewise re-derive the lane coordinates per coset so the 64-bit
             // gather/store addresses are not all precomputed outside the loop.
@@ -304,15 +444,15 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
                 const uint32_t v = PARK ? lds_coef[t * NT + tid] : keep[PARK ? 0 : t];
                 x[t] = fmul<F>(v, sc[rev_dev(n_lo * blk + pl * E + t, a.L)]);
             }
-            tile_dit<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask);
+            tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask);
             if (act && live) {
 #pragma unroll
                 for (int r = 0; r < E; r++) {
                     const uint32_t j = dit_out_index<LOG_E, LOG_P>(pl, r);
                     if constexpr (KIND == KIND_BC)
-                        a.tmp[((uint64_t)cs * h + n_lo * blk + j) * w + coll] = fmul<F>(x[r], twl[j * rblk]);
+                        g_tmp[((uint64_t)cs * h + n_lo * blk + j) * w + coll] = fmul<F>(x[r], twl[j * rblk]);
                     else
-                        a.out[out_row(a, cs, j) * w + coll] = x[r];
+                        g_out[out_row(a, cs, j) * a.ld_out + coll] = x[r];
                 }
             }
         }
@@ -321,14 +461,14 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t j_lo = (uint32_t)rest;
 #pragma unroll
         for (int m = 0; m < E; m++)
-            x[m] = act ? a.in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + col] : 0u;
-        tile_dif<F, LOG_E, LOG_P, C, true>(x, lds, p, c, a.tw, a.k2, mask);
+            x[m] = act ? g_in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + col] : 0u;
+        tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k2, mask);
         if (act) {
 #pragma unroll
             for (int t = 0; t < E; t++) {
                 const uint32_t pos = (uint32_t)(p * E + t);
                 const uint32_t e = j_lo * rev_dev(pos, a.k1);
-                a.out[(uint64_t)(j_lo + n_lo * pos) * w + col] = fmul<F>(x[t], tw_at<true>(a.tw, e, mask));
+                g_out[(uint64_t)(j_lo + n_lo * pos) * a.ld_out + col] = fmul<F>(x[t], tw_at<true>(g_tw, e, mask));
             }
         }
     } else {  // KIND_D
@@ -337,13 +477,13 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t cs = (uint32_t)(rest / n_lo);
 #pragma unroll
         for (int t = 0; t < E; t++)
-            x[t] = act ? a.tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + col] : 0u;
-        tile_dit<F, LOG_E, LOG_P, C, false>(x, lds, p, c, a.tw, a.k2, mask);
+            x[t] = act ? g_tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + col] : 0u;
+        tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, p, c, g_tw, a.k2, mask);
         if (act) {
 #pragma unroll
             for (int r = 0; r < E; r++) {
                 const uint32_t natural = j_lo + n_lo * dit_out_index<LOG_E, LOG_P>(p, r);
-                a.out[out_row(a, cs, natural) * w + col] = x[r];
+                g_out[out_row(a, cs, natural) * a.ld_out + col] = x[r];
             }
         }
     }
@@ -430,7 +570,7 @@ static void run(const PassLaunch& pl, hipStream_t s, KernelTimer* kt) {
     if (kt) kt->end(s);
 }
 
-// tile shapes (LOG_E, LOG_P) per log S, with E >= P
+// tile shapes (LOG_E, LOG_P) per log S from shape_log_e
 template <class F, int KIND>
 static bool run_logs(int log_s, const PassLaunch& a, hipStream_t s, KernelTimer* kt) {
     switch (log_s) {
@@ -440,12 +580,12 @@ static bool run_logs(int log_s, const PassLaunch& a, hipStream_t s, KernelTimer*
         case 3: if constexpr (KIND == KIND_SINGLE) { run<F, 3, 0, KIND>(a, s, kt); return true; } break;
         case 4: if constexpr (KIND == KIND_SINGLE) { run<F, 4, 0, KIND>(a, s, kt); return true; } break;
         case 5: if constexpr (KIND == KIND_SINGLE) { run<F, 5, 0, KIND>(a, s, kt); return true; } break;
-        case 6: run<F, 3, 3, KIND>(a, s, kt); return true;
-        case 7: run<F, 4, 3, KIND>(a, s, kt); return true;
-        case 8: run<F, 4, 4, KIND>(a, s, kt); return true;
-        case 9: run<F, 5, 4, KIND>(a, s, kt); return true;
-        case 10: run<F, 5, 5, KIND>(a, s, kt); return true;
-        case 11: run<F, 6, 5, KIND>(a, s, kt); return true;
+        case 6: run<F, shape_log_e(6, KIND), 6 - shape_log_e(6, KIND), KIND>(a, s, kt); return true;
+        case 7: run<F, shape_log_e(7, KIND), 7 - shape_log_e(7, KIND), KIND>(a, s, kt); return true;
+        case 8: run<F, shape_log_e(8, KIND), 8 - shape_log_e(8, KIND), KIND>(a, s, kt); return true;
+        case 9: run<F, shape_log_e(9, KIND), 9 - shape_log_e(9, KIND), KIND>(a, s, kt); return true;
+        case 10: run<F, shape_log_e(10, KIND), 10 - shape_log_e(10, KIND), KIND>(a, s, kt); return true;
+        case 11: run<F, shape_log_e(11, KIND), 11 - shape_log_e(11, KIND), KIND>(a, s, kt); return true;
         default: break;
     }
     return false;
@@ -457,7 +597,7 @@ void lde_split(int L, int* k1, int* k2) {
 }
 
 int lde_pass_cols(int kind, int log_s) {
-    return (log_s >= 11 && (kind == KIND_BC || kind == KIND_SINGLE)) ? 8 : 16;
+    return cols_for(shape_log_e(log_s, kind), log_s - shape_log_e(log_s, kind), kind);
 }
 
 uint64_t lde_pass_row_tiles(int kind, int L, int b) {
@@ -495,11 +635,12 @@ hipError_t launch_lde_pass_batch(int field, int kind, int log_s, const LdeArgs* 
 template <class F>
 static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t w, int b,
                               const uint32_t* tw, const uint32_t* scale, uint32_t* tmp, uint32_t* out,
-                              int bitrev, hipStream_t s, KernelTimer* kt) {
+                              size_t ld_out, int bitrev, hipStream_t s, KernelTimer* kt) {
     LdeArgs a{};
     a.in = in;
     a.ld_in = ld_in;
     a.out = out;
+    a.ld_out = ld_out;
     a.tmp = tmp;
     a.tw = tw;
     a.scale = scale;
@@ -521,11 +662,13 @@ static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t 
 
 hipError_t launch_coset_lde(int field, const uint32_t* in, size_t ld_in, int log_h, uint32_t w,
                             int added_bits, const uint32_t* tw, const uint32_t* scale, uint32_t* tmp,
-                            uint32_t* out, int bit_reverse, hipStream_t stream, KernelTimer* kt) {
-    if (log_h < 0 || log_h > lde_max_log_h() || w == 0) return hipErrorInvalidValue;
+                            uint32_t* out, size_t ld_out, int bit_reverse, hipStream_t stream, KernelTimer* kt) {
+    if (log_h < 0 || log_h > lde_max_log_h() || w == 0 || ld_out < w) return hipErrorInvalidValue;
     return field == FIELD_KOALABEAR
-               ? coset_lde_t<KoalaBear>(in, ld_in, log_h, w, added_bits, tw, scale, tmp, out, bit_reverse, stream, kt)
-               : coset_lde_t<BabyBear>(in, ld_in, log_h, w, added_bits, tw, scale, tmp, out, bit_reverse, stream, kt);
+               ? coset_lde_t<KoalaBear>(in, ld_in, log_h, w, added_bits, tw, scale, tmp, out, ld_out, bit_reverse,
+                                        stream, kt)
+               : coset_lde_t<BabyBear>(in, ld_in, log_h, w, added_bits, tw, scale, tmp, out, ld_out, bit_reverse,
+                                       stream, kt);
 }
 
 }  // namespace pg

@@ -208,20 +208,59 @@ PG_HD uint32_t sum16(const uint32_t (&s)[16]) {
     return r;
 }
 
-// s_i <- s_i * diag_i + S.  Both fields' diagonals start with
-// -2, 1, 2, 1/2, 3, 4, -1/2 (consts.rs:110-154); those entries use
-// additions and a halving instead of a Montgomery product.
+// S + c x mod p for a small constant c, x and S below p.  v = c x + S < (c+1) p
+// is one 64-bit multiply-add; with q = v >> 31 <= c, v - q p lies in
+// [0, 2^31 + c (2^31 - p)), which fits 32 bits for c < 2^31 / (2^31 - p)
+// (128 for KoalaBear, 16 for BabyBear) and needs one conditional subtraction
+// when it is below 2p, two otherwise.  The low word of v + q (2^32 - p) is
+// v - q p.
+template <class F, uint32_t C>
+PG_HD uint32_t fmadd_small(uint32_t x, uint32_t S) {
+    static_assert((uint64_t)C * ((1ull << 31) - F::P) < (1ull << 31), "bound needs c < 2^31 / (2^31 - p)");
+    const uint64_t v = (uint64_t)x * C + S;
+    const uint32_t q = (uint32_t)(v >> 31);
+    uint32_t r = (uint32_t)v + q * (0u - F::P);
+    r = freduce<F>(r);
+    if constexpr ((1ull << 31) + (uint64_t)C * ((1ull << 31) - F::P) > 2ull * F::P) r = freduce<F>(r);
+    return r;
+}
+
+// S - c x = S + c (p - x)
+template <class F, uint32_t C>
+PG_HD uint32_t fmsub_small(uint32_t x, uint32_t S) {
+    return fmadd_small<F, C>(F::P - x, S);
+}
+
+// s_i <- s_i * diag_i + S.  Both fields' diagonals (consts.rs:110-154) are
+//   -2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, then +-2^-k entries and +-c
+// with c = 127 (KoalaBear: entries 11 = -127, 15 = 127) or c = 15 (BabyBear:
+// 12 = -15, 15 = 15).  The entries -2 .. 1/2 and -1/2 use additions and a
+// halving, the small integers a 64-bit multiply-add (fmadd_small), the
+// powers of two a Montgomery product.
 template <class F>
 PG_HD void internal_layer(uint32_t (&s)[16]) {
+    constexpr bool KB = F::P == KoalaBear::P;
+    constexpr uint32_t CBIG = KB ? 127u : 15u;
     const uint32_t sum = sum16<F>(s);
     s[0] = fsub<F>(sum, fadd<F>(s[0], s[0]));
     s[1] = fadd<F>(sum, s[1]);
     s[2] = fadd<F>(sum, fadd<F>(s[2], s[2]));
     s[3] = fadd<F>(sum, fhalve<F>(s[3]));
+    s[4] = fmadd_small<F, 3>(s[4], sum);
+    s[5] = fmadd_small<F, 4>(s[5], sum);
     s[6] = fsub<F>(sum, fhalve<F>(s[6]));
+    s[7] = fmsub_small<F, 3>(s[7], sum);
+    s[8] = fmsub_small<F, 4>(s[8], sum);
+    if constexpr (KB) {
+        s[11] = fmsub_small<F, CBIG>(s[11], sum);
 #pragma unroll
-    for (int i = 4; i < 16; i++)
-        if (i != 6) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
+        for (int i : {9, 10, 12, 13, 14}) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
+    } else {
+        s[12] = fmsub_small<F, CBIG>(s[12], sum);
+#pragma unroll
+        for (int i : {9, 10, 11, 13, 14}) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
+    }
+    s[15] = fmadd_small<F, CBIG>(s[15], sum);
 }
 
 template <class F>

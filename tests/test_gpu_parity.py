@@ -36,7 +36,7 @@ def test_upload_download_roundtrip(ctx, oracle, f):
 # --------------------------------------------------------------------- LDE --
 LDE_CASES = [(L, w, b) for L in range(0, 12) for (w, b) in [(1, 1), (5, 2), (16, 1), (33, 3)]]
 LDE_CASES += [(L, w, b) for L in range(12, 17) for (w, b) in [(3, 1), (17, 2), (40, 1)]]
-LDE_CASES += [(18, 2, 3), (20, 3, 1), (22, 1, 1)]
+LDE_CASES += [(18, 2, 3), (20, 3, 1), (21, 5, 2), (22, 1, 1), (22, 2, 2)]
 
 
 @pytest.mark.parametrize("L,w,b", LDE_CASES)
