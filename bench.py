@@ -24,20 +24,41 @@ METRIC = "KoalaBear LDE+Merkle commit cells/s at 2^20x256; p50/p99 job latency"
 # VALU lane-instructions each -> 7328.  Reported beside the count of the
 # built kernel (pico_amd/_build/isa_stats.json, loop body per sponge chunk).
 TEXTBOOK_OPS_PER_PERM = 616 * 5 + 1416 * 3
-# MI355X VALU issue peak, 32-bit lane-instructions: 256 CU x 4 SIMD x 16
-# lanes x 2.4 GHz = 39.3 T/s (the published 157.3 TFLOP/s FP32 vector figure
-# counts an FMA as 2 flops and packed-FP32 dual issue as 2 lanes).  gfx950
-# issues the 32-bit integer multiplies at this full rate.
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# MI355X VALU single-issue ceiling: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz =
+# 39.3 T lane-slots/s (the published 157.3 TFLOP/s FP32 vector peak is this
+# times 2 for FMA times 2 for packed FP32, which integer code cannot use).
+# v_add_u32 / v_sub_u32 / v_mov_b32 / v_and_b32 issue up to twice per slot and
+# are priced at half a slot (pico_amd/probes/valu_rates.hip).
+VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
+# per-opcode rates measured by pico_amd/probes/valu_rates.hip on MI355X
+# (T lane-ops/s); an opcode not listed is taken at the single-issue rate
+MEASURED_RATES = {"v_add_u32": 64.2, "v_sub_u32": 63.7, "v_mov_b32": 69.0, "v_min_u32": 37.7,
+                  "v_alignbit_b32": 38.4, "v_lshlrev_b32": 38.4, "v_and_b32": 69.5, "v_add3_u32": 37.9,
+                  "v_mad_u32_u24": 38.1, "v_mul_lo_u32": 36.2, "v_mul_hi_u32": 38.5, "v_mad_u64_u32": 35.2,
+                  "v_lshl_add_u64": 35.5, "v_lshlrev_b64": 38.7}
 
 
-def isa_valu_per_chunk(field="koalabear"):
+def isa_stats(field="koalabear"):
     path = os.path.join(ROOT, "pico_amd", "_build", "isa_stats.json")
     try:
         with open(path) as f:
-            return json.load(f)[field]["leaf_hash_w8_valu_per_chunk"]
+            return json.load(f)[field]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def pmc_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE in KiB, each times its copy-kernel factor)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
+            d = json.load(f)
+        k = d["workloads"][workload][kernel]
+        return (k["fetch_kib"] * d["fetch_factor"] + k["write_kib"] * d["write_factor"]) * 1024
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -161,18 +182,24 @@ def main():
     n_rows = h << lb
     perms_leaf = n_rows * ((w + 7) // 8)
     roofline = None
-    if dominant in ("leaf_hash_w8", "leaf_hash"):
+    workload = f"koalabear_pcs_commit_2^{args.log_h}x{w}_blowup{1 << lb}"
+    st = isa_stats()
+    if dominant == "leaf_hash_w8" and st:
         avg_s = kern[dominant]["avg_ms"] / 1e3
-        per_perm = isa_valu_per_chunk() or TEXTBOOK_OPS_PER_PERM
-        achieved = perms_leaf * per_perm / avg_s / 1e12
-        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
-                    "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                    "kernel": dominant, "perms_per_launch": perms_leaf,
-                    "perm_rate_per_s": perms_leaf / avg_s,
-                    "valu_instr_per_perm": per_perm,
+        rate = perms_leaf / avg_s
+        slots = st["leaf_hash_w8_slots_per_chunk"]
+        achieved = rate * slots / 1e12
+        floor_s = perms_leaf * sum(n / (MEASURED_RATES.get(op, 38.3) * 1e12)
+                                   for op, n in st["leaf_hash_w8_opcodes"].items())
+        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TSLOTS, 3),
+                    "unit": "T VALU issue-slots/s", "frac": round(achieved / VALU_PEAK_TSLOTS, 4),
+                    "traffic": pmc_traffic(workload, dominant),
+                    "kernel": dominant, "avg_ms": round(kern[dominant]["avg_ms"], 4),
+                    "perms_per_launch": perms_leaf, "perm_rate_per_s": rate,
+                    "slots_per_perm": slots, "valu_instr_per_perm": st["leaf_hash_w8_valu_per_chunk"],
                     "textbook_instr_per_perm": TEXTBOOK_OPS_PER_PERM,
-                    "perm_rate_vs_textbook_at_peak": round(perms_leaf / avg_s * TEXTBOOK_OPS_PER_PERM
-                                                           / (VALU_PEAK_TOPS * 1e12), 4)}
+                    "algorithmic_bytes": n_rows * w * 4 + n_rows * 32,
+                    "frac_of_measured_opcode_rate_floor": round(floor_s / avg_s, 4)}
     # HBM view of the NTT passes: algorithmic bytes per launch / duration
     by = 4 * w
     pass_bytes = {"lde_pass_a": 2 * h * by, "lde_pass_bc": h * by + n_rows * by,
@@ -197,7 +224,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": f"koalabear_pcs_commit_2^{args.log_h}x{w}_blowup{1 << lb}",
+        "config": {"workload": workload,
                    "field": "KoalaBear", "rows": h, "cols": w, "log_blowup": lb,
                    "hash": "Poseidon2-W16 sponge(8)/truncated-compress", "lde_rows": n_rows},
         "p50_ms": percentile(lat, 50),
