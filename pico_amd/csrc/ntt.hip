@@ -78,8 +78,10 @@ __device__ __forceinline__ void reg_dif(uint32_t* x) {
             for (int j = 0; j < half; j++) {
                 const uint32_t a = x[blk + j], b = x[blk + j + half];
                 x[blk + j] = fadd<F>(a, b);
-                const uint32_t d = fsub<F>(a, b);
-                x[blk + j + half] = j == 0 ? d : fmul<F>(d, level_root<F, INV>(s, j));
+                // a - b + p lies in [1, 2p) and may feed the Montgomery product
+                // unreduced: (2p p + 2^32 p) / 2^32 < 2p, which fmul's single
+                // conditional subtraction brings below p
+                x[blk + j + half] = j == 0 ? fsub<F>(a, b) : fmul<F>(a - b + F::P, level_root<F, INV>(s, j));
             }
     }
 }
@@ -120,6 +122,31 @@ __device__ __forceinline__ uint32_t tw_at(gcptr __restrict__ tw, uint32_t e, uin
     return tw[INV ? ((0u - e) & mask) : e];
 }
 
+// Scheduling fence: the compiler may not move any instruction across it.
+// Loads are issued in a batch before it and their uses come after, so a
+// thread has all its loads in flight at once.  Left to itself the scheduler
+// interleaved uses with the loads, and each use then waited for every
+// earlier access (s_waitcnt vmcnt(0)): one HBM round trip per element.
+__device__ __forceinline__ void issue_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// x[i] *= tw_at(tw, e(i)) for i in [I0, N), the twiddle loads of each chunk
+// of 16 issued together ahead of the products.
+template <class F, bool INV, int N, int I0, class Exp>
+__device__ __forceinline__ void twiddle_mul(uint32_t* x, gcptr __restrict__ tw, uint32_t mask, Exp e) {
+    constexpr int CH = 16;
+#pragma unroll
+    for (int b = I0; b < N; b += CH) {
+        uint32_t v[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++)
+            if (b + i < N) v[i] = tw_at<INV>(tw, e(b + i), mask);
+        issue_fence();
+#pragma unroll
+        for (int i = 0; i < CH; i++)
+            if (b + i < N) x[b + i] = fmul<F>(x[b + i], v[i]);
+    }
+}
+
 // LDS word of (tile position, column).  A wave's lanes are C columns x
 // (64 / C) values of p; positions strided by E p would all map to the same
 // banks (E C is a multiple of 64), so C extra words follow every E positions:
@@ -142,11 +169,9 @@ __device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* ld
     static_assert(LOG_P <= LOG_E, "two-level tile needs P <= E");
     reg_dif<F, LOG_E, INV>(x);
     if constexpr (LOG_P > 0) {
-#pragma unroll
-        for (int m = 1; m < E; m++) {
-            const uint32_t e = ((uint32_t)p * (uint32_t)crev(m, LOG_E)) << tw_sh;
-            x[m] = fmul<F>(x[m], tw_at<INV>(tw, e, mask));
-        }
+        twiddle_mul<F, INV, E, 1>(x, tw, mask, [&](int m) {
+            return ((uint32_t)p * (uint32_t)crev(m, LOG_E)) << tw_sh;
+        });
 #pragma unroll
         for (int m = 0; m < E; m++) lds[lds_at<LOG_E, C>(p + P * m, c)] = x[m];
         __syncthreads();
@@ -170,8 +195,7 @@ __device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* ld
     reg_dit<F, LOG_E, INV>(x);
     if constexpr (LOG_P > 0) {
         const uint32_t rp = rev_dev((uint32_t)p, LOG_P);
-#pragma unroll
-        for (int j = 1; j < E; j++) x[j] = fmul<F>(x[j], tw_at<INV>(tw, ((uint32_t)j * rp) << tw_sh, mask));
+        twiddle_mul<F, INV, E, 1>(x, tw, mask, [&](int j) { return ((uint32_t)j * rp) << tw_sh; });
 #pragma unroll
         for (int j = 0; j < E; j++) lds[lds_at<LOG_E, C>(E * p + j, c)] = x[j];
         __syncthreads();
@@ -199,26 +223,30 @@ __device__ __forceinline__ void tile_dif3(uint32_t (&x)[1 << LOG_E], uint32_t* l
                                           gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
     constexpr int LOG_G = LOG_E - LOG_R, G = 1 << LOG_G;
-    // round 1: thread p owns positions p + P m (n1 = m, n2 R + n3 = p)
+    // round 1: thread p owns positions p + P m (n1 = m, n2 R + n3 = p); the
+    // round's twiddles are fetched before its register DFT
+    uint32_t tv[E];
+#pragma unroll
+    for (int t = 1; t < E; t++) tv[t] = tw_at<INV>(tw, ((uint32_t)p * (uint32_t)crev(t, LOG_E)) << tw_sh, mask);
+    issue_fence();
     reg_dif<F, LOG_E, INV>(x);  // x[t]: k1 = rev_E(t)
 #pragma unroll
-    for (int t = 1; t < E; t++) {
-        const uint32_t e = ((uint32_t)p * (uint32_t)crev(t, LOG_E)) << tw_sh;
-        x[t] = fmul<F>(x[t], tw_at<INV>(tw, e, mask));
-    }
+    for (int t = 1; t < E; t++) x[t] = fmul<F>(x[t], tv[t]);
 #pragma unroll
     for (int t = 0; t < E; t++) lds[lds_at<LOG_E, C>(crev(t, LOG_E) * P + p, c)] = x[t];
-    __syncthreads();
-    // round 2: thread p -> (k1, n3) = (p / R, p % R), values over n2
+    // round 2: thread p -> (k1, n3) = (p / R, p % R), values over n2; its
+    // twiddles are in flight across the barrier and the exchange
     const int k1 = p >> LOG_R, n3 = p & (R - 1);
+#pragma unroll
+    for (int t = 1; t < E; t++)
+        tv[t] = tw_at<INV>(tw, ((uint32_t)n3 * (uint32_t)crev(t, LOG_E)) << (tw_sh + LOG_E), mask);
+    issue_fence();
+    __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; m++) x[m] = lds[lds_at<LOG_E, C>(k1 * P + m * R + n3, c)];
     reg_dif<F, LOG_E, INV>(x);  // x[t]: k2 = rev_E(t)
 #pragma unroll
-    for (int t = 1; t < E; t++) {
-        const uint32_t e = ((uint32_t)n3 * (uint32_t)crev(t, LOG_E)) << (tw_sh + LOG_E);
-        x[t] = fmul<F>(x[t], tw_at<INV>(tw, e, mask));
-    }
+    for (int t = 1; t < E; t++) x[t] = fmul<F>(x[t], tv[t]);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < E; t++) lds[lds_at<LOG_E, C>((k1 * E + crev(t, LOG_E)) * R + n3, c)] = x[t];
@@ -247,6 +275,14 @@ __device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* l
     // round A: groups g of R values over n3 (bit-reversed), n1 = rev_E(p / R),
     // n2 = rev_E((p % R) G + g)
     const uint32_t n1 = rev_dev((uint32_t)(p >> LOG_R), LOG_E);
+    uint32_t tv[E];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t n2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
+#pragma unroll
+        for (int i = 1; i < R; i++) tv[g * R + i] = tw_at<INV>(tw, (n2 * (uint32_t)i) << (tw_sh + LOG_E), mask);
+    }
+    issue_fence();
 #pragma unroll
     for (int g = 0; g < G; g++) reg_dit<F, LOG_R, INV>(&x[g * R]);  // x[g R + i]: k3 = i
 #pragma unroll
@@ -254,21 +290,22 @@ __device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* l
         const uint32_t n2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
 #pragma unroll
         for (int i = 0; i < R; i++) {
-            if (i > 0) x[g * R + i] = fmul<F>(x[g * R + i], tw_at<INV>(tw, (n2 * (uint32_t)i) << (tw_sh + LOG_E), mask));
+            if (i > 0) x[g * R + i] = fmul<F>(x[g * R + i], tv[g * R + i]);
             lds[lds_at<LOG_E, C>((int)((n1 * R + i) * E + n2), c)] = x[g * R + i];
         }
     }
+    // round B's twiddles are in flight across the barrier and the exchange
+    const int m1 = p >> LOG_R, k3 = p & (R - 1);
+#pragma unroll
+    for (int i = 0; i < E; i++) tv[i] = tw_at<INV>(tw, ((uint32_t)m1 * (uint32_t)(k3 + R * i)) << tw_sh, mask);
+    issue_fence();
     __syncthreads();
     // round B: thread p -> (n1, k3) = (p / R, p % R), values over n2 (bit-reversed)
-    const int m1 = p >> LOG_R, k3 = p & (R - 1);
 #pragma unroll
     for (int i = 0; i < E; i++) x[i] = lds[lds_at<LOG_E, C>((m1 * R + k3) * E + crev(i, LOG_E), c)];
     reg_dit<F, LOG_E, INV>(x);  // x[i]: k2 = i
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-        const uint32_t kr = (uint32_t)(k3 + R * i);
-        if (m1 > 0 || i > 0) x[i] = fmul<F>(x[i], tw_at<INV>(tw, ((uint32_t)m1 * kr) << tw_sh, mask));
-    }
+    for (int i = 0; i < E; i++) x[i] = fmul<F>(x[i], tv[i]);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < E; i++) lds[lds_at<LOG_E, C>((k3 + R * i) * E + m1, c)] = x[i];
@@ -357,8 +394,16 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 
 // njobs == 0: one job, `a0`, tiles counted from 0.  Otherwise the tile's job
 // is found in the device table: prefix[j] <= tile < prefix[j + 1].
+// Waves per SIMD to ask of the compiler: the three-level strided passes (A,
+// D) fit two 1024-thread workgroups per CU at <= 64 VGPRs without spilling;
+// the coset-looping passes do not, and are left unconstrained.
+constexpr int waves_for(int log_e, int log_p, int kind) {
+    return log_p > log_e && threads_for(log_e, log_p, kind) == 1024 && (kind == KIND_A || kind == KIND_D) ? 8 : 1;
+}
+
 template <class F, int LOG_E, int LOG_P, int KIND>
 __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
+    __attribute__((amdgpu_waves_per_eu(waves_for(LOG_E, LOG_P, KIND))))
     lde_kernel(LdeArgs a0, const LdeArgs* __restrict__ jobs, const uint64_t* __restrict__ prefix, int njobs) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P;
     constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
@@ -392,8 +437,13 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     const uint32_t w = a.w;
     const uint32_t ncb = (w + C - 1) / C;
     const uint32_t col = (uint32_t)(tile % ncb) * C + c;  // column block fastest
-    const uint64_t rest = tile / ncb;
     const bool act = valid && col < w;
+    // Loads are unconditional (a select on `act` after each load made the
+    // compiler wait for every load in turn), so inactive lanes read row 0 /
+    // column 0 of their job; every table index below derives from these
+    // clamped coordinates and stays in range.  Stores are guarded by `act`.
+    const uint64_t rest = valid ? tile / ncb : 0;
+    const uint32_t colr = act ? col : 0u;
     const gcptr g_in = (gcptr)a.in, g_tw = (gcptr)a.tw, g_scale = (gcptr)a.scale;
     const gptr g_out = (gptr)a.out, g_tmp = (gptr)a.tmp;
     uint32_t x[E];
@@ -407,7 +457,8 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint64_t ld = KIND == KIND_BC ? a.ld_out : a.ld_in;
 #pragma unroll
         for (int m = 0; m < E; m++)
-            x[m] = act ? src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * ld + col] : 0u;
+            x[m] = src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * ld + colr];
+        issue_fence();
         tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k1, mask);
         // The coefficients are reused for every coset: large tiles park them
         // in a thread-private LDS slice (VGPRs would spill), small ones keep a
@@ -440,19 +491,32 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
             uint32_t coll = col;
             if constexpr (LOG_P > 0) asm volatile("" : "+v"(pl), "+v"(coll));
 #pragma unroll
-            for (int t = 0; t < E; t++) {
-                const uint32_t v = PARK ? lds_coef[t * NT + tid] : keep[PARK ? 0 : t];
-                x[t] = fmul<F>(v, sc[rev_dev(n_lo * blk + pl * E + t, a.L)]);
+            for (int b0 = 0; b0 < E; b0 += 16) {
+                constexpr int CH = E < 16 ? E : 16;
+#pragma unroll
+                for (int t = 0; t < CH; t++) x[b0 + t] = sc[n_lo * blk + pl * E + b0 + t];
+                issue_fence();
+#pragma unroll
+                for (int t = 0; t < CH; t++) {
+                    const uint32_t v = PARK ? lds_coef[(b0 + t) * NT + tid] : keep[PARK ? 0 : b0 + t];
+                    x[b0 + t] = fmul<F>(v, x[b0 + t]);
+                }
             }
             tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask);
             if (act && live) {
+                if constexpr (KIND == KIND_BC) {
+                    twiddle_mul<F, false, E, 0>(x, twl, mask, [&](int r) {
+                        return dit_out_index<LOG_E, LOG_P>(pl, r) * rblk;
+                    });
 #pragma unroll
-                for (int r = 0; r < E; r++) {
-                    const uint32_t j = dit_out_index<LOG_E, LOG_P>(pl, r);
-                    if constexpr (KIND == KIND_BC)
-                        g_tmp[((uint64_t)cs * h + n_lo * blk + j) * w + coll] = fmul<F>(x[r], twl[j * rblk]);
-                    else
-                        g_out[out_row(a, cs, j) * a.ld_out + coll] = x[r];
+                    for (int r = 0; r < E; r++) {
+                        const uint32_t j = dit_out_index<LOG_E, LOG_P>(pl, r);
+                        g_tmp[((uint64_t)cs * h + n_lo * blk + j) * w + coll] = x[r];
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < E; r++)
+                        g_out[out_row(a, cs, dit_out_index<LOG_E, LOG_P>(pl, r)) * a.ld_out + coll] = x[r];
                 }
             }
         }
@@ -461,15 +525,15 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t j_lo = (uint32_t)rest;
 #pragma unroll
         for (int m = 0; m < E; m++)
-            x[m] = act ? g_in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + col] : 0u;
+            x[m] = g_in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + colr];
+        issue_fence();
         tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k2, mask);
         if (act) {
+            twiddle_mul<F, true, E, 0>(x, g_tw, mask,
+                                       [&](int t) { return j_lo * rev_dev((uint32_t)(p * E + t), a.k1); });
 #pragma unroll
-            for (int t = 0; t < E; t++) {
-                const uint32_t pos = (uint32_t)(p * E + t);
-                const uint32_t e = j_lo * rev_dev(pos, a.k1);
-                g_out[(uint64_t)(j_lo + n_lo * pos) * a.ld_out + col] = fmul<F>(x[t], tw_at<true>(g_tw, e, mask));
-            }
+            for (int t = 0; t < E; t++)
+                g_out[(uint64_t)(j_lo + n_lo * (uint32_t)(p * E + t)) * a.ld_out + col] = x[t];
         }
     } else {  // KIND_D
         const uint32_t n_lo = 1u << a.k2;
@@ -477,7 +541,8 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t cs = (uint32_t)(rest / n_lo);
 #pragma unroll
         for (int t = 0; t < E; t++)
-            x[t] = act ? g_tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + col] : 0u;
+            x[t] = g_tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + colr];
+        issue_fence();
         tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, p, c, g_tw, a.k2, mask);
         if (act) {
 #pragma unroll
@@ -496,6 +561,9 @@ __global__ void twiddle_kernel(uint32_t* tw, uint32_t n, uint32_t root) {
     if (e < n) tw[e] = fpow<F>(root, e);
 }
 
+// scale[c h + i] = shift_c^{rev_h(i)} / h: entry i multiplies the coefficient
+// that the inverse DFT leaves at bit-reversed position i, so a tile reads its
+// scale factors contiguously.
 template <class F>
 __global__ void coset_scale_kernel(uint32_t* scale, int log_h, int b, uint32_t shift,
                                    uint32_t root_n, uint32_t h_inv) {
@@ -504,7 +572,7 @@ __global__ void coset_scale_kernel(uint32_t* scale, int log_h, int b, uint32_t s
     if (i >= n) return;
     const uint32_t cs = (uint32_t)(i >> log_h), k = (uint32_t)(i & ((1ull << log_h) - 1));
     const uint32_t shift_c = fmul<F>(shift, fpow<F>(root_n, rev_dev(cs, b)));
-    scale[i] = fmul<F>(fpow<F>(shift_c, k), h_inv);
+    scale[i] = fmul<F>(fpow<F>(shift_c, rev_dev(k, log_h)), h_inv);
 }
 
 template <class F>
