@@ -147,6 +147,33 @@ int pg_pcs_commit(pg_ctx *ctx, int field, const pg_buf *const *evals,
  * returned in natural order (rows * width words). */
 int pg_pcs_evaluations(pg_ctx *ctx, const pg_tree *tree, size_t k, size_t rows, uint32_t *host);
 
+/* ---- Pcs::open (device part) -----------------------------------------------
+ * p3 TwoAdicFriPcs::open evaluates every committed matrix at its points,
+ * has the challenger observe the values (round -> matrix -> point -> column,
+ * circuit/fri.rs:84-99), samples alpha and reduces all committed rows to one
+ * ext4 vector per LDE height (the verifier side: circuit/fri.rs:118-241),
+ * which FRI then folds (pg_fri_commit_phase takes these buffers as inputs).
+ *
+ * pg_pcs_open_values: matrix k of tree t at npts ext4 points (4 Montgomery
+ * words each), from the trace-domain coset GEN <w_h> held in the first
+ * height >> log_blowup rows of its LDE.  out: npts x width x 4 words,
+ * point-major.  Points must lie off that coset (an ext4 point with a
+ * non-zero higher coefficient always does).
+ *
+ * pg_pcs_reduce_openings: rounds[r] are the committed trees in opening order;
+ * npoints[] (0..2) gives the point count of every matrix of every round in
+ * that order, points_mont the points (4 words each) and values_mont the
+ * opened values (per point, width x 4 words) in the same order.  Writes a
+ * new 2^lh x 4 buffer (ext4, committed row order) to out_by_log[lh] for each
+ * LDE log-height lh that has points, NULL elsewhere (33 entries).  At most
+ * 4 distinct points per height. */
+int pg_pcs_open_values(pg_ctx *ctx, const pg_tree *t, size_t k, uint32_t log_blowup,
+                       const uint32_t *points_mont, size_t npts, uint32_t *out_mont);
+int pg_pcs_reduce_openings(pg_ctx *ctx, const pg_tree *const *rounds, size_t nrounds,
+                           const uint32_t *npoints, const uint32_t *points_mont,
+                           const uint32_t *values_mont, const uint32_t alpha_mont[4],
+                           pg_buf **out_by_log);
+
 /* ---- batched commits ------------------------------------------------------
  * Many independent single-matrix Pcs::commit jobs (the reference proves the
  * shards of a batch in parallel rayon workers, vm/src/instances/machine/

@@ -296,6 +296,94 @@ void or_coset_lde(int f, const uint32_t *in, size_t h, size_t w, int added_bits,
     }
 }
 
+/* ------------------------------------------------------------ pcs open -- */
+/* Values of a committed matrix at ext4 points (p3 TwoAdicFriPcs::open's
+ * evaluation step; the points are then observed in circuit/fri.rs:84-99
+ * order).  The first h = N >> log_blowup rows of the LDE (committed order)
+ * hold f on GEN <w_h> in bit-reversed order.  Method: bring them to natural
+ * order, inverse DFT to the coefficients of g(X) = f(GEN X), then Horner at
+ * z / GEN in the extension. */
+void or_pcs_open_values(int f, const uint32_t *lde, size_t n, size_t w, int log_blowup,
+                        const uint32_t z[4], uint32_t *out) {
+    uint32_t p = P_OF(f);
+    size_t h = n >> log_blowup;
+    int lh = log2_exact(h);
+    uint32_t wh_inv = fpow(p, or_two_adic_generator(f, lh), p - 2), h_inv = fpow(p, (uint32_t)(h % p), p - 2);
+    uint32_t g_inv = fpow(p, or_generator(f), p - 2);
+    uint32_t u[4] = {fmul(p, z[0], g_inv), fmul(p, z[1], g_inv), fmul(p, z[2], g_inv), fmul(p, z[3], g_inv)};
+    uint32_t *col = (uint32_t *)malloc(h * sizeof(uint32_t));
+    for (size_t c = 0; c < w; c++) {
+        for (size_t j = 0; j < h; j++) col[j] = lde[rev_bits(j, lh) * w + c];
+        dft_inplace(p, col, h, wh_inv);
+        uint32_t acc[4] = {0, 0, 0, 0}, t[4];
+        for (size_t k = h; k-- > 0;) { /* acc = acc * u + coef_k */
+            or_ext_mul(f, acc, u, t);
+            t[0] = fadd(p, t[0], fmul(p, col[k], h_inv));
+            memcpy(acc, t, sizeof t);
+        }
+        memcpy(out + 4 * c, acc, sizeof acc);
+    }
+    free(col);
+}
+
+/* Reduced openings, restated from the verifier's loop (circuit/fri.rs:
+ * 118-241) applied to every row: matrices in the order given, each with its
+ * points; one alpha-power counter per LDE log-height, advanced per column;
+ * row i of a height-N matrix is x = GEN w_N^rev(i);
+ *   ro[lh][i] += (sum_k alpha^pow_k (p_k(z) - M[i][k])) / (z - x).
+ * mats[m] is N_m x w_m canonical; npts[m] points, points and values
+ * consecutive over (m, point) (values: w_m x 4 per point).  out[lh] is
+ * (2^lh x 4) or NULL; rows of heights that occur are overwritten. */
+void or_pcs_reduce(int f, size_t nmats, const uint32_t *const *mats, const size_t *heights,
+                   const size_t *widths, const uint32_t *npts, const uint32_t *points,
+                   const uint32_t *values, const uint32_t alpha[4], uint32_t *const *out) {
+    uint32_t p = P_OF(f);
+    for (size_t m = 0; m < nmats; m++)
+        if (out[log2_exact(heights[m])]) memset(out[log2_exact(heights[m])], 0, heights[m] * 16);
+    size_t pow_ctr[64] = {0};
+    size_t pi = 0, vi = 0;
+    for (size_t m = 0; m < nmats; m++) {
+        size_t n = heights[m], w = widths[m];
+        int lh = log2_exact(n);
+        uint32_t wn = or_two_adic_generator(f, lh);
+        for (uint32_t q = 0; q < npts[m]; q++, pi++) {
+            const uint32_t *z = points + 4 * pi;
+            const uint32_t *pz = values + 4 * vi;
+            /* alpha^pow for this (matrix, point)'s columns */
+            uint32_t *ap = (uint32_t *)malloc(w * 16 + 16);
+            uint32_t a0[4] = {1, 0, 0, 0}, t[4];
+            for (size_t e = 0; e < pow_ctr[lh]; e++) {
+                or_ext_mul(f, a0, alpha, t);
+                memcpy(a0, t, sizeof t);
+            }
+            for (size_t k = 0; k < w; k++) {
+                memcpy(ap + 4 * k, a0, 16);
+                or_ext_mul(f, a0, alpha, t);
+                memcpy(a0, t, sizeof t);
+            }
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < n; i++) {
+                uint32_t x = fmul(p, or_generator(f), fpow(p, wn, rev_bits(i, lh)));
+                uint32_t acc[4] = {0, 0, 0, 0}, d[4], tt[4];
+                for (size_t k = 0; k < w; k++) {
+                    uint32_t diff[4] = {fsub(p, pz[4 * k], mats[m][i * w + k]), pz[4 * k + 1], pz[4 * k + 2],
+                                        pz[4 * k + 3]};
+                    or_ext_mul(f, ap + 4 * k, diff, tt);
+                    for (int c = 0; c < 4; c++) acc[c] = fadd(p, acc[c], tt[c]);
+                }
+                uint32_t den[4] = {fsub(p, z[0], x), z[1], z[2], z[3]};
+                or_ext_inv(f, den, d);
+                or_ext_mul(f, acc, d, tt);
+                uint32_t *o = out[lh] + 4 * i;
+                for (int c = 0; c < 4; c++) o[c] = fadd(p, o[c], tt[c]);
+            }
+            free(ap);
+            pow_ctr[lh] += w;
+            vi += w;
+        }
+    }
+}
+
 void or_coset_lde_naive(int f, const uint32_t *in, size_t h, size_t w, int added_bits,
                         uint32_t shift, int bit_reverse, uint32_t *out) {
     /* coefficients by the O(h^2) inverse-DFT sum, then O(h*N) evaluation */

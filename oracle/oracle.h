@@ -26,6 +26,9 @@
  *   or_mmcs_*             p3 MerkleTreeMmcs (pinned by circuit/fri.rs:389-436)
  *   or_fri_fold           p3 fold_even_odd (pinned by circuit/fri.rs:288-387)
  *   or_ch_*               p3 DuplexChallenger (circuit/challenger.rs:57-227)
+ *   or_pcs_open_values    p3 TwoAdicFriPcs::open, evaluation at the opening points
+ *   or_pcs_reduce         p3 TwoAdicFriPcs::open, reduced openings (the verifier's
+ *                         loop, circuit/fri.rs:118-241, applied to every row)
  */
 #ifndef PICO_ORACLE_H
 #define PICO_ORACLE_H
@@ -74,6 +77,11 @@ void or_fill_random(int field, uint64_t seed, uint32_t *out, size_t n);
 void or_coset_lde(int field, const uint32_t *in, size_t h, size_t w, int added_bits,
                   uint32_t shift, int bit_reverse, uint32_t *out);
 /* Same result by direct O(h*N) Lagrange-free evaluation (tiny sizes only). */
+void or_pcs_open_values(int field, const uint32_t *lde, size_t n, size_t w, int log_blowup,
+                        const uint32_t z[4], uint32_t *out);
+void or_pcs_reduce(int field, size_t nmats, const uint32_t *const *mats, const size_t *heights,
+                   const size_t *widths, const uint32_t *npts, const uint32_t *points,
+                   const uint32_t *values, const uint32_t alpha[4], uint32_t *const *out);
 void or_coset_lde_naive(int field, const uint32_t *in, size_t h, size_t w, int added_bits,
                         uint32_t shift, int bit_reverse, uint32_t *out);
 

@@ -55,6 +55,8 @@ def lib():
             "or_ch_sample_bits": (u32, [P, i]), "or_ch_check_witness": (i, [P, i, u32]),
             "or_fri_commit_phase": (i, [P, P, i, i, P, P, P, P, P]),
             "or_num_threads": (i, []),
+            "or_pcs_open_values": (None, [i, P, sz, sz, i, P, P]),
+            "or_pcs_reduce": (None, [i, sz, P, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -203,6 +205,38 @@ def mmcs_verify(field, heights, widths, root, index, rows, path):
 
 
 # -------------------------------------------------------------------- FRI --
+def pcs_open_values(field, lde, log_blowup, z):
+    """Values (w x 4) of a committed LDE (committed row order) at ext4 z."""
+    a = _u32(lde)
+    n, w = a.shape
+    out = np.zeros((w, 4), np.uint32)
+    lib().or_pcs_open_values(field, _ptr(a), n, w, log_blowup, _ptr(_u32(z)), _ptr(out))
+    return out
+
+
+def pcs_reduce(field, mats, npts, points, values, alpha):
+    """Reduced openings.  mats: committed LDEs in opening order; npts[m]
+    points for matrix m; points: (sum npts) x 4; values: per (matrix, point)
+    a w_m x 4 array, in order.  Returns {log_height: (2^lh x 4) array}."""
+    mats = [_u32(m) for m in mats]
+    hs = np.array([m.shape[0] for m in mats], np.uintp)
+    ws = np.array([m.shape[1] for m in mats], np.uintp)
+    ptrs = (ctypes.c_void_p * len(mats))(*[_ptr(m).value for m in mats])
+    np_ = np.ascontiguousarray(npts, dtype=np.uint32)
+    pts = _u32(np.asarray(points).reshape(-1, 4)) if len(points) else np.zeros((1, 4), np.uint32)
+    vals = _u32(np.concatenate([np.asarray(v).reshape(-1, 4) for v in values])) if values else np.zeros((1, 4), np.uint32)
+    outs = {}
+    arr = (ctypes.c_void_p * 64)()
+    for m, k in zip(mats, np_):
+        lh = m.shape[0].bit_length() - 1
+        if k and lh not in outs:
+            outs[lh] = np.zeros((m.shape[0], 4), np.uint32)
+            arr[lh] = _ptr(outs[lh]).value
+    lib().or_pcs_reduce(field, len(mats), ptrs, _ptr(hs), _ptr(ws), _ptr(np_), _ptr(pts), _ptr(vals),
+                        _ptr(_u32(alpha)), arr)
+    return outs
+
+
 def fri_fold(field, v, beta):
     v = _u32(v).reshape(-1, 4)
     n = v.shape[0]

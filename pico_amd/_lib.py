@@ -77,6 +77,8 @@ SIGNATURES = {
     "pg_tree_free": (None, [_P]),
     "pg_pcs_commit": (_i, [_P, _i, _P, _P, _sz, _u32, _P, _pp]),
     "pg_pcs_evaluations": (_i, [_P, _P, _sz, _sz, _P]),
+    "pg_pcs_open_values": (_i, [_P, _P, _sz, _u32, _P, _sz, _P]),
+    "pg_pcs_reduce_openings": (_i, [_P, _P, _sz, _P, _P, _P, _P, _P]),
     "pg_batch_create": (_i, [_P, _i, _P, _P, _P, _sz, _i, _pp]),
     "pg_batch_run": (_i, [_P, _P, _P]),
     "pg_batch_take_tree": (_i, [_P, _sz, _pp]),

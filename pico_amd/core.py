@@ -305,6 +305,78 @@ class TwoAdicFriPcs:
         check(lib().pg_pcs_evaluations(self.ctx.handle, data.handle, idx, domain.size(), _ptr(out)))
         return from_monty(self.field, out)
 
+    # ---- open -------------------------------------------------------------
+    def open_values(self, data, idx, points):
+        """Matrix `idx` of committed `data` at canonical ext4 points
+        (npts x 4) -> canonical values, npts x width x 4."""
+        pts = np.ascontiguousarray(to_monty(self.field, np.asarray(points, np.uint64).reshape(-1, 4)))
+        m = lib().pg_tree_matrix(data.handle, idx)
+        w = lib().pg_buf_cols(m)
+        out = np.zeros((len(pts), w, 4), np.uint32)
+        check(lib().pg_pcs_open_values(self.ctx.handle, data.handle, idx, self.log_blowup, _ptr(pts),
+                                       len(pts), _ptr(out)))
+        return from_monty(self.field, out)
+
+    def reduce_openings(self, rounds, values, alpha):
+        """rounds: [(ProverData, [points of matrix 0, points of matrix 1, ...])]
+        with each matrix's points an (n x 4) canonical array (n <= 2);
+        values: the opened values in the same nesting (per matrix, n x w x 4).
+        Returns {log_height: DeviceMatrix (2^lh x 4, ext4 rows)}."""
+        trees = (ctypes.c_void_p * len(rounds))(*[d.handle for d, _ in rounds])
+        npts, pts, vals = [], [], []
+        for (d, mat_points), mat_values in zip(rounds, values):
+            for p, v in zip(mat_points, mat_values):
+                p = np.asarray(p, np.uint64).reshape(-1, 4)
+                npts.append(len(p))
+                pts.append(p)
+                vals.append(np.asarray(v, np.uint64).reshape(-1, 4))
+        npts = np.ascontiguousarray(npts, dtype=np.uint32)
+        pts = np.ascontiguousarray(to_monty(self.field, np.concatenate(pts) if pts else np.zeros((1, 4))))
+        vals = np.ascontiguousarray(to_monty(self.field, np.concatenate(vals) if vals else np.zeros((1, 4))))
+        a = np.ascontiguousarray(to_monty(self.field, np.asarray(alpha, np.uint64)))
+        outs = (ctypes.c_void_p * 33)()
+        check(lib().pg_pcs_reduce_openings(self.ctx.handle, trees, len(rounds), _ptr(npts), _ptr(pts), _ptr(vals),
+                                           _ptr(a), outs))
+        return {lh: DeviceMatrix(self.ctx, outs[lh], owned=True) for lh in range(33) if outs[lh]}
+
+    def open(self, rounds, challenger, num_queries, pow_bits, log_final=None):
+        """TwoAdicFriPcs::open: rounds = [(ProverData, [points per matrix])].
+        Evaluates, has the challenger observe every value (round -> matrix ->
+        point -> column, 4 coefficients each), samples alpha, reduces, runs the
+        FRI commit phase down to 2^log_final (default log_blowup, the
+        reference's stop, final vector constant), grinds, samples the query
+        indices and answers them.  Returns (opened_values, proof dict)."""
+        opened = []
+        for d, mat_points in rounds:
+            per_mat = []
+            for k, p in enumerate(mat_points):
+                p = np.asarray(p, np.uint64).reshape(-1, 4)
+                per_mat.append(self.open_values(d, k, p) if len(p) else np.zeros((0, 0, 4), np.uint32))
+            opened.append(per_mat)
+        for per_mat in opened:
+            for v in per_mat:
+                challenger.observe(np.asarray(v).ravel())
+        alpha = challenger.sample_ext()
+        ro = self.reduce_openings(rounds, opened, alpha)
+        log_stop = self.log_blowup if log_final is None else log_final
+        fri = fri_commit_phase(self.ctx, self.field, challenger, ro, log_stop,
+                               final_mode=0 if log_final is None else 1)
+        witness = fri_grind(self.ctx, self.field, challenger, pow_bits)
+        log_max = max(ro)
+        idx = [challenger.sample_bits(log_max) for _ in range(num_queries)]
+        mmcs = MerkleTreeMmcs(self.ctx, self.field)
+        inputs = []
+        for i in idx:
+            per_round = []
+            for d, _ in rounds:
+                per_round.append(mmcs.open_batch(i >> (log_max - d.log_height), d))
+            inputs.append(per_round)
+        sib, paths = fri_answer_queries(fri, idx, log_max)
+        proof = {"alpha": alpha, "commits": fri.commits, "betas": fri.betas, "final_poly": fri.final_poly,
+                 "pow_witness": witness, "query_indices": idx, "input_openings": inputs,
+                 "fri_siblings": sib, "fri_paths": paths, "reduced_openings": ro, "fri_data": fri}
+        return opened, proof
+
 
 class CommitBatch:
     """Many single-matrix Pcs::commit jobs as one launch sequence (optionally a

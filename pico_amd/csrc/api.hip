@@ -954,6 +954,230 @@ extern "C" int pg_pcs_evaluations(pg_ctx* ctx, const pg_tree* t, size_t k, size_
     return PG_OK;
 }
 
+// ------------------------------------------------------------ PCS open --
+// Device part of TwoAdicFriPcs::open; the caller's challenger observes the
+// values in between (circuit/fri.rs:84-99 is the verifier's order).
+
+template <class F>
+static Ext4 ext_from(const uint32_t* w) {
+    return Ext4{{w[0], w[1], w[2], w[3]}};
+}
+
+// factor = (z^h - s^h) / (h s^(h-1)) of the barycentric form on s <w_h>
+template <class F>
+static Ext4 bary_factor(const Ext4& z, uint32_t s_mont, int log_h) {
+    const uint64_t h = 1ull << log_h;
+    Ext4 zh = epow<F>(z, h);
+    zh.c[0] = fsub<F>(zh.c[0], fpow<F>(s_mont, h));
+    const uint32_t hm = to_monty<F>((uint32_t)(h % F::P));
+    const uint32_t den = fmul<F>(hm, fpow<F>(s_mont, h - 1));
+    return escale<F>(zh, finv<F>(den));
+}
+
+template <class F>
+static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k, uint32_t log_blowup,
+                         const uint32_t* points, size_t npts, uint32_t* out) {
+    const pg_buf* m = t->mats[k];
+    if (!is_pow2(m->rows) || (m->rows >> log_blowup) == 0 || log2_of(m->rows) < (int)log_blowup)
+        return fail(PG_E_INVALID, "matrix %zu: height %zu has no trace coset at log_blowup %u", k, m->rows,
+                    log_blowup);
+    const int log_h = log2_of(m->rows) - (int)log_blowup;
+    const size_t h = (size_t)1 << log_h, w = m->cols;
+    if (npts == 0 || w == 0) return PG_OK;
+    std::shared_ptr<DevMem> tw;
+    PG_TRY(get_twiddles(ctx, s, t->field, log_h, &tw));
+    const uint32_t chunk_rows = 256;
+    const uint32_t nchunks = (uint32_t)((h + chunk_rows - 1) / chunk_rows);
+    const size_t b_w = h * 16, b_p = (size_t)nchunks * w * 16, b_o = npts * w * 16;
+    int rc;
+    auto scratch = make_devmem(ctx, b_w + b_p + b_o, &rc);
+    if (!scratch) return rc;
+    uint8_t* base = (uint8_t*)scratch->ptr;
+    const uint32_t s_mont = to_monty<F>(F::GEN);
+    auto kt = make_timer(ctx);
+    for (size_t q = 0; q < npts; q++) {
+        const Ext4 z = ext_from<F>(points + 4 * q);
+        const Ext4 fac = bary_factor<F>(z, s_mont, log_h);
+        TIMED(kt, "open_values", s,
+              launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, (const uint32_t*)tw->ptr, z, fac,
+                                 (Ext4*)base, (Ext4*)(base + b_w), nchunks, chunk_rows,
+                                 (Ext4*)(base + b_w + b_p + q * w * 16), s));
+    }
+    HIP_TRY(hipMemcpyAsync(out, base + b_w + b_p, b_o, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return PG_OK;
+}
+
+extern "C" int pg_pcs_open_values(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t log_blowup,
+                                  const uint32_t* points_mont, size_t npts, uint32_t* out_mont) {
+    if (!ctx || !t || (npts && (!points_mont || !out_mont)) || k >= t->mats.size())
+        return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    return t->field == PG_KOALABEAR ? open_values_t<KoalaBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont)
+                                    : open_values_t<BabyBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont);
+}
+
+template <class F>
+static int reduce_openings_t(pg_ctx* ctx, hipStream_t s, const pg_tree* const* rounds, size_t nrounds,
+                             const uint32_t* npoints, const uint32_t* points, const uint32_t* values,
+                             const uint32_t* alpha_w, pg_buf** out_by_log) {
+    const int field = rounds[0]->field;
+    const Ext4 alpha = ext_from<F>(alpha_w);
+    // walk the batches in round / matrix / point order, as the verifier does
+    struct Item {
+        const pg_buf* m;
+        int lh;
+        RoMat rm;
+        Ext4 z[2];
+    };
+    std::vector<Item> items;
+    size_t maxw = 0, pi = 0, vi = 0, mi = 0;
+    uint64_t reduced[64] = {0};
+    for (size_t r = 0; r < nrounds; r++) {
+        const pg_tree* t = rounds[r];
+        if (!t || t->field != field) return fail(PG_E_INVALID, "round %zu: missing tree or mixed fields", r);
+        for (size_t k = 0; k < t->mats.size(); k++, mi++) {
+            const pg_buf* m = t->mats[k];
+            const uint32_t np = npoints[mi];
+            if (np > 2) return fail(PG_E_UNSUPPORTED, "more than 2 points on one matrix");
+            if (!is_pow2(m->rows)) return fail(PG_E_INVALID, "height %zu is not a power of two", m->rows);
+            maxw = std::max(maxw, m->cols);
+            Item it{};
+            it.m = m;
+            it.lh = log2_of(m->rows);
+            it.rm.data = m->ptr;
+            it.rm.ld = m->cols;
+            it.rm.width = (uint32_t)m->cols;
+            it.rm.vec4 = (m->cols % 4 == 0) && ((uintptr_t)m->ptr % 16 == 0);
+            it.rm.npts = (int)np;
+            for (uint32_t q = 0; q < np; q++, pi++) {
+                it.z[q] = ext_from<F>(points + 4 * pi);
+                // alpha^off and alpha^off * sum_c alpha^c p_c(z)
+                const Ext4 aoff = epow<F>(alpha, reduced[it.lh]);
+                Ext4 acc{{0, 0, 0, 0}}, ap{{F::R1, 0, 0, 0}};
+                for (size_t c = 0; c < m->cols; c++, vi++) {
+                    acc = eadd<F>(acc, emul<F>(ap, ext_from<F>(values + 4 * vi)));
+                    ap = emul<F>(ap, alpha);
+                }
+                it.rm.aoff[q] = aoff;
+                it.rm.a[q] = emul<F>(aoff, acc);
+                reduced[it.lh] += m->cols;
+            }
+            if (np) items.push_back(it);
+        }
+    }
+    for (int lh = 0; lh < 33; lh++) out_by_log[lh] = nullptr;
+    // alpha^c for c < the widest matrix
+    std::vector<Ext4> apows(std::max<size_t>(maxw, 1));
+    apows[0] = Ext4{{F::R1, 0, 0, 0}};
+    for (size_t c = 1; c < apows.size(); c++) apows[c] = emul<F>(apows[c - 1], alpha);
+    std::vector<int> heights;
+    for (auto& it : items)
+        if (std::find(heights.begin(), heights.end(), it.lh) == heights.end()) heights.push_back(it.lh);
+    std::sort(heights.rbegin(), heights.rend());
+    // one device block: alpha powers, then per height the point and matrix tables
+    size_t bytes = align_up(apows.size() * 16, 256);
+    for (int lh : heights) {
+        size_t nm = 0;
+        for (auto& it : items) nm += it.lh == lh;
+        bytes += align_up(RO_MAX_POINTS * 16, 256) + align_up(nm * sizeof(RoMat), 256);
+    }
+    std::vector<uint8_t> host(bytes, 0);
+    memcpy(host.data(), apows.data(), apows.size() * 16);
+    struct Launch {
+        int lh, nm, np;
+        size_t off_pts, off_mats;
+    };
+    std::vector<Launch> launches;
+    size_t off = align_up(apows.size() * 16, 256);
+    for (int lh : heights) {
+        std::vector<Ext4> pts;
+        std::vector<RoMat> rms;
+        for (auto& it : items) {
+            if (it.lh != lh) continue;
+            RoMat rm = it.rm;
+            for (int q = 0; q < rm.npts; q++) {
+                int found = -1;
+                for (size_t u = 0; u < pts.size(); u++)
+                    if (!memcmp(pts[u].c, it.z[q].c, 16)) found = (int)u;
+                if (found < 0) {
+                    if (pts.size() == RO_MAX_POINTS)
+                        return fail(PG_E_UNSUPPORTED, "more than %d distinct points at one height", RO_MAX_POINTS);
+                    pts.push_back(it.z[q]);
+                    found = (int)pts.size() - 1;
+                }
+                rm.pt[q] = found;
+            }
+            rms.push_back(rm);
+        }
+        Launch la{lh, (int)rms.size(), (int)pts.size(), off, off + align_up(RO_MAX_POINTS * 16, 256)};
+        memcpy(host.data() + la.off_pts, pts.data(), pts.size() * 16);
+        memcpy(host.data() + la.off_mats, rms.data(), rms.size() * sizeof(RoMat));
+        off = la.off_mats + align_up(rms.size() * sizeof(RoMat), 256);
+        launches.push_back(la);
+    }
+    int rc;
+    auto tables = make_devmem(ctx, bytes, &rc);
+    if (!tables) return rc;
+    uint8_t* dt = (uint8_t*)tables->ptr;
+    HIP_TRY(hipMemcpyAsync(dt, host.data(), bytes, hipMemcpyHostToDevice, s));
+    std::vector<pg_buf*> made;
+    auto cleanup = [&]() {
+        for (auto* b : made) pg_buf_free(b);
+    };
+    auto kt = make_timer(ctx);
+    std::vector<std::shared_ptr<DevMem>> keep;
+    for (auto& la : launches) {
+        std::shared_ptr<DevMem> tw;
+        rc = get_twiddles(ctx, s, field, la.lh, &tw);
+        if (rc != PG_OK) {
+            cleanup();
+            return rc;
+        }
+        keep.push_back(tw);
+        pg_buf* o;
+        rc = buf_new(ctx, (size_t)1 << la.lh, 4, &o);
+        if (rc != PG_OK) {
+            cleanup();
+            return rc;
+        }
+        made.push_back(o);
+        out_by_log[la.lh] = o;
+        if (kt) kt->begin("reduce_openings", s);
+        hipError_t e = launch_reduce_rows(field, (const RoMat*)(dt + la.off_mats), la.nm, (const Ext4*)(dt + la.off_pts),
+                                          la.np, (const Ext4*)dt, la.lh, (const uint32_t*)tw->ptr, (Ext4*)o->ptr, s);
+        if (kt) kt->end(s);
+        if (e != hipSuccess) {
+            cleanup();
+            for (int lh = 0; lh < 33; lh++) out_by_log[lh] = nullptr;
+            return fail(PG_E_DEVICE, "reduce_rows: %s", hipGetErrorString(e));
+        }
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        cleanup();
+        for (int lh = 0; lh < 33; lh++) out_by_log[lh] = nullptr;
+        return fail(PG_E_DEVICE, "reduce_openings: %s", hipGetErrorString(e));
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_pcs_reduce_openings(pg_ctx* ctx, const pg_tree* const* rounds, size_t nrounds,
+                                      const uint32_t* npoints, const uint32_t* points_mont,
+                                      const uint32_t* values_mont, const uint32_t alpha_mont[4],
+                                      pg_buf** out_by_log) {
+    if (!ctx || !rounds || nrounds == 0 || !npoints || !alpha_mont || !out_by_log || !rounds[0])
+        return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    return rounds[0]->field == PG_KOALABEAR
+               ? reduce_openings_t<KoalaBear>(ctx, l.s, rounds, nrounds, npoints, points_mont, values_mont,
+                                              alpha_mont, out_by_log)
+               : reduce_openings_t<BabyBear>(ctx, l.s, rounds, nrounds, npoints, points_mont, values_mont,
+                                             alpha_mont, out_by_log);
+}
+
 // ----------------------------------------------------------------- batch --
 // Many independent single-matrix PCS commits as one launch sequence: each
 // LDE pass runs once per distinct log-height over all jobs of that height,

@@ -181,6 +181,42 @@ PG_HD Ext4 emul(const Ext4& a, const Ext4& b) {
     return r;
 }
 
+// Inverse in F[x]/(x^4 - W).  With y = x^2, a = E(y) + x O(y) and
+// a(-x) = E - x O, the product a a(-x) = E^2 - y O^2 = B0 + B1 y lies in
+// F[y]/(y^2 - W), where (B0 + B1 y)^-1 = (B0 - B1 y) / (B0^2 - W B1^2).  So
+// a^-1 = a(-x) (B0 - B1 y) / N: one base-field inversion.  a = 0 gives 0.
+template <class F>
+PG_HD Ext4 einv(const Ext4& a) {
+    const uint32_t a0 = a.c[0], a1 = a.c[1], a2 = a.c[2], a3 = a.c[3];
+    // B0 = a0^2 + W a2^2 - 2 W a1 a3,  B1 = 2 a0 a2 - a1^2 - W a3^2
+    const uint32_t a1a3 = fmul<F>(a1, a3);
+    const uint32_t b0 = fsub<F>(fadd<F>(fmul<F>(a0, a0), mul_by_w<F>(fmul<F>(a2, a2))),
+                                mul_by_w<F>(fadd<F>(a1a3, a1a3)));
+    const uint32_t a0a2 = fmul<F>(a0, a2);
+    const uint32_t b1 = fsub<F>(fsub<F>(fadd<F>(a0a2, a0a2), fmul<F>(a1, a1)), mul_by_w<F>(fmul<F>(a3, a3)));
+    const uint32_t n = fsub<F>(fmul<F>(b0, b0), mul_by_w<F>(fmul<F>(b1, b1)));
+    const uint32_t ninv = finv<F>(n);
+    const uint32_t c0 = fmul<F>(b0, ninv), c1 = fmul<F>(fsub<F>(0u, b1), ninv);  // (B0 - B1 y) / N
+    // a(-x) = a0 - a1 x + a2 x^2 - a3 x^3 times c0 + c1 x^2
+    const uint32_t m1 = fsub<F>(0u, a1), m3 = fsub<F>(0u, a3);
+    Ext4 r;
+    r.c[0] = fadd<F>(fmul<F>(a0, c0), mul_by_w<F>(fmul<F>(a2, c1)));
+    r.c[1] = fadd<F>(fmul<F>(m1, c0), mul_by_w<F>(fmul<F>(m3, c1)));
+    r.c[2] = fadd<F>(fmul<F>(a2, c0), fmul<F>(a0, c1));
+    r.c[3] = fadd<F>(fmul<F>(m3, c0), fmul<F>(m1, c1));
+    return r;
+}
+template <class F>
+PG_HD Ext4 epow(Ext4 a, uint64_t e) {
+    Ext4 r{{F::R1, 0u, 0u, 0u}};
+    while (e) {
+        if (e & 1) r = emul<F>(r, a);
+        a = emul<F>(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
 // reverse the low `bits` bits of x (host and device; kernels use __brev)
 PG_HD uint32_t bit_rev(uint32_t x, int bits) {
     x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);

@@ -3,6 +3,8 @@
 // hipError_t of the launch; none of them synchronises.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "field.h"
 #include <stddef.h>
 #include <stdint.h>
 
@@ -117,6 +119,31 @@ hipError_t launch_coset_lde(int field, const uint32_t* in, size_t ld_in, int log
                             uint32_t* tmp, uint32_t* out, size_t ld_out, int bit_reverse,
                             hipStream_t stream, KernelTimer* kt = nullptr);
 int lde_max_log_h();
+
+// ---- Pcs::open (open.hip) ------------------------------------------------
+// Values of one matrix at z: weights (h Ext4) and partial sums (nchunks x w
+// Ext4) are caller-provided scratch; out gets w Ext4 = factor * sum_r M[r] c_r
+// over the first h = 2^log_h rows, tw = w_h^e table.
+hipError_t launch_open_values(int field, const uint32_t* m, uint64_t ld, uint32_t w, int log_h,
+                              const uint32_t* tw, const Ext4& z, const Ext4& factor, Ext4* weights,
+                              Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* out,
+                              hipStream_t stream);
+// Reduced opening of one LDE log-height (see open.hip).  mats / pts are
+// device tables; alpha_pows[k] = alpha^k for k < the widest matrix.
+constexpr int RO_MAX_POINTS = 4;  // distinct points per log-height
+struct RoMat {
+    const uint32_t* data;
+    uint64_t ld;
+    uint32_t width;
+    int vec4;           // rows 16-byte aligned (width, ld multiples of 4)
+    int npts;           // points opened on this matrix (<= 2)
+    int pt[2];          // index into the height's distinct points
+    Ext4 a[2];          // alpha^off * sum_k alpha^k p_k(z)
+    Ext4 aoff[2];       // alpha^off
+};
+hipError_t launch_reduce_rows(int field, const RoMat* mats, int nmats, const Ext4* pts, int npts,
+                              const Ext4* alpha_pows, int log_n, const uint32_t* tw, Ext4* out,
+                              hipStream_t stream);
 
 // ---- FRI (fri.hip) ----------------------------------------------------------
 // out[i] = (1/2 + beta/(2 x_i)) v[2i] + (1/2 - beta/(2 x_i)) v[2i+1],

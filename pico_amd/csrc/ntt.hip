@@ -361,11 +361,15 @@ constexpr int shape_log_e(int log_s, int kind) {
 }
 
 // Columns per tile: 16 (64-byte row runs) up to 1024 threads per workgroup;
-// 8 for the two-level 2^11 tile when it also parks coefficients in LDS
-// (exchange + coefficient buffers <= 128 KiB).
+// 8 for the coset-looping passes' three-level tiles (two 512-thread
+// workgroups per CU desynchronise their barriers better than one of 1024:
+// measured 8% on 2^10 BC) and for the two-level 2^11 tile when it also parks
+// coefficients in LDS (exchange + coefficient buffers <= 128 KiB).
 constexpr int cols_for(int log_e, int log_p, int kind) {
-    return log_p > log_e ? (1024 >> log_p < 16 ? 1024 >> log_p : 16)
-           : (log_e + log_p >= 11 && (kind == KIND_BC || kind == KIND_SINGLE)) ? 8 : 16;
+    const bool coset_loop = kind == KIND_BC || kind == KIND_SINGLE;
+    const int cmax = coset_loop ? 8 : 16;
+    return log_p > log_e ? (1024 >> log_p < cmax ? 1024 >> log_p : cmax)
+           : (log_e + log_p >= 11 && coset_loop) ? 8 : 16;
 }
 constexpr int threads_for(int log_e, int log_p, int kind) {
     return cols_for(log_e, log_p, kind) * (1 << log_p) >= 256 ? cols_for(log_e, log_p, kind) * (1 << log_p) : 256;

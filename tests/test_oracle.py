@@ -288,3 +288,135 @@ def test_seeded_inputs(oracle, f):
     assert [int(x) for x in m] == [pyref.splitmix_element(f, 0xC0FFEE, i) for i in range(64)]
     big = oracle.random_matrix(f, 1, 1 << 12, 16)
     assert big.max() < pyref.P[f]
+
+
+# ------------------------------------------------------------------ pcs open --
+def _ext_pow(f, a, e):
+    r = [1, 0, 0, 0]
+    for _ in range(e):
+        r = pyref.ext_mul(f, r, a)
+    return r
+
+
+@pytest.mark.parametrize("f", [0, 1])
+def test_oracle_open_values_monomials(oracle, f):
+    """The value at z of the column holding x^k on the trace domain is z^k.
+    Evaluation is linear in the column, and the monomials k < h span every
+    column, so this pins or_pcs_open_values completely; linearity is checked
+    on a random combination."""
+    p = pyref.P[f]
+    rng = np.random.default_rng(5 + f)
+    for lh, b in [(0, 1), (1, 2), (3, 1), (3, 3), (5, 2)]:
+        h = 1 << lh
+        w = pyref.root_of_unity(f, lh)
+        trace = np.array([[pow(w, j * k, p) for k in range(h)] for j in range(h)], np.uint32)
+        lde = oracle.coset_lde(f, trace, b, pyref.GEN[f])
+        z = [int(x) for x in rng.integers(0, p, 4)]
+        vals = oracle.pcs_open_values(f, lde, b, z)
+        for k in range(h):
+            assert [int(x) for x in vals[k]] == _ext_pow(f, z, k), (lh, b, k)
+    # linearity: open(3 A + 5 B) = 3 open(A) + 5 open(B)
+    a = oracle.random_matrix(f, 1, 16, 2)
+    bm = oracle.random_matrix(f, 2, 16, 2)
+    c = ((3 * a.astype(np.uint64) + 5 * bm.astype(np.uint64)) % p).astype(np.uint32)
+    z = [7, 11, 13, 17]
+    va, vb, vc = (oracle.pcs_open_values(f, oracle.coset_lde(f, m, 1, pyref.GEN[f]), 1, z) for m in (a, bm, c))
+    assert np.array_equal(vc, (3 * va.astype(np.uint64) + 5 * vb.astype(np.uint64)) % p)
+
+
+def _coset_coeffs(f, col, shift):
+    """Coefficients (natural order) of the polynomial whose values on
+    shift * <w_n> in bit-reversed order are col."""
+    p = pyref.P[f]
+    n = len(col)
+    ln = n.bit_length() - 1
+    nat = [0] * n
+    for i in range(n):
+        nat[pyref.rev(i, ln)] = int(col[i])
+    g = pyref.interpolate_coeffs(f, nat)  # g(X) = poly(shift X)
+    sinv = pow(shift, p - 2, p)
+    return [g[k] * pow(sinv, k, p) % p for k in range(n)]
+
+
+def _reduce_setup(oracle, f, corrupt=False):
+    p = pyref.P[f]
+    rng = np.random.default_rng(40 + f)
+    shapes = [(4, 3, 2), (3, 2, 2), (4, 1, 1)]  # (log h, width, points) ; blowup 2 -> heights 32, 16, 32
+    mats, npts, pts, vals = [], [], [], []
+    zeta = [int(x) for x in rng.integers(0, p, 4)]
+    g_zeta = [zeta[i] * pyref.root_of_unity(f, 4) % p for i in range(4)]  # a second, shifted point
+    for i, (lh, w, k) in enumerate(shapes):
+        lde = oracle.coset_lde(f, oracle.random_matrix(f, 60 + i, 1 << lh, w), 1, pyref.GEN[f])
+        mats.append(lde)
+        npts.append(k)
+        for z in [zeta, g_zeta][:k]:
+            pts.append(z)
+            v = oracle.pcs_open_values(f, lde, 1, z)
+            if corrupt and i == 0:
+                v = v.copy()
+                v[0, 0] = (int(v[0, 0]) + 1) % p
+            vals.append(v)
+    alpha = [int(x) for x in rng.integers(0, p, 4)]
+    return mats, npts, np.array(pts, np.uint32), vals, alpha
+
+
+@pytest.mark.parametrize("f", [0, 1])
+def test_oracle_reduce_low_degree(oracle, f):
+    """With correct opened values every (p(z) - p(x)) / (z - x) is a
+    polynomial of degree < h - 1, so each height's reduced opening is a
+    low-degree vector on GEN <w_N>; a corrupted value breaks that."""
+    for corrupt in (False, True):
+        mats, npts, pts, vals, alpha = _reduce_setup(oracle, f, corrupt)
+        ro = oracle.pcs_reduce(f, mats, npts, pts, vals, alpha)
+        assert sorted(ro) == [4, 5]
+        for lh, h_trace in ((5, 16), (4, 8)):
+            for c in range(4):
+                coeffs = _coset_coeffs(f, ro[lh][:, c], pyref.GEN[f])
+                top_zero = all(x == 0 for x in coeffs[h_trace - 1:])
+                assert top_zero != (corrupt and lh == 5), (corrupt, lh, c)
+
+
+@pytest.mark.parametrize("f", [0, 1])
+def test_oracle_reduce_matches_verifier_loop(oracle, f):
+    """At sampled rows, the verifier's loop (circuit/fri.rs:118-241: alpha
+    powers counted per height across matrices, points and columns) gives
+    the oracle's value; checked cross-multiplied, without inverses."""
+    p = pyref.P[f]
+    mats, npts, pts, vals, alpha = _reduce_setup(oracle, f)
+    ro = oracle.pcs_reduce(f, mats, npts, pts, vals, alpha)
+    for lh in (5, 4):
+        n = 1 << lh
+        wn = pyref.root_of_unity(f, lh)
+        for i in (0, 1, 7, n - 1):
+            x = pyref.GEN[f] * pow(wn, pyref.rev(i, lh), p) % p
+            pow_ctr, acc_by_point, pi, vi = 0, {}, 0, 0
+            for m, k in zip(mats, npts):
+                for q in range(k):
+                    z = tuple(int(t) for t in pts[pi])
+                    v = vals[vi]
+                    pi += 1
+                    vi += 1
+                    if m.shape[0] != n:
+                        continue
+                    acc = [0, 0, 0, 0]
+                    for col in range(m.shape[1]):
+                        ap = _ext_pow(f, alpha, pow_ctr)
+                        diff = [(int(v[col, 0]) - int(m[i, col])) % p] + [int(t) for t in v[col, 1:]]
+                        acc = [(a + b) % p for a, b in zip(acc, pyref.ext_mul(f, ap, diff))]
+                        pow_ctr += 1
+                    prev = acc_by_point.get(z, [0, 0, 0, 0])
+                    acc_by_point[z] = [(a + b) % p for a, b in zip(prev, acc)]
+            # ro * prod (z - x) == sum_z acc_z * prod_{z' != z} (z' - x)
+            zs = list(acc_by_point)
+            dens = {z: [(z[0] - x) % p, z[1], z[2], z[3]] for z in zs}
+            lhs = [int(t) for t in ro[lh][i]]
+            for z in zs:
+                lhs = pyref.ext_mul(f, lhs, dens[z])
+            rhs = [0, 0, 0, 0]
+            for z in zs:
+                term = acc_by_point[z]
+                for z2 in zs:
+                    if z2 != z:
+                        term = pyref.ext_mul(f, term, dens[z2])
+                rhs = [(a + b) % p for a, b in zip(rhs, term)]
+            assert lhs == rhs, (lh, i)
