@@ -129,6 +129,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import numpy as np
+
     import pico_amd
 
     # Every C-ABI call returns after its device work has completed, so the
@@ -253,6 +255,32 @@ def main():
                          "h2d_GB_per_s": host_trace.nbytes / h2d_s / 1e9, "host_memory": "pageable",
                          "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
     del host_trace
+    # Pcs::open's device part on the same commitment: values at zeta and
+    # g zeta, then the reduced opening over all 2^21 x 256 LDE cells.
+    _, pd = pcs.commit([(dom, trace)])
+    p = pico_amd.PRIMES[f]
+    zeta = np.array([123456789, 987654321, 55555, 77777], np.uint64) % p
+    pts = np.stack([zeta, zeta * pico_amd.two_adic_generator(f, args.log_h) % p])
+    alpha = np.array([31, 41, 59, 26], np.uint64)
+    for rep in range(3):
+        ctx.profile(rep == 2)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        vals = pcs.open_values(pd, 0, pts)
+        t1 = time.perf_counter()
+        ro = pcs.reduce_openings([(pd, [pts])], [[vals]], alpha)
+        t2 = time.perf_counter()
+        for buf in ro.values():
+            buf.free()
+    ctx.profile(False)
+    red_ms = ctx.profile_get("reduce_openings")[0]
+    val_ms = ctx.profile_get("open_values")[0]
+    red_bytes = n_rows * w * 4 + n_rows * 16
+    out["pcs_open"] = {"points": 2, "open_values_ms": (t1 - t0) * 1e3, "open_values_kernel_ms": val_ms,
+                       "reduce_openings_ms": (t2 - t1) * 1e3, "reduce_kernel_ms": red_ms,
+                       "reduce_bytes": red_bytes, "reduce_GB_per_s": red_bytes / (red_ms / 1e3) / 1e9,
+                       "reduce_frac_hbm": red_bytes / (red_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    pd.free()
     trace.free()
     if not args.no_workloads:
         from pico_amd import workloads

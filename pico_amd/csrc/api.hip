@@ -986,22 +986,30 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
     if (npts == 0 || w == 0) return PG_OK;
     std::shared_ptr<DevMem> tw;
     PG_TRY(get_twiddles(ctx, s, t->field, log_h, &tw));
-    const uint32_t chunk_rows = 256;
+    // about 2^20 (chunk, column) threads, at least 16 rows per chunk
+    uint32_t chunk_rows = 16;
+    while ((uint64_t)chunk_rows * (1u << 20) < (uint64_t)h * w) chunk_rows <<= 1;
     const uint32_t nchunks = (uint32_t)((h + chunk_rows - 1) / chunk_rows);
-    const size_t b_w = h * 16, b_p = (size_t)nchunks * w * 16, b_o = npts * w * 16;
+    const size_t per = 4;  // points per launch
+    const size_t q_max = std::min(npts, per);
+    const size_t b_w = q_max * h * 16, b_p = q_max * nchunks * w * 16, b_o = npts * w * 16;
     int rc;
     auto scratch = make_devmem(ctx, b_w + b_p + b_o, &rc);
     if (!scratch) return rc;
     uint8_t* base = (uint8_t*)scratch->ptr;
     const uint32_t s_mont = to_monty<F>(F::GEN);
     auto kt = make_timer(ctx);
-    for (size_t q = 0; q < npts; q++) {
-        const Ext4 z = ext_from<F>(points + 4 * q);
-        const Ext4 fac = bary_factor<F>(z, s_mont, log_h);
+    for (size_t q0 = 0; q0 < npts; q0 += per) {
+        const int nq = (int)std::min(per, npts - q0);
+        Ext4 z[4], fac[4];
+        for (int q = 0; q < nq; q++) {
+            z[q] = ext_from<F>(points + 4 * (q0 + q));
+            fac[q] = bary_factor<F>(z[q], s_mont, log_h);
+        }
         TIMED(kt, "open_values", s,
-              launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, (const uint32_t*)tw->ptr, z, fac,
+              launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, (const uint32_t*)tw->ptr, z, fac, nq,
                                  (Ext4*)base, (Ext4*)(base + b_w), nchunks, chunk_rows,
-                                 (Ext4*)(base + b_w + b_p + q * w * 16), s));
+                                 (Ext4*)(base + b_w + b_p + q0 * w * 16), s));
     }
     HIP_TRY(hipMemcpyAsync(out, base + b_w + b_p, b_o, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -121,11 +121,12 @@ hipError_t launch_coset_lde(int field, const uint32_t* in, size_t ld_in, int log
 int lde_max_log_h();
 
 // ---- Pcs::open (open.hip) ------------------------------------------------
-// Values of one matrix at z: weights (h Ext4) and partial sums (nchunks x w
-// Ext4) are caller-provided scratch; out gets w Ext4 = factor * sum_r M[r] c_r
-// over the first h = 2^log_h rows, tw = w_h^e table.
+// Values of one matrix at npts <= 4 points z[q]: weights (npts x h Ext4) and
+// partial sums (npts x nchunks x w Ext4) are caller-provided scratch; out
+// gets npts x w Ext4, factor[q] * sum_r M[r] c_q[r] over the first
+// h = 2^log_h rows, tw = w_h^e table.
 hipError_t launch_open_values(int field, const uint32_t* m, uint64_t ld, uint32_t w, int log_h,
-                              const uint32_t* tw, const Ext4& z, const Ext4& factor, Ext4* weights,
+                              const uint32_t* tw, const Ext4* z, const Ext4* factor, int npts, Ext4* weights,
                               Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* out,
                               hipStream_t stream);
 // Reduced opening of one LDE log-height (see open.hip).  mats / pts are
