@@ -94,9 +94,34 @@ void pg_buf_free(pg_buf *buf);
 void *pg_buf_device_ptr(const pg_buf *buf);
 size_t pg_buf_rows(const pg_buf *buf);
 size_t pg_buf_cols(const pg_buf *buf);
+/* Non-owning strided view: row r of the view is row row0 + r * row_step of
+ * `parent`, columns col0 .. col0 + cols - 1.  It replaces the host-side
+ * re-layouts that sit in front of Pcs::commit in the reference:
+ *  - quotient chunks, `domain.split_evals(d, quotient_flat)`
+ *    (vm/src/machine/prover.rs:513-523): chunk i of an (n d) x 4 quotient is
+ *    view(row0 = i, row_step = d, col0 = 0, rows = n, cols = 4), committed on
+ *    `domain.split_domains(d)[i]` (shift s * w_{n d}^i);
+ *  - a column block of a wider device matrix.
+ * (`flatten_to_base` of an extension-field trace, prover.rs:404-411, needs no
+ * view: an h x w ext4 matrix in this layout is the h x 4w base matrix.)
+ * Views are accepted as LDE inputs (pg_coset_lde, pg_pcs_commit,
+ * pg_batch_create) and by pg_download; other entry points return
+ * PG_E_INVALID for them.  The parent must outlive the view and anything
+ * committed from it. */
+int pg_buf_view(pg_ctx *ctx, const pg_buf *parent, size_t row0, size_t row_step, size_t col0,
+                size_t rows, size_t cols, pg_buf **out);
+/* row pitch in words (cols for a dense matrix) */
+size_t pg_buf_ld(const pg_buf *buf);
 /* whole-buffer upload / row-range download.  canonical != 0 means the host
  * side holds canonical values of `field`, converted on the device. */
 int pg_upload(pg_ctx *ctx, pg_buf *buf, const uint32_t *host, int field, int canonical);
+/* Page-locked host memory (hipHostMalloc) for traces produced on the host:
+ * pg_upload from it runs at the link's rate, from pageable memory it goes
+ * through the runtime's staging copies (measured on MI355X, 1 GiB:
+ * 58 GB/s page-locked, 29-32 GB/s pageable).  Trace generation writes rows straight into it; no function
+ * here registers memory that already exists. */
+int pg_host_alloc(pg_ctx *ctx, size_t bytes, void **out);
+void pg_host_free(void *p);
 int pg_download(pg_ctx *ctx, const pg_buf *buf, size_t row0, size_t nrows, uint32_t *host,
                 int field, int canonical);
 /* Seeded synthetic input: element i of stream `seed` is the first of

@@ -58,9 +58,13 @@ SIGNATURES = {
     "pg_buf_alloc": (_i, [_P, _sz, _sz, _pp]),
     "pg_buf_wrap": (_i, [_P, _P, _sz, _sz, _pp]),
     "pg_buf_free": (None, [_P]),
+    "pg_host_alloc": (_i, [_P, _sz, _pp]),
+    "pg_host_free": (None, [_P]),
     "pg_buf_device_ptr": (_P, [_P]),
     "pg_buf_rows": (_sz, [_P]),
     "pg_buf_cols": (_sz, [_P]),
+    "pg_buf_ld": (_sz, [_P]),
+    "pg_buf_view": (_i, [_P, _P, _sz, _sz, _sz, _sz, _sz, _pp]),
     "pg_upload": (_i, [_P, _P, _P, _i, _i]),
     "pg_download": (_i, [_P, _P, _sz, _sz, _P, _i, _i]),
     "pg_fill_random": (_i, [_P, _P, _i, _u64]),

@@ -106,10 +106,38 @@ class Context:
         check(lib().pg_fill_random(self.handle, m.handle, field, seed))
         return m
 
+    def host_matrix(self, rows, cols):
+        """A rows x cols uint32 array in page-locked host memory (pg_host_alloc);
+        uploads from it run at the link's rate.  Free with .free()."""
+        return PinnedMatrix(self, rows, cols)
+
     def wrap(self, device_ptr, rows, cols):
         h = ctypes.c_void_p()
         check(lib().pg_buf_wrap(self.handle, ctypes.c_void_p(device_ptr), rows, cols, ctypes.byref(h)))
         return DeviceMatrix(self, h, owned=True)  # the view object itself is freed; memory is not
+
+
+class PinnedMatrix:
+    """Page-locked host buffer viewed as a numpy uint32 matrix (.array)."""
+
+    def __init__(self, ctx, rows, cols):
+        p = ctypes.c_void_p()
+        check(lib().pg_host_alloc(ctx.handle, rows * cols * 4, ctypes.byref(p)))
+        self.ptr = p
+        buf = (ctypes.c_uint32 * (rows * cols)).from_address(p.value)
+        self.array = np.ctypeslib.as_array(buf).reshape(rows, cols)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().pg_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceMatrix:
@@ -131,6 +159,25 @@ class DeviceMatrix:
 
     def width(self):
         return self.cols
+
+    @property
+    def ld(self):
+        """Row pitch in words (cols unless this is a strided view)."""
+        return lib().pg_buf_ld(self.handle)
+
+    def view(self, row0=0, row_step=1, col0=0, rows=None, cols=None):
+        """Rows row0 + row_step * r and columns col0 .. col0 + cols - 1, as a
+        matrix that shares this one's memory (it keeps this one alive).  Views
+        are LDE/commit inputs and can be downloaded; nothing writes to them."""
+        if rows is None:
+            rows = 0 if row0 >= self.rows else (self.rows - 1 - row0) // row_step + 1
+        cols = self.cols - col0 if cols is None else cols
+        h = ctypes.c_void_p()
+        check(lib().pg_buf_view(self.ctx.handle, self.handle, row0, row_step, col0, rows, cols,
+                                ctypes.byref(h)))
+        v = DeviceMatrix(self.ctx, h, owned=True)
+        v._parent = self
+        return v
 
     def to_host(self, field=None, row0=0, nrows=None):
         """Rows [row0, row0 + nrows): canonical values of `field`, or the raw
@@ -271,11 +318,36 @@ class MerkleTreeMmcs:
 
 # ------------------------------------------------------------------ PCS --
 class TwoAdicMultiplicativeCoset:
-    def __init__(self, log_n, shift=1):
-        self.log_n, self.shift = log_n, shift
+    """shift * <w_{2^log_n}>, shift canonical.  `field` is needed only by
+    split_domains (the chunk shifts are field elements)."""
+
+    def __init__(self, log_n, shift=1, field=None):
+        self.log_n, self.shift, self.field = log_n, shift, field
 
     def size(self):
         return 1 << self.log_n
+
+    def split_domains(self, num_chunks, field=None):
+        """Chunk i: shift * w_N^i * <w_{N / num_chunks}> (N = this size), the
+        domains of the quotient chunks (vm/src/machine/prover.rs:513-523;
+        verifier side vm/src/compiler/recursion/circuit/stark.rs:269-270)."""
+        f = self.field if field is None else field
+        lc = num_chunks.bit_length() - 1
+        if num_chunks != 1 << lc or lc > self.log_n:
+            raise ValueError("num_chunks must be a power of two <= the domain size")
+        p = PRIMES[f]
+        w = two_adic_generator(f, self.log_n)
+        return [TwoAdicMultiplicativeCoset(self.log_n - lc, self.shift * pow(w, i, p) % p, f)
+                for i in range(num_chunks)]
+
+    def split_evals(self, num_chunks, evals):
+        """Row i of `evals` (a DeviceMatrix of this domain's size) goes to
+        chunk i mod num_chunks: strided views of the same device memory, no
+        copy (the reference builds new host matrices, prover.rs:520)."""
+        if evals.rows != self.size() or num_chunks < 1 or self.size() % num_chunks:
+            raise ValueError("evaluations do not match the domain or the chunk count")
+        return [evals.view(row0=i, row_step=num_chunks, rows=self.size() // num_chunks)
+                for i in range(num_chunks)]
 
 
 class TwoAdicFriPcs:
@@ -285,7 +357,7 @@ class TwoAdicFriPcs:
         self.ctx, self.field, self.log_blowup = ctx, field, log_blowup
 
     def natural_domain_for_degree(self, n):
-        return TwoAdicMultiplicativeCoset(n.bit_length() - 1, 1)
+        return TwoAdicMultiplicativeCoset(n.bit_length() - 1, 1, self.field)
 
     def commit(self, evaluations):
         """evaluations: [(domain, DeviceMatrix)] -> (root, ProverData)."""

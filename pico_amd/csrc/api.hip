@@ -444,9 +444,18 @@ struct pg_buf {
     pg_ctx* ctx = nullptr;
     uint32_t* ptr = nullptr;
     size_t rows = 0, cols = 0;
+    size_t ld = 0;  // row pitch in words: cols, or more for a strided view
     size_t bytes = 0;
     bool owned = false;
 };
+
+// Entry points whose kernels read a matrix as rows * cols contiguous words
+// turn strided views away; LDE inputs and downloads take them.
+static int need_dense(const pg_buf* b, const char* what) {
+    if (b && b->ld != b->cols && b->rows > 1)
+        return fail(PG_E_INVALID, "%s: a strided view is accepted only as an LDE input or for download", what);
+    return PG_OK;
+}
 
 static int is_pow2(size_t x) { return x && !(x & (x - 1)); }
 static int log2_of(size_t x) {
@@ -461,6 +470,7 @@ static int buf_new(pg_ctx* ctx, size_t rows, size_t cols, pg_buf** out) {
     b->ctx = ctx;
     b->rows = rows;
     b->cols = cols;
+    b->ld = cols;
     b->bytes = rows * cols * 4;
     b->owned = true;
     void* p;
@@ -483,6 +493,30 @@ extern "C" int pg_buf_wrap(pg_ctx* ctx, void* dev, size_t rows, size_t cols, pg_
     b->ptr = (uint32_t*)dev;
     b->rows = rows;
     b->cols = cols;
+    b->ld = cols;
+    b->bytes = rows * cols * 4;
+    b->owned = false;
+    *out = b;
+    return PG_OK;
+}
+
+extern "C" int pg_buf_view(pg_ctx* ctx, const pg_buf* parent, size_t row0, size_t row_step, size_t col0,
+                           size_t rows, size_t cols, pg_buf** out) {
+    if (!ctx || !parent || !out) return fail(PG_E_INVALID, "null argument");
+    *out = nullptr;
+    if (row_step == 0 || cols == 0) return fail(PG_E_INVALID, "row step and width must be positive");
+    if (col0 > parent->cols || cols > parent->cols - col0)
+        return fail(PG_E_INVALID, "columns %zu..%zu outside width %zu", col0, col0 + cols, parent->cols);
+    if (rows && (row0 >= parent->rows || (rows - 1) > (parent->rows - 1 - row0) / row_step))
+        return fail(PG_E_INVALID, "rows %zu + %zu * k (k < %zu) outside height %zu", row0, row_step, rows,
+                    parent->rows);
+    if (parent->ld && row_step > SIZE_MAX / parent->ld) return fail(PG_E_INVALID, "row step too large");
+    pg_buf* b = new pg_buf;
+    b->ctx = ctx;
+    b->ptr = parent->ptr + (rows ? row0 * parent->ld + col0 : 0);
+    b->rows = rows;
+    b->cols = cols;
+    b->ld = rows > 1 ? parent->ld * row_step : cols;
     b->bytes = rows * cols * 4;
     b->owned = false;
     *out = b;
@@ -501,18 +535,37 @@ extern "C" void pg_buf_free(pg_buf* b) {
 extern "C" void* pg_buf_device_ptr(const pg_buf* b) { return b ? b->ptr : nullptr; }
 extern "C" size_t pg_buf_rows(const pg_buf* b) { return b ? b->rows : 0; }
 extern "C" size_t pg_buf_cols(const pg_buf* b) { return b ? b->cols : 0; }
+extern "C" size_t pg_buf_ld(const pg_buf* b) { return b ? b->ld : 0; }
 
 static bool field_ok(int f) { return f == PG_KOALABEAR || f == PG_BABYBEAR; }
 
 extern "C" int pg_upload(pg_ctx* ctx, pg_buf* b, const uint32_t* host, int field, int canonical) {
     if (!ctx || !b || (!host && b->bytes)) return fail(PG_E_INVALID, "null argument");
     if (canonical && !field_ok(field)) return fail(PG_E_INVALID, "unknown field %d", field);
+    PG_TRY(need_dense(b, "pg_upload"));
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
     HIP_TRY(hipMemcpyAsync(b->ptr, host, b->bytes, hipMemcpyHostToDevice, l.s));
     if (canonical) HIP_TRY(launch_convert(field, b->ptr, b->rows * b->cols, 1, l.s));
     HIP_TRY(hipStreamSynchronize(l.s));
     return PG_OK;
+}
+
+extern "C" int pg_host_alloc(pg_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(PG_E_INVALID, "null argument");
+    *out = nullptr;
+    PG_TRY(bind_device(ctx));
+    hipError_t e = hipHostMalloc(out, bytes ? bytes : 4, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(PG_E_OOM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return PG_OK;
+}
+
+extern "C" void pg_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
 }
 
 extern "C" int pg_download(pg_ctx* ctx, const pg_buf* b, size_t row0, size_t nrows, uint32_t* host,
@@ -522,11 +575,18 @@ extern "C" int pg_download(pg_ctx* ctx, const pg_buf* b, size_t row0, size_t nro
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
     const size_t n = nrows * b->cols;
+    if (n == 0) return PG_OK;
+    const uint32_t* src = b->ptr + row0 * b->ld;
+    // rows x cols words out of a pitch-ld source into a dense destination
+    auto copy_rows = [&](void* dst, hipMemcpyKind kind) {
+        return b->ld == b->cols ? hipMemcpyAsync(dst, src, n * 4, kind, l.s)
+                                : hipMemcpy2DAsync(dst, b->cols * 4, src, b->ld * 4, b->cols * 4, nrows, kind, l.s);
+    };
     if (canonical) {
         if (!field_ok(field)) return fail(PG_E_INVALID, "unknown field %d", field);
         void* t;
         PG_TRY(ctx_alloc(ctx, n * 4, &t));
-        hipError_t e = hipMemcpyAsync(t, b->ptr + row0 * b->cols, n * 4, hipMemcpyDeviceToDevice, l.s);
+        hipError_t e = copy_rows(t, hipMemcpyDeviceToDevice);
         if (e == hipSuccess) e = launch_convert(field, (uint32_t*)t, n, 0, l.s);
         if (e == hipSuccess) e = hipMemcpyAsync(host, t, n * 4, hipMemcpyDeviceToHost, l.s);
         if (e == hipSuccess) e = hipStreamSynchronize(l.s);
@@ -534,13 +594,14 @@ extern "C" int pg_download(pg_ctx* ctx, const pg_buf* b, size_t row0, size_t nro
         HIP_TRY(e);
         return PG_OK;
     }
-    HIP_TRY(hipMemcpyAsync(host, b->ptr + row0 * b->cols, n * 4, hipMemcpyDeviceToHost, l.s));
+    HIP_TRY(copy_rows(host, hipMemcpyDeviceToHost));
     HIP_TRY(hipStreamSynchronize(l.s));
     return PG_OK;
 }
 
 extern "C" int pg_fill_random(pg_ctx* ctx, pg_buf* b, int field, uint64_t seed) {
     if (!ctx || !b || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(need_dense(b, "pg_fill_random"));
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
     HIP_TRY(launch_fill_random(field, seed, b->ptr, b->rows * b->cols, 1, l.s));
@@ -587,7 +648,7 @@ static int lde_on_stream(pg_ctx* ctx, hipStream_t s, int field, const pg_buf* in
         tmp = (uint32_t*)(*scratch)->ptr;
     }
     auto kt = make_timer(ctx);
-    hipError_t e = launch_coset_lde(field, in->ptr, in->cols, L, (uint32_t)in->cols, (int)b,
+    hipError_t e = launch_coset_lde(field, in->ptr, in->ld, L, (uint32_t)in->cols, (int)b,
                                     (const uint32_t*)tw->ptr, (const uint32_t*)sc->ptr, tmp, o->ptr, in->cols,
                                     bitrev, s, kt.get());
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // tables and scratch stay referenced until done
@@ -769,6 +830,7 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
 extern "C" int pg_mmcs_commit(pg_ctx* ctx, int field, pg_buf* const* mats, size_t n, int take,
                               uint32_t root[8], pg_tree** out) {
     if (!ctx || !mats || !root || !out || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    for (size_t k = 0; k < n; k++) PG_TRY(need_dense(mats[k], "pg_mmcs_commit"));
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
     std::vector<pg_buf*> v(mats, mats + n);
@@ -1313,7 +1375,7 @@ extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* eval
             b->keep.push_back(sc);
             LdeArgs a{};
             a.in = evals[j]->ptr;
-            a.ld_in = evals[j]->cols;
+            a.ld_in = evals[j]->ld;
             a.out = b->trees[j]->mats[0]->ptr;
             a.ld_out = W[j];
             a.tmp = scratch && Lg > 11 ? (uint32_t*)scratch->ptr + tmp_off[j] : nullptr;
@@ -1618,6 +1680,7 @@ static int svc_enqueue(pg_service* sv, std::unique_ptr<SvcJob> j, uint64_t* id_o
 extern "C" int pg_service_submit(pg_service* sv, int field, pg_buf* trace, uint32_t log_blowup, uint64_t* id_out) {
     if (!sv || !trace || !id_out || !field_ok(field) || !is_pow2(trace->rows) || trace->cols == 0)
         return fail(PG_E_INVALID, "bad argument");
+    PG_TRY(need_dense(trace, "pg_service_submit"));
     if (log2_of(trace->rows) + (int)log_blowup > two_adicity(field)) return fail(PG_E_INVALID, "LDE too large");
     std::unique_ptr<SvcJob> j(new SvcJob);
     j->field = field;
@@ -1770,6 +1833,7 @@ extern "C" int pg_challenger_check_witness(pg_challenger* c, int bits, uint32_t 
 extern "C" int pg_fri_fold(pg_ctx* ctx, int field, const pg_buf* v, const uint32_t beta[4], pg_buf** out) {
     if (!ctx || !v || !beta || !out || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
     if (v->cols != 4 || v->rows < 2 || !is_pow2(v->rows)) return fail(PG_E_INVALID, "v must be n x 4, n a power of two >= 2");
+    PG_TRY(need_dense(v, "pg_fri_fold"));
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
     const int ln = log2_of(v->rows);
@@ -1821,6 +1885,7 @@ extern "C" int pg_fri_commit_phase(pg_ctx* ctx, int field, pg_challenger* ch, co
     for (int k = 0; k <= max_log; k++)
         if (inputs[k] && (inputs[k]->rows != ((size_t)1 << k) || inputs[k]->cols != 4))
             return fail(PG_E_INVALID, "input %d must be 2^%d x 4", k, k);
+    for (int k = 0; k <= max_log; k++) PG_TRY(need_dense(inputs[k], "pg_fri_commit_phase"));
     if (log_stop > L0) return fail(PG_E_INVALID, "log_stop > input length");
     PG_TRY(bind_device(ctx));
     Lease l(ctx);

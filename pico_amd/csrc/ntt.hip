@@ -270,7 +270,7 @@ __device__ __forceinline__ void tile_dif3(uint32_t (&x)[1 << LOG_E], uint32_t* l
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
                                           gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
-    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
+    constexpr int E = 1 << LOG_E, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
     constexpr int LOG_G = LOG_E - LOG_R, G = 1 << LOG_G;
     // round A: groups g of R values over n3 (bit-reversed), n1 = rev_E(p / R),
     // n2 = rev_E((p % R) G + g)
@@ -409,7 +409,7 @@ template <class F, int LOG_E, int LOG_P, int KIND>
 __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     __attribute__((amdgpu_waves_per_eu(waves_for(LOG_E, LOG_P, KIND))))
     lde_kernel(LdeArgs a0, const LdeArgs* __restrict__ jobs, const uint64_t* __restrict__ prefix, int njobs) {
-    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, S = E * P;
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
     constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
     constexpr int NT = threads_for(LOG_E, LOG_P, KIND);
     constexpr bool PARK = parks_coefficients(KIND) && E >= 32;

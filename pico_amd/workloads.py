@@ -6,7 +6,9 @@ device through the C ABI and reports wall time from the host clock.
 
   shard_commit_fri   one shard: 14 matrices with log-heights in 14..22 and
                      widths in {8,16,32,64,128,256,300}, one mixed-height
-                     commit at blowup 2; a quotient-domain commit at blowup 8;
+                     commit at blowup 2; a quotient vector (2^21 ext4) split
+                     into 4 chunks on their coset domains (strided views, no
+                     copy) and committed at blowup 8;
                      the open's values (zeta, g zeta) and reduced openings;
                      FRI on a 2^22 ext4 vector down to 2^6 with a commit per
                      round, 16-bit proof of work and 100 query openings
@@ -64,12 +66,12 @@ def shard_shapes(seed_base=3 << 32, n=14, min_log=14, max_log=22):
 
 
 def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_bits=16,
-                     quotient_log_h=19, quotient_width=16):
+                     quotient_log_h=19, quotient_chunks=4):
     """One shard's commits and FRI on the device; returns a timing dict."""
     f = core.KOALABEAR
     shapes = shapes or shard_shapes()
     mats = [ctx.random_matrix(f, (3 << 32) | (100 + i), 1 << L, w) for i, (L, w) in enumerate(shapes)]
-    q = ctx.random_matrix(f, (3 << 32) | 200, 1 << quotient_log_h, quotient_width)
+    q = ctx.random_matrix(f, (3 << 32) | 200, quotient_chunks << quotient_log_h, 4)  # flattened ext4
     v = ctx.random_matrix(f, (3 << 32) | 300, 1 << log_fri, 4)
     core.device_synchronize()
     cells = sum((1 << L) * w for L, w in shapes)
@@ -80,8 +82,13 @@ def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_
     t1 = time.perf_counter()
     qpcs = core.TwoAdicFriPcs(ctx, f, 3)
     p = core.PRIMES[f]
-    qshift = core.GENERATORS[f] * core.two_adic_generator(f, quotient_log_h + 3) % p
-    qroot, quot = qpcs.commit([(core.TwoAdicMultiplicativeCoset(quotient_log_h, qshift), q)])
+    # the quotient domain is the disjoint coset GEN <w>; chunk i lives on GEN w^i <w^d>
+    qdom = core.TwoAdicMultiplicativeCoset(quotient_log_h + quotient_chunks.bit_length() - 1,
+                                           core.GENERATORS[f], f)
+    chunks = qdom.split_evals(quotient_chunks, q)
+    qroot, quot = qpcs.commit(list(zip(qdom.split_domains(quotient_chunks), chunks)))
+    for c in chunks:
+        c.free()
     t2 = time.perf_counter()
     ch = core.DuplexChallenger(f)
     ch.observe(root)
@@ -93,9 +100,9 @@ def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_
     pts2, pts1 = np.stack([zeta, gz]), zeta[None, :]
     t_o0 = time.perf_counter()
     main_vals = [pcs.open_values(main, k, pts2) for k in range(len(shapes))]
-    quot_vals = [qpcs.open_values(quot, 0, pts1)]
+    quot_vals = [qpcs.open_values(quot, k, pts1) for k in range(quotient_chunks)]
     t_o1 = time.perf_counter()
-    ro = pcs.reduce_openings([(main, [pts2] * len(shapes)), (quot, [pts1])], [main_vals, quot_vals],
+    ro = pcs.reduce_openings([(main, [pts2] * len(shapes)), (quot, [pts1] * quotient_chunks)], [main_vals, quot_vals],
                              ch.sample_ext())
     t_o2 = time.perf_counter()
     for buf in ro.values():
@@ -122,6 +129,7 @@ def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_
     })
     main.free()
     quot.free()
+    q.free()
     fri.free()
     return t
 

@@ -1,0 +1,145 @@
+"""Trace ingest on the device: strided views (pg_buf_view) and the quotient
+chunk split that Pcs::commit receives in the reference
+(vm/src/machine/prover.rs:513-523: `RowMajorMatrix::new_col(values)
+.flatten_to_base()`, `domain.split_evals(d, ..)`, `domain.split_domains(d)`).
+
+The split is checked two ways: the commitment of the d views equals the
+oracle's commitment of the host-split chunks, and the chunk polynomials
+opened at an out-of-domain point recombine to the original quotient there
+with the verifier's selector formula (each zp_i is 1 on chunk domain i and 0
+on the others), which fixes the chunk order and the chunk shifts without the
+oracle."""
+import numpy as np
+import pytest
+
+import pico_amd
+from pico_amd import BABYBEAR, KOALABEAR
+from pico_amd._lib import PgError, lib
+from tests import pyref
+
+pytestmark = pytest.mark.gpu
+FIELDS = [KOALABEAR, BABYBEAR]
+
+
+def test_views_download_and_bounds(ctx, oracle):
+    f = KOALABEAR
+    host = oracle.random_matrix(f, 31, 64, 12)
+    m = ctx.upload(host, field=f)
+    v = m.view(row0=3, row_step=5, col0=2, cols=7)
+    assert (v.rows, v.cols, v.ld) == (13, 7, 60)
+    assert np.array_equal(v.to_host(f), host[3::5, 2:9])
+    assert np.array_equal(v.to_host(f, row0=4, nrows=6), host[3::5, 2:9][4:10])
+    raw = m.to_host()
+    assert np.array_equal(v.to_host(), raw[3::5, 2:9])
+    vv = v.view(row0=1, row_step=2, col0=3, cols=2)  # a view of a view
+    assert np.array_equal(vv.to_host(f), host[3::5, 2:9][1::2, 3:5])
+    one = m.view(row0=63, row_step=7, rows=1)
+    assert np.array_equal(one.to_host(f), host[63:64])
+    empty = m.view(row0=0, rows=0, cols=3)
+    assert empty.to_host(f).shape == (0, 3)
+    for bad in [dict(row0=64, rows=1), dict(row0=3, row_step=5, rows=14), dict(col0=10, cols=3),
+                dict(row_step=0, rows=2), dict(cols=0)]:
+        with pytest.raises(PgError):
+            m.view(**bad)
+    # kernels that read rows * cols contiguous words turn strided views away
+    with pytest.raises(PgError):
+        pico_amd.MerkleTreeMmcs(ctx, f).commit([v])
+    assert lib().pg_fill_random(ctx.handle, v.handle, f, 5) != 0
+    assert lib().pg_upload(ctx.handle, v.handle, pico_amd.core._ptr(np.zeros((13, 7), np.uint32)), -1, 0) != 0
+    # a view with unit step over whole rows is dense and allowed everywhere
+    d = m.view(row0=8, rows=16)
+    assert d.ld == d.cols == 12
+    assert np.array_equal(d.to_host(f), host[8:24])
+    for x in (vv, v, one, empty, d):
+        x.free()
+    m.free()
+
+
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("log_n,d,lb", [(9, 2, 1), (8, 4, 3), (3, 8, 1), (12, 2, 2)])
+def test_quotient_split_commit_matches_oracle(ctx, oracle, f, log_n, d, lb):
+    p = pico_amd.PRIMES[f]
+    g = pico_amd.GENERATORS[f]
+    nd = (1 << log_n) * d
+    flat = oracle.random_matrix(f, 4000 + log_n * 10 + d, nd, 4)  # (n d) x 4, ext4 rows flattened
+    dev = ctx.upload(flat, field=f)
+    qdom = pico_amd.TwoAdicMultiplicativeCoset(nd.bit_length() - 1, g, f)
+    doms = qdom.split_domains(d)
+    chunks = qdom.split_evals(d, dev)
+    assert all(c.rows == 1 << log_n and c.ld == 4 * d for c in chunks)
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, lb)
+    root, pd = pcs.commit(list(zip(doms, chunks)))
+    w = pico_amd.two_adic_generator(f, nd.bit_length() - 1)
+    for i, dm in enumerate(doms):
+        assert dm.log_n == log_n and dm.shift == g * pow(w, i, p) % p
+    ldes = [oracle.coset_lde(f, flat[i::d], lb, g * pow(dm.shift, p - 2, p) % p) for i, dm in enumerate(doms)]
+    assert np.array_equal(root, oracle.MerkleTree(f, ldes).root)
+    # same root from chunks split on the host and uploaded one by one
+    dense = [ctx.upload(np.ascontiguousarray(flat[i::d]), field=f) for i in range(d)]
+    root2, pd2 = pcs.commit(list(zip(doms, dense)))
+    assert np.array_equal(root, root2)
+    for k in range(d):
+        assert np.array_equal(pcs.get_evaluations_on_domain(pd, k, doms[k]), pcs.get_evaluations_on_domain(pd2, k, doms[k]))
+    pd.free()
+    pd2.free()
+    dev.free()
+
+
+def _ext_pow(f, a, e):
+    r = [1, 0, 0, 0]
+    while e:
+        if e & 1:
+            r = pyref.ext_mul(f, r, a)
+        a = pyref.ext_mul(f, a, a)
+        e >>= 1
+    return r
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_quotient_chunks_recombine_at_zeta(ctx, f):
+    """q(X) of degree < n d on D = g<w_{nd}>; chunk i interpolates q on
+    D_i = g w^i <w_n>.  sum_i zp_i(zeta) q_i(zeta) = q(zeta) with
+    zp_i = prod_{j != i} Z_{D_j}(zeta) / Z_{D_j}(g w^i), Z_{D_j}(X) = X^n - s_j^n."""
+    p = pico_amd.PRIMES[f]
+    g = pico_amd.GENERATORS[f]
+    log_n, d = 7, 4
+    n, nd = 1 << log_n, (1 << log_n) * d
+    rng = np.random.default_rng(21 + f)
+    coeffs = rng.integers(0, p, (nd, 4)).astype(np.uint64)  # 4 base polynomials of degree < n d
+    w = pico_amd.two_adic_generator(f, nd.bit_length() - 1)
+    xs = np.array([g * pow(w, j, p) % p for j in range(nd)], np.uint64)
+    vals = np.zeros((nd, 4), np.uint64)
+    pw = np.ones(nd, np.uint64)
+    for k in range(nd):
+        vals = (vals + coeffs[k][None, :] * pw[:, None]) % p
+        pw = pw * xs % p
+    dev = ctx.upload(vals.astype(np.uint32), field=f)
+    qdom = pico_amd.TwoAdicMultiplicativeCoset(nd.bit_length() - 1, g, f)
+    doms = qdom.split_domains(d)
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    _, pd = pcs.commit(list(zip(doms, qdom.split_evals(d, dev))))
+    zeta = [int(t) for t in rng.integers(0, p, 4)]
+    opened = [pcs.open_values(pd, i, np.array([zeta], np.uint32))[0] for i in range(d)]  # d x 4 cols x ext4
+    zn = _ext_pow(f, zeta, n)
+    sn = [pow(dm.shift, n, p) for dm in doms]
+    want = []
+    for c in range(4):
+        acc = [0, 0, 0, 0]
+        for k in range(nd - 1, -1, -1):  # Horner: q_c(zeta)
+            acc = pyref.ext_mul(f, acc, zeta)
+            acc[0] = (acc[0] + int(coeffs[k, c])) % p
+        want.append(acc)
+    for c in range(4):
+        got = [0, 0, 0, 0]
+        for i in range(d):
+            zp = [1, 0, 0, 0]
+            for j in range(d):
+                if j != i:
+                    num = [(zn[0] - sn[j]) % p] + zn[1:]
+                    den = pow((sn[i] - sn[j]) % p, p - 2, p)
+                    zp = pyref.ext_mul(f, zp, [t * den % p for t in num])
+            term = pyref.ext_mul(f, zp, [int(t) for t in opened[i][c]])
+            got = [(a + b) % p for a, b in zip(got, term)]
+        assert got == want[c], c
+    pd.free()
+    dev.free()

@@ -69,3 +69,26 @@ def test_pool_small_budget_context(oracle):
         assert np.array_equal(root, oracle.MerkleTree(f, [lde]).root)
         with pytest.raises(pico_amd.AdmissionError):
             ctx.matrix(1 << 20, 64)  # 256 MiB > budget
+
+
+def test_pinned_host_upload(oracle):
+    """A trace written into page-locked memory from pg_host_alloc uploads and
+    commits to the same root as the same trace from pageable memory."""
+    f = KOALABEAR
+    with pico_amd.Context(0) as ctx:
+        host = oracle.random_matrix(f, 77, 1 << 12, 37)
+        pinned = ctx.host_matrix(*host.shape)
+        pinned.array[:] = host
+        pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+        dom = pcs.natural_domain_for_degree(1 << 12)
+        r_pin, pd1 = pcs.commit([(dom, ctx.upload(pinned.array, field=f))])
+        r_page, pd2 = pcs.commit([(dom, ctx.upload(host, field=f))])
+        lde = oracle.coset_lde(f, host, 1, pico_amd.GENERATORS[f])
+        assert np.array_equal(r_pin, r_page)
+        assert np.array_equal(r_pin, oracle.MerkleTree(f, [lde]).root)
+        pd1.free()
+        pd2.free()
+        pinned.free()
+        empty = ctx.host_matrix(0, 5)
+        assert empty.array.shape == (0, 5)
+        empty.free()
