@@ -106,6 +106,11 @@ def cpu_baseline(log_h, w, lb, budget_cols, leaf_rows):
     }
 
 
+def progress(msg):
+    """A line on stderr per finished section (stdout carries only the JSON)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +169,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     cells = h * w
+    progress(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step")
 
     # per-kernel durations: HIP events on the launch stream, separate pass
     ctx.profile(True)
@@ -272,7 +278,21 @@ def main():
     out["end_to_end_pinned"] = {"ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "h2d_ms": h2d_s * 1e3,
                                 "h2d_GB_per_s": host_trace.nbytes / h2d_s / 1e9, "host_memory": "page-locked",
                                 "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
+    # pg_pcs_commit_host from page-locked memory: column blocks uploaded,
+    # extended and absorbed behind one another
+    e2e = []
+    for _ in range(3):
+        sync()
+        t0 = time.perf_counter()
+        root_e2e, pd = pcs.commit_host([(dom, pinned.array)], canonical=False)
+        e2e.append(time.perf_counter() - t0)
+        pd.free()
+    tot_s = min(e2e)
+    out["end_to_end_pinned_overlapped"] = {
+        "ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "host_memory": "page-locked",
+        "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
     pinned.free()
+    progress("end-to-end legs done")
     del host_trace
     # Pcs::open's device part on the same commitment: values at zeta and
     # g zeta, then the reduced opening over all 2^21 x 256 LDE cells.
@@ -305,11 +325,16 @@ def main():
         from pico_amd import workloads
         workloads.shard_commit_fri(ctx)  # warm tables
         out["shard_commit_fri"] = workloads.shard_commit_fri(ctx)
+        progress("shard commit + FRI done")
+        out["shard_ingest"] = workloads.shard_ingest(ctx)
+        progress("shard ingest done")
         out["batched_jobs"] = workloads.batched_jobs(ctx)
+        progress("batched jobs done")
         jobs = workloads.served_jobs(n=args.served_jobs)
         served = {"jobs": [list(j) for j in jobs], "sequential": workloads.served_sequential(ctx, jobs)}
         served["service"] = [workloads.served_stream(ctx, float(r), jobs) for r in args.rates.split(",")]
         out["served_stream"] = served
+        progress("served stream done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.log_h, w, lb, args.cpu_cols, args.cpu_leaf_rows)
     ctx.close()

@@ -168,6 +168,17 @@ void pg_tree_free(pg_tree *tree);
 int pg_pcs_commit(pg_ctx *ctx, int field, const pg_buf *const *evals,
                   const uint32_t *domain_shifts_mont, size_t n, uint32_t log_blowup,
                   uint32_t root[8], pg_tree **out);
+/* Pcs::commit as the reference calls it, on matrices in host memory
+ * (vm/src/machine/prover.rs:244 commits the traces `generate_main` built on
+ * the host, :169-223, sorted by (height desc, name) at :192).  Matrix k is
+ * rows[k] x cols[k] words at host[k] (Montgomery form, or canonical values
+ * when canonical != 0).  Same result as pg_upload of each matrix followed by
+ * pg_pcs_commit; the uploads run on their own stream, tallest height group
+ * first, overlapping the LDE and hashing of the groups before them.  Host
+ * memory from pg_host_alloc uploads at the link's rate. */
+int pg_pcs_commit_host(pg_ctx *ctx, int field, const uint32_t *const *host, const size_t *rows,
+                       const size_t *cols, const uint32_t *domain_shifts_mont, size_t n,
+                       uint32_t log_blowup, int canonical, uint32_t root[8], pg_tree **out);
 /* get_evaluations_on_domain: the first `rows` committed rows of matrix k,
  * returned in natural order (rows * width words). */
 int pg_pcs_evaluations(pg_ctx *ctx, const pg_tree *tree, size_t k, size_t rows, uint32_t *host);

@@ -80,6 +80,7 @@ SIGNATURES = {
     "pg_tree_digest_layer": (_i, [_P, _P, _i, _P]),
     "pg_tree_free": (None, [_P]),
     "pg_pcs_commit": (_i, [_P, _i, _P, _P, _sz, _u32, _P, _pp]),
+    "pg_pcs_commit_host": (_i, [_P, _i, _P, _P, _P, _P, _sz, _u32, _i, _P, _pp]),
     "pg_pcs_evaluations": (_i, [_P, _P, _sz, _sz, _P]),
     "pg_pcs_open_values": (_i, [_P, _P, _sz, _u32, _P, _sz, _P]),
     "pg_pcs_reduce_openings": (_i, [_P, _P, _sz, _P, _P, _P, _P, _P]),

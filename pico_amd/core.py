@@ -370,6 +370,24 @@ class TwoAdicFriPcs:
                                   _ptr(root), ctypes.byref(h)))
         return from_monty(self.field, root), ProverData(self.ctx, self.field, h)
 
+    def commit_host(self, evaluations, canonical=True):
+        """evaluations: [(domain, 2-D uint32 host array)] -> (root, ProverData).
+        The arrays stay on the host until the library uploads them, a height
+        group at a time, overlapped with the commit of the taller groups.
+        canonical=False means the arrays already hold Montgomery words."""
+        n = len(evaluations)
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for _, a in evaluations]
+        arrs = [a.reshape(-1, 1) if a.ndim == 1 else a for a in arrs]
+        ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in arrs])
+        rows = np.array([a.shape[0] for a in arrs], np.uint64)
+        cols = np.array([a.shape[1] for a in arrs], np.uint64)
+        shifts = np.ascontiguousarray(to_monty(self.field, [d.shift for d, _ in evaluations]))
+        root = np.zeros(8, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().pg_pcs_commit_host(self.ctx.handle, self.field, ptrs, _ptr(rows), _ptr(cols), _ptr(shifts), n,
+                                       self.log_blowup, int(bool(canonical)), _ptr(root), ctypes.byref(h)))
+        return from_monty(self.field, root), ProverData(self.ctx, self.field, h)
+
     def get_evaluations_on_domain(self, data, idx, domain):
         m = lib().pg_tree_matrix(data.handle, idx)
         cols = lib().pg_buf_cols(m)

@@ -615,18 +615,26 @@ static uint32_t gen_monty(int field) {
 }
 static int two_adicity(int field) { return field == PG_KOALABEAR ? KoalaBear::TWO_ADICITY : BabyBear::TWO_ADICITY; }
 
+// Shape check of an LDE input; *L_out = log2(rows).
+static int lde_check(int field, size_t rows, size_t cols, uint32_t b, int* L_out) {
+    if (!is_pow2(rows)) return fail(PG_E_INVALID, "height %zu is not a power of two", rows);
+    if (cols == 0) return fail(PG_E_INVALID, "width is zero");
+    const int L = log2_of(rows);
+    if (L > lde_max_log_h()) return fail(PG_E_UNSUPPORTED, "log height %d > %d", L, lde_max_log_h());
+    if (L + (int)b > two_adicity(field))
+        return fail(PG_E_INVALID, "LDE size 2^%d exceeds the field's two-adicity %d", L + (int)b, two_adicity(field));
+    if (b > 8) return fail(PG_E_UNSUPPORTED, "added_bits %u > 8", b);
+    *L_out = L;
+    return PG_OK;
+}
+
 // LDE of `in` into a new N x w buffer on stream s; `scratch` (may be null)
 // is reused when large enough.
 static int lde_on_stream(pg_ctx* ctx, hipStream_t s, int field, const pg_buf* in, uint32_t b,
                          uint32_t shift, int bitrev, pg_buf** out, std::shared_ptr<DevMem>* scratch) {
     *out = nullptr;
-    if (!is_pow2(in->rows)) return fail(PG_E_INVALID, "height %zu is not a power of two", in->rows);
-    if (in->cols == 0) return fail(PG_E_INVALID, "width is zero");
-    const int L = log2_of(in->rows);
-    if (L > lde_max_log_h()) return fail(PG_E_UNSUPPORTED, "log height %d > %d", L, lde_max_log_h());
-    if (L + (int)b > two_adicity(field))
-        return fail(PG_E_INVALID, "LDE size 2^%d exceeds the field's two-adicity %d", L + (int)b, two_adicity(field));
-    if (b > 8) return fail(PG_E_UNSUPPORTED, "added_bits %u > 8", b);
+    int L;
+    PG_TRY(lde_check(field, in->rows, in->cols, b, &L));
     std::shared_ptr<DevMem> tw, sc;
     PG_TRY(get_twiddles(ctx, s, field, L, &tw));
     PG_TRY(get_scale(ctx, s, field, L, (int)b, shift, &sc));
@@ -785,8 +793,19 @@ static int tree_prepare(pg_ctx* ctx, hipStream_t s, int field, const std::vector
     return PG_OK;
 }
 
+// `stage`, when given, is called with each run [g0, g1) of equal-height
+// matrices in height-sorted order right before the first kernel that reads
+// them is enqueued on s: it may enqueue their production on s (or make s
+// wait for it), so a commit can start hashing tall matrices while shorter
+// ones are still arriving.  For the tallest run it also gets the leaf
+// digest layer; returning STAGE_HASHED says it has queued those digests
+// itself (and the leaf hash is skipped).
+static constexpr int STAGE_HASHED = 1;
+using StageFn = std::function<int(const std::vector<size_t>& order, size_t g0, size_t g1, uint32_t* leaves)>;
+
 static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const std::vector<pg_buf*>& mats,
-                                 const std::vector<bool>& owned, uint32_t root[8], pg_tree** out) {
+                                 const std::vector<bool>& owned, uint32_t root[8], pg_tree** out,
+                                 const StageFn& stage = nullptr) {
     *out = nullptr;
     const size_t n = mats.size();
     TreePtr t(nullptr, tree_discard);
@@ -800,7 +819,14 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
     while (g1 < n && mats[order[g1]]->rows == H) wsum += (uint32_t)mats[order[g1++]]->cols;
     const pg_buf* first = mats[order[0]];
     auto kt = make_timer(ctx);
-    if (g1 - g0 == 1 && first->cols % 8 == 0 && first->cols > 0 && ((uintptr_t)first->ptr % 16) == 0)
+    bool hashed = false;
+    if (stage) {
+        const int r = stage(order, g0, g1, t->digests);
+        if (r < 0) return r;
+        hashed = r == STAGE_HASHED;
+    }
+    if (hashed) {
+    } else if (g1 - g0 == 1 && first->cols % 8 == 0 && first->cols > 0 && ((uintptr_t)first->ptr % 16) == 0)
         TIMED(kt, "leaf_hash_w8", s,
               launch_leaf_hash_w8(field, first->ptr, first->cols, (uint32_t)first->cols, H, t->digests, s));
     else
@@ -812,6 +838,7 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
         size_t gj = gi;
         uint32_t iw = 0;
         while (gj < n && mats[order[gj]]->rows == nl) iw += (uint32_t)mats[order[gj++]]->cols;
+        if (stage && gj > gi) PG_TRY(stage(order, gi, gj, nullptr));
         TIMED(kt, gj > gi ? "compress_inject" : "compress", s,
               launch_compress(field, t->digests + 8 * off, t->digests + 8 * (off + len), nl,
                               gj > gi ? d_sorted + gi : nullptr, (int)(gj - gi), iw, s));
@@ -1001,6 +1028,148 @@ extern "C" int pg_pcs_commit(pg_ctx* ctx, int field, const pg_buf* const* evals,
     scratch.reset();
     int rc = mmcs_commit_on_stream(ctx, l.s, field, ldes, std::vector<bool>(n, true), root, out);
     if (rc != PG_OK) cleanup();
+    return rc;
+}
+
+// Pcs::commit of matrices in host memory.  Uploads run on a second stream,
+// one height group at a time in the order the tree consumes them (tallest
+// first); the compute stream waits for a group's upload event, runs its
+// LDEs, then hashes or injects it, so the upload of shorter matrices overlaps
+// the LDE and hashing of taller ones.  With pageable memory the host thread
+// blocks inside each upload while the device works on the previous group.
+extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const* host, const size_t* rows,
+                                  const size_t* cols, const uint32_t* dshift, size_t n, uint32_t log_blowup,
+                                  int canonical, uint32_t root[8], pg_tree** out) {
+    if (!ctx || !host || !rows || !cols || !root || !out || !field_ok(field) || n == 0)
+        return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    std::vector<int> L(n);
+    std::vector<uint32_t> shift(n, gen_monty(field));
+    for (size_t k = 0; k < n; k++) {
+        if (!host[k]) return fail(PG_E_INVALID, "null matrix %zu", k);
+        PG_TRY(lde_check(field, rows[k], cols[k], log_blowup, &L[k]));
+        if (rows[k] && cols[k] > (SIZE_MAX / 4) / (rows[k] << log_blowup)) return fail(PG_E_INVALID, "matrix too large");
+        if (dshift) {
+            if (dshift[k] == 0) return fail(PG_E_INVALID, "domain shift is zero");
+            shift[k] = field == PG_KOALABEAR ? fmul<KoalaBear>(shift[k], finv<KoalaBear>(dshift[k]))
+                                             : fmul<BabyBear>(shift[k], finv<BabyBear>(dshift[k]));
+        }
+    }
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx), lc(ctx);
+    std::vector<pg_buf*> in(n, nullptr), ldes(n, nullptr);
+    std::vector<std::shared_ptr<DevMem>> tw(n), sc(n);
+    std::shared_ptr<DevMem> scratch;
+    std::vector<hipEvent_t> events;
+    bool committed = false;
+    auto cleanup = [&]() {  // after an error some work may still be queued
+        (void)hipStreamSynchronize(lc.s);
+        (void)hipStreamSynchronize(l.s);
+        for (auto* b : in) pg_buf_free(b);
+        if (!committed)
+            for (auto* b : ldes) pg_buf_free(b);
+        for (auto e : events) (void)hipEventDestroy(e);
+    };
+    int rc = PG_OK;
+    size_t tmp_bytes = 0;
+    for (size_t k = 0; k < n && rc == PG_OK; k++) {
+        rc = buf_new(ctx, rows[k], cols[k], &in[k]);
+        if (rc == PG_OK) rc = buf_new(ctx, rows[k] << log_blowup, cols[k], &ldes[k]);
+        // tables are built (and their stream synchronised) before anything is queued
+        if (rc == PG_OK) rc = get_twiddles(ctx, l.s, field, L[k], &tw[k]);
+        if (rc == PG_OK) rc = get_scale(ctx, l.s, field, L[k], (int)log_blowup, shift[k], &sc[k]);
+        if (L[k] > 11) tmp_bytes = std::max(tmp_bytes, ldes[k] ? ldes[k]->bytes : 0);
+    }
+    if (rc == PG_OK && tmp_bytes) {
+        scratch = make_devmem(ctx, tmp_bytes, &rc);
+    }
+    if (rc != PG_OK) {
+        cleanup();
+        return rc;
+    }
+    auto kt = make_timer(ctx);
+    // The tallest matrices are cut into column blocks (page-locked Montgomery
+    // input only: the canonical conversion works on whole matrices, and a
+    // pitched copy out of pageable memory is staged row by row, far below
+    // the link's rate): each block is uploaded as a pitched copy, extended
+    // into its columns of the LDE and absorbed into every row's sponge, so
+    // the leaf hash runs behind the upload instead of after it.  512-byte
+    // block rows keep pitched copies at the link's rate.
+    constexpr size_t CB = 128, SPLIT_BYTES = 64ull << 20;
+    size_t Hmax = 0, segments = 0;
+    for (size_t k = 0; k < n; k++) Hmax = std::max(Hmax, rows[k]);
+    std::vector<bool> locked(n, false);
+    for (size_t k = 0; k < n; k++) {
+        hipPointerAttribute_t a;
+        if (hipPointerGetAttributes(&a, host[k]) == hipSuccess)
+            locked[k] = a.type == hipMemoryTypeHost;
+        else
+            (void)hipGetLastError();  // pageable memory is unknown to the runtime
+    }
+    auto block_cols = [&](size_t k) {
+        return !canonical && locked[k] && rows[k] * cols[k] * 4 >= SPLIT_BYTES ? std::min(CB, cols[k]) : cols[k];
+    };
+    for (size_t k = 0; k < n; k++)
+        if (rows[k] == Hmax) segments += (cols[k] + block_cols(k) - 1) / block_cols(k);
+    std::shared_ptr<DevMem> state;
+    if (segments > 1) {
+        state = make_devmem(ctx, (Hmax << log_blowup) * 64, &rc);
+        if (!state) {
+            cleanup();
+            return rc;
+        }
+    }
+    auto upload_wait = [&](size_t k, size_t c0, size_t cb) -> int {
+        if (cb == cols[k])
+            HIP_TRY(hipMemcpyAsync(in[k]->ptr, host[k], in[k]->bytes, hipMemcpyHostToDevice, lc.s));
+        else
+            HIP_TRY(hipMemcpy2DAsync(in[k]->ptr + c0, cols[k] * 4, host[k] + c0, cols[k] * 4, cb * 4, rows[k],
+                                     hipMemcpyHostToDevice, lc.s));
+        if (canonical) HIP_TRY(launch_convert(field, in[k]->ptr, rows[k] * cols[k], 1, lc.s));
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        events.push_back(e);
+        HIP_TRY(hipEventRecord(e, lc.s));
+        HIP_TRY(hipStreamWaitEvent(l.s, e, 0));
+        return PG_OK;
+    };
+    auto lde_cols = [&](size_t k, size_t c0, size_t cb) -> hipError_t {
+        return launch_coset_lde(field, in[k]->ptr + c0, cols[k], L[k], (uint32_t)cb, (int)log_blowup,
+                                (const uint32_t*)tw[k]->ptr, (const uint32_t*)sc[k]->ptr,
+                                L[k] > 11 ? (uint32_t*)scratch->ptr : nullptr, ldes[k]->ptr + c0, cols[k], 1, l.s,
+                                kt.get());
+    };
+    StageFn stage = [&](const std::vector<size_t>& order, size_t g0, size_t g1, uint32_t* leaves) -> int {
+        if (leaves && segments > 1) {
+            uint64_t wsum = 0, done = 0;
+            for (size_t j = g0; j < g1; j++) wsum += cols[order[j]];
+            const size_t N = Hmax << log_blowup;
+            for (size_t j = g0; j < g1; j++) {
+                const size_t k = order[j], bc = block_cols(k);
+                for (size_t c0 = 0; c0 < cols[k]; c0 += bc) {
+                    const size_t cb = std::min(bc, cols[k] - c0);
+                    PG_TRY(upload_wait(k, c0, cb));
+                    HIP_TRY(lde_cols(k, c0, cb));
+                    TIMED(kt, "absorb_segment", l.s,
+                          launch_absorb_segment(field, ldes[k]->ptr + c0, cols[k], (uint32_t)cb, (uint32_t)(done % 8),
+                                                done == 0, done + cb == wsum, N, (uint32_t*)state->ptr, leaves, l.s));
+                    done += cb;
+                }
+            }
+            return STAGE_HASHED;
+        }
+        // one event per matrix: its LDE starts as soon as its own upload is
+        // done, overlapping the uploads of the rest of its group
+        for (size_t j = g0; j < g1; j++) {
+            const size_t k = order[j];
+            PG_TRY(upload_wait(k, 0, cols[k]));
+            HIP_TRY(lde_cols(k, 0, cols[k]));
+        }
+        return PG_OK;
+    };
+    rc = mmcs_commit_on_stream(ctx, l.s, field, ldes, std::vector<bool>(n, true), root, out, stage);
+    committed = rc == PG_OK;  // the tree owns the LDEs from here
+    cleanup();
     return rc;
 }
 

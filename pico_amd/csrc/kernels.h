@@ -40,6 +40,11 @@ struct MatRef {
 // ---- Merkle (merkle.hip) -------------------------------------------------
 // digests[i] = sponge(concat_k row i of mats[k]) for i < rows.  `mats` is a
 // device array of n MatRef; total_width = sum of widths.
+// One segment of every row's leaf sponge (state kept in `state`, 16 words
+// per row, between segments); `last` writes the digests.
+hipError_t launch_absorb_segment(int field, const uint32_t* data, uint64_t stride, uint32_t width, uint32_t pos0,
+                                 int first, int last, uint64_t rows, uint32_t* state, uint32_t* digests,
+                                 hipStream_t stream);
 hipError_t launch_leaf_hash(int field, const MatRef* mats, int n, uint32_t total_width,
                             size_t rows, uint32_t* digests, hipStream_t stream);
 // next[i] = C(prev[2i], prev[2i+1]), then, if n_inj > 0,

@@ -83,6 +83,72 @@ __global__ void __launch_bounds__(256) leaf_hash_w8_kernel(const uint32_t* data,
     o[1] = make_uint4(st[4], st[5], st[6], st[7]);
 }
 
+// One segment of the leaf sponge: columns [0, width) of a matrix view are
+// the next `width` inputs of every row's sponge, whose state lives in
+// `state` (16 words per row) between segments.  pos0 = inputs already
+// written into the pending chunk (the running input count mod 8): in
+// overwrite mode a chunk's inputs go to st[0..8) and the permutation runs
+// when it is full, so st itself carries a partial chunk across segments.
+// first: start from the zero state; last: run the final partial chunk's
+// permutation (if any) and write the digest instead of the state.  The
+// segments of a row, concatenated, give the same digest as one pass of
+// absorb_rows over the whole row.
+template <class F, bool VEC>
+__global__ void __launch_bounds__(256) absorb_segment_kernel(const uint32_t* data, uint64_t stride,
+                                                             uint32_t width, uint32_t pos0, int first,
+                                                             int last, uint64_t rows, uint32_t* state,
+                                                             uint32_t* digests) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    uint32_t st[16];
+    if (first) {
+#pragma unroll
+        for (int t = 0; t < 16; t++) st[t] = 0;
+    } else {
+        const GLOBAL u32x4* sp = reinterpret_cast<const GLOBAL u32x4*>((gcptr)state + 16 * i);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            u32x4 v = sp[q];
+            st[4 * q] = v.x; st[4 * q + 1] = v.y; st[4 * q + 2] = v.z; st[4 * q + 3] = v.w;
+        }
+    }
+    gcptr row = (gcptr)data + i * stride;
+    uint32_t pos = pos0;
+    if constexpr (VEC) {  // pos0 == 0, width % 8 == 0, 16-byte aligned rows
+        const GLOBAL u32x4* rp = reinterpret_cast<const GLOBAL u32x4*>(row);
+        for (uint32_t c = 0; c < width / 8; c++) {
+            u32x4 a = rp[2 * c], b = rp[2 * c + 1];
+            st[0] = a.x; st[1] = a.y; st[2] = a.z; st[3] = a.w;
+            st[4] = b.x; st[5] = b.y; st[6] = b.z; st[7] = b.w;
+            permute<F>(st);
+        }
+    } else {
+        uint32_t e = 0;
+        while (e < width) {  // pos and e are the same in every lane
+            const uint32_t take = width - e < 8 - pos ? width - e : 8 - pos;
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if ((uint32_t)t >= pos && (uint32_t)t < pos + take) st[t] = row[e + (uint32_t)t - pos];
+            e += take;
+            pos += take;
+            if (pos == 8) {
+                permute<F>(st);
+                pos = 0;
+            }
+        }
+    }
+    if (last) {
+        if (pos > 0) permute<F>(st);
+        GLOBAL u32x4* o = reinterpret_cast<GLOBAL u32x4*>((gptr)digests + 8 * i);
+        o[0] = u32x4{st[0], st[1], st[2], st[3]};
+        o[1] = u32x4{st[4], st[5], st[6], st[7]};
+    } else {
+        GLOBAL u32x4* o = reinterpret_cast<GLOBAL u32x4*>((gptr)state + 16 * i);
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q] = u32x4{st[4 * q], st[4 * q + 1], st[4 * q + 2], st[4 * q + 3]};
+    }
+}
+
 template <class F>
 __global__ void __launch_bounds__(256) compress_kernel(const uint32_t* prev, uint32_t* next,
                                                        uint64_t n_out, const MatRef* inj,
@@ -251,6 +317,27 @@ hipError_t launch_leaf_hash(int field, const MatRef* mats, int n, uint32_t total
 // Fast path entry used by the host when a height group is one matrix with
 // width % 8 == 0 and 32-byte aligned rows (declared here, not in the shared
 // header, because only api.hip's commit planner chooses it).
+hipError_t launch_absorb_segment(int field, const uint32_t* data, uint64_t stride, uint32_t width, uint32_t pos0,
+                                 int first, int last, uint64_t rows, uint32_t* state, uint32_t* digests,
+                                 hipStream_t stream) {
+    if (rows == 0) return hipSuccess;
+    if (pos0 > 7 || (!last && !state) || (last && !digests) || (!first && !state)) return hipErrorInvalidValue;
+    const bool vec = pos0 == 0 && width % 8 == 0 && stride % 4 == 0 && ((uintptr_t)data % 16) == 0;
+    const unsigned g = grid_for(rows, 256);
+    if (field == FIELD_KOALABEAR) {
+        if (vec)
+            absorb_segment_kernel<KoalaBear, true><<<g, 256, 0, stream>>>(data, stride, width, 0, first, last, rows, state, digests);
+        else
+            absorb_segment_kernel<KoalaBear, false><<<g, 256, 0, stream>>>(data, stride, width, pos0, first, last, rows, state, digests);
+    } else {
+        if (vec)
+            absorb_segment_kernel<BabyBear, true><<<g, 256, 0, stream>>>(data, stride, width, 0, first, last, rows, state, digests);
+        else
+            absorb_segment_kernel<BabyBear, false><<<g, 256, 0, stream>>>(data, stride, width, pos0, first, last, rows, state, digests);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_leaf_hash_w8(int field, const uint32_t* data, uint64_t stride, uint32_t width,
                                size_t rows, uint32_t* digests, hipStream_t stream) {
     if (rows == 0) return hipSuccess;

@@ -12,6 +12,8 @@ device through the C ABI and reports wall time from the host clock.
                      the open's values (zeta, g zeta) and reduced openings;
                      FRI on a 2^22 ext4 vector down to 2^6 with a commit per
                      round, 16-bit proof of work and 100 query openings
+  shard_ingest       the same 14 traces in host memory (page-locked and
+                     pageable): upload then commit vs pg_pcs_commit_host
   batched_jobs       64 single-matrix commits, log-heights 10..18, widths
                      8..512, blowup 2/4/8: one launch sequence (hipGraph) vs
                      the same jobs committed one by one
@@ -132,6 +134,56 @@ def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_
     q.free()
     fri.free()
     return t
+
+
+def shard_ingest(ctx, shapes=None, reps=2):
+    """The shard's main commit from traces in host memory: upload every
+    matrix then commit, against pg_pcs_commit_host (uploads overlapped with
+    the commit of the taller height groups), from page-locked and from
+    pageable memory.  Host clock, best of `reps`; roots must agree."""
+    f = core.KOALABEAR
+    shapes = shapes or shard_shapes()
+    pcs = core.TwoAdicFriPcs(ctx, f, 1)
+    pinned = []
+    for i, (L, w) in enumerate(shapes):
+        m = ctx.random_matrix(f, (3 << 32) | (100 + i), 1 << L, w)
+        pm = ctx.host_matrix(1 << L, w)
+        core.check(core.lib().pg_download(ctx.handle, m.handle, 0, 1 << L, pm.ptr, -1, 0))
+        m.free()
+        pinned.append(pm)
+    pageable = [np.array(pm.array) for pm in pinned]
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    cells = sum((1 << L) * w for L, w in shapes)
+    nbytes = 4 * cells
+    out = {"cells": cells, "bytes": nbytes}
+    roots = []
+    for mem, arrays in (("pinned", [pm.array for pm in pinned]), ("pageable", pageable)):
+        best = {}
+        for _ in range(reps):
+            core.device_synchronize()
+            t0 = time.perf_counter()
+            devs = [ctx.upload(a) for a in arrays]
+            t1 = time.perf_counter()
+            r1, pd = pcs.commit(list(zip(doms, devs)))
+            t2 = time.perf_counter()
+            pd.free()
+            for d in devs:
+                d.free()
+            t3 = time.perf_counter()
+            r2, pd = pcs.commit_host(list(zip(doms, arrays)), canonical=False)
+            t4 = time.perf_counter()
+            pd.free()
+            roots += [r1, r2]
+            cur = {"upload_ms": (t1 - t0) * 1e3, "upload_then_commit_ms": (t2 - t0) * 1e3,
+                   "commit_host_ms": (t4 - t3) * 1e3}
+            best = cur if not best or cur["commit_host_ms"] < best["commit_host_ms"] else best
+        best["upload_GB_per_s"] = nbytes / (best["upload_ms"] / 1e3) / 1e9
+        best["commit_host_cells_per_s"] = cells / (best["commit_host_ms"] / 1e3)
+        out[mem] = best
+    out["roots_identical"] = bool(all(np.array_equal(r, roots[0]) for r in roots))
+    for pm in pinned:
+        pm.free()
+    return out
 
 
 # ---------------------------------------------------------------- batched --

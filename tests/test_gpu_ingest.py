@@ -143,3 +143,88 @@ def test_quotient_chunks_recombine_at_zeta(ctx, f):
         assert got == want[c], c
     pd.free()
     dev.free()
+
+
+def _to_monty(f, a):
+    p = pico_amd.PRIMES[f]
+    return ((a.astype(np.uint64) << np.uint64(32)) % np.uint64(p)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("lb", [1, 2])
+def test_commit_host_matches_upload_then_commit(ctx, oracle, f, lb):
+    """Mixed heights in an order that is not sorted (the tree sorts, uploads
+    follow the tree's order), widths that are and are not multiples of 8."""
+    p = pico_amd.PRIMES[f]
+    shapes = [(9, 16), (6, 5), (9, 3), (12, 8), (3, 40), (12, 1), (0, 7)]
+    hosts = [oracle.random_matrix(f, 5000 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, lb)
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    # one matrix on a shifted (quotient-like) domain
+    doms[1] = pico_amd.TwoAdicMultiplicativeCoset(6, pico_amd.GENERATORS[f] * pico_amd.two_adic_generator(f, 8) % p, f)
+    root_h, pd_h = pcs.commit_host(list(zip(doms, hosts)))
+    root_d, pd_d = pcs.commit([(d, ctx.upload(a, field=f)) for d, a in zip(doms, hosts)])
+    ldes = [oracle.coset_lde(f, a, lb, pico_amd.GENERATORS[f] * pow(d.shift, p - 2, p) % p) for d, a in zip(doms, hosts)]
+    assert np.array_equal(root_h, oracle.MerkleTree(f, ldes).root)
+    assert np.array_equal(root_h, root_d)
+    for k, d in enumerate(doms):
+        assert np.array_equal(pcs.get_evaluations_on_domain(pd_h, k, d), pcs.get_evaluations_on_domain(pd_d, k, d)), k
+    # Montgomery words on the host, from page-locked memory
+    pinned = []
+    for a in hosts:
+        pm = ctx.host_matrix(*a.shape)
+        pm.array[:] = _to_monty(f, a)
+        pinned.append(pm)
+    root_m, pd_m = pcs.commit_host([(d, pm.array) for d, pm in zip(doms, pinned)], canonical=False)
+    assert np.array_equal(root_m, root_h)
+    for x in (pd_h, pd_d, pd_m):
+        x.free()
+    for pm in pinned:
+        pm.free()
+
+
+def test_commit_host_rejects_bad_shapes_without_leaking(ctx):
+    f = KOALABEAR
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    base = ctx.bytes_in_use
+    good = np.ones((64, 4), np.uint32)
+    for bad in [np.ones((48, 4), np.uint32), np.ones((64, 0), np.uint32)]:
+        with pytest.raises(PgError):
+            pcs.commit_host([(pcs.natural_domain_for_degree(64), good), (pcs.natural_domain_for_degree(64), bad)])
+    too_big = pico_amd.TwoAdicFriPcs(ctx, f, 30)
+    with pytest.raises(PgError):
+        too_big.commit_host([(pcs.natural_domain_for_degree(64), good)])
+    assert ctx.bytes_in_use == base
+
+
+def test_commit_host_column_blocks_match_oracle(ctx, oracle):
+    """Tallest matrices large enough to be cut into column blocks (uploaded,
+    extended and absorbed block by block): widths 300 (blocks 128, 128, 44)
+    and 200 (128, 72, starting mid-chunk at input 300 = 37 * 8 + 4), an
+    injected shorter matrix, Montgomery words in page-locked memory."""
+    f = KOALABEAR
+    p = pico_amd.PRIMES[f]
+    shapes = [(17, 300), (15, 9), (17, 200)]
+    hosts = [oracle.random_matrix(f, 6000 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    pinned = []  # column blocks are cut only from page-locked memory
+    for a in hosts:
+        pm = ctx.host_matrix(*a.shape)
+        pm.array[:] = _to_monty(f, a)
+        pinned.append(pm)
+    root_h, pd_h = pcs.commit_host([(d, pm.array) for d, pm in zip(doms, pinned)], canonical=False)
+    root_d, pd_d = pcs.commit([(d, ctx.upload(a, field=f)) for d, a in zip(doms, hosts)])
+    assert np.array_equal(root_h, root_d)
+    ldes = [oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f]) for a in hosts]
+    assert np.array_equal(root_h, oracle.MerkleTree(f, ldes).root)
+    mats = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd_h)
+    for k in (0, 2):
+        assert np.array_equal(mats[k].to_host(f, row0=1000, nrows=64), ldes[k][1000:1064]), k
+    # the same from pageable memory (whole matrices)
+    root_p, pd_p = pcs.commit_host([(d, pm.array.copy()) for d, pm in zip(doms, pinned)], canonical=False)
+    assert np.array_equal(root_p, root_h)
+    for x in (pd_h, pd_d, pd_p):
+        x.free()
+    for pm in pinned:
+        pm.free()
