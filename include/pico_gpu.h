@@ -45,6 +45,7 @@ extern "C" {
 #define PG_E_DEVICE (-3)      /* HIP runtime / kernel error */
 #define PG_E_ADMISSION (-4)   /* would exceed the context's HBM budget */
 #define PG_E_UNSUPPORTED (-5) /* valid request outside the implemented range */
+#define PG_E_CANCELLED (-6)   /* service job cancelled before it ran */
 
 #define PG_KOALABEAR 0 /* p = 2^31 - 2^24 + 1 */
 #define PG_BABYBEAR 1  /* p = 2^31 - 2^27 + 1 */
@@ -85,6 +86,12 @@ size_t pg_bytes_in_use(const pg_ctx *ctx);
 int pg_profile_enable(pg_ctx *ctx, int enable);
 int pg_profile_reset(pg_ctx *ctx);
 int pg_profile_get(pg_ctx *ctx, const char *family, double *total_ms, uint64_t *launches);
+/* Fault injection for tests of the error paths: the device allocation
+ * `after` allocations from now (0 = the next one) fails once with
+ * PG_E_OOM; a negative value disarms.  No counterpart in the reference
+ * (which panics on allocation failure); every entry point must then return
+ * the error and leave no device memory behind. */
+int pg_debug_fail_alloc(pg_ctx *ctx, long long after);
 
 /* ---- device matrices ------------------------------------------------------ */
 int pg_buf_alloc(pg_ctx *ctx, size_t rows, size_t cols, pg_buf **out);
@@ -251,6 +258,12 @@ int pg_service_wait(pg_service *svc, uint64_t job_id, uint32_t root[8], double t
 int pg_service_open(pg_service *svc, uint64_t job_id, const uint64_t *indices, size_t n_idx,
                     uint32_t *rows_out, uint32_t *paths_out);
 int pg_service_release(pg_service *svc, uint64_t job_id);
+/* Takes a job that has not started off the queue: it never runs, its trace
+ * is freed and pg_service_wait returns PG_E_CANCELLED for it (release it as
+ * any finished job).  A running or finished job: PG_E_INVALID.  No
+ * counterpart in the reference (it has no job queue); for a prover that
+ * abandons a shard. */
+int pg_service_cancel(pg_service *svc, uint64_t job_id);
 int pg_service_stats(pg_service *svc, size_t *reserved_bytes, size_t *pending_jobs, size_t *peak_running);
 void pg_service_destroy(pg_service *svc);
 

@@ -15,6 +15,7 @@ PG_E_OOM = -2
 PG_E_DEVICE = -3
 PG_E_ADMISSION = -4
 PG_E_UNSUPPORTED = -5
+PG_E_CANCELLED = -6
 
 KOALABEAR = 0
 BABYBEAR = 1
@@ -55,6 +56,7 @@ SIGNATURES = {
     "pg_profile_reset": (_i, [_P]),
     "pg_profile_get": (_i, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_uint64)]),
+    "pg_debug_fail_alloc": (_i, [_P, ctypes.c_longlong]),
     "pg_buf_alloc": (_i, [_P, _sz, _sz, _pp]),
     "pg_buf_wrap": (_i, [_P, _P, _sz, _sz, _pp]),
     "pg_buf_free": (None, [_P]),
@@ -95,6 +97,7 @@ SIGNATURES = {
     "pg_service_wait": (_i, [_P, _u64, _P, _P]),
     "pg_service_open": (_i, [_P, _u64, _P, _sz, _P, _P]),
     "pg_service_release": (_i, [_P, _u64]),
+    "pg_service_cancel": (_i, [_P, _u64]),
     "pg_service_stats": (_i, [_P, _P, _P, _P]),
     "pg_service_destroy": (None, [_P]),
     "pg_fri_fold": (_i, [_P, _i, _P, _P, _pp]),

@@ -71,6 +71,11 @@ class Context:
     def profile(self, enable=True):
         check(lib().pg_profile_enable(self.handle, int(enable)))
 
+    def debug_fail_alloc(self, after):
+        """Make the device allocation `after` allocations from now fail once
+        with PG_E_OOM (negative: disarm).  For tests of error paths."""
+        check(lib().pg_debug_fail_alloc(self.handle, int(after)))
+
     def profile_reset(self):
         check(lib().pg_profile_reset(self.handle))
 
@@ -550,6 +555,11 @@ class CommitService:
 
     def release(self, job_id):
         check(lib().pg_service_release(self.handle, job_id))
+
+    def cancel(self, job_id):
+        """Take a job that has not started off the queue (wait() then raises
+        PgError with PG_E_CANCELLED); release it afterwards."""
+        check(lib().pg_service_cancel(self.handle, job_id))
 
     def stats(self):
         r, p, k = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()

@@ -109,6 +109,9 @@ struct pg_ctx {
     // per-kernel-family event timings (profiling mode only)
     std::atomic<bool> profiling{false};
     std::map<std::string, std::pair<double, uint64_t>> prof;
+    // fault injection: the allocation this many allocations from now fails
+    // once with PG_E_OOM; negative = off
+    std::atomic<long long> fail_countdown{-1};
 };
 
 // Times each launch with a pair of HIP events recorded on the launch stream
@@ -254,6 +257,8 @@ static hipError_t pool_grow(pg_ctx* c, size_t n) {  // pool_mu held
 
 static int ctx_alloc(pg_ctx* ctx, size_t bytes, void** out) {
     *out = nullptr;
+    if (ctx->fail_countdown.load() >= 0 && ctx->fail_countdown.fetch_sub(1) == 0)
+        return fail(PG_E_OOM, "injected allocation failure (%zu bytes)", bytes);
     const size_t n = round_up(bytes ? bytes : 1, POOL_ALIGN);
     size_t prev = ctx->in_use.fetch_add(n);
     if (ctx->budget && prev + n > ctx->budget) {
@@ -384,6 +389,12 @@ extern "C" int pg_profile_get(pg_ctx* ctx, const char* name, double* total_ms, u
     auto it = ctx->prof.find(name);
     *total_ms = it == ctx->prof.end() ? 0.0 : it->second.first;
     *launches = it == ctx->prof.end() ? 0 : it->second.second;
+    return PG_OK;
+}
+
+extern "C" int pg_debug_fail_alloc(pg_ctx* ctx, long long after) {
+    if (!ctx) return fail(PG_E_INVALID, "null context");
+    ctx->fail_countdown.store(after < 0 ? -1 : after);
     return PG_OK;
 }
 
@@ -1361,44 +1372,48 @@ static int reduce_openings_t(pg_ctx* ctx, hipStream_t s, const pg_tree* const* r
     if (!tables) return rc;
     uint8_t* dt = (uint8_t*)tables->ptr;
     HIP_TRY(hipMemcpyAsync(dt, host.data(), bytes, hipMemcpyHostToDevice, s));
+    // every table and output is in place before the first launch, so a
+    // failed allocation never frees memory a queued kernel still uses
     std::vector<pg_buf*> made;
-    auto cleanup = [&]() {
-        for (auto* b : made) pg_buf_free(b);
-    };
-    auto kt = make_timer(ctx);
     std::vector<std::shared_ptr<DevMem>> keep;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);
+        for (auto* b : made) pg_buf_free(b);
+        for (int lh = 0; lh < 33; lh++) out_by_log[lh] = nullptr;
+    };
     for (auto& la : launches) {
         std::shared_ptr<DevMem> tw;
         rc = get_twiddles(ctx, s, field, la.lh, &tw);
+        if (rc == PG_OK) {
+            keep.push_back(tw);
+            pg_buf* o;
+            rc = buf_new(ctx, (size_t)1 << la.lh, 4, &o);
+            if (rc == PG_OK) made.push_back(o);
+        }
         if (rc != PG_OK) {
             cleanup();
             return rc;
         }
-        keep.push_back(tw);
-        pg_buf* o;
-        rc = buf_new(ctx, (size_t)1 << la.lh, 4, &o);
-        if (rc != PG_OK) {
-            cleanup();
-            return rc;
-        }
-        made.push_back(o);
-        out_by_log[la.lh] = o;
+    }
+    auto kt = make_timer(ctx);
+    for (size_t i = 0; i < launches.size(); i++) {
+        const Launch& la = launches[i];
         if (kt) kt->begin("reduce_openings", s);
         hipError_t e = launch_reduce_rows(field, (const RoMat*)(dt + la.off_mats), la.nm, (const Ext4*)(dt + la.off_pts),
-                                          la.np, (const Ext4*)dt, la.lh, (const uint32_t*)tw->ptr, (Ext4*)o->ptr, s);
+                                          la.np, (const Ext4*)dt, la.lh, (const uint32_t*)keep[i]->ptr,
+                                          (Ext4*)made[i]->ptr, s);
         if (kt) kt->end(s);
         if (e != hipSuccess) {
             cleanup();
-            for (int lh = 0; lh < 33; lh++) out_by_log[lh] = nullptr;
             return fail(PG_E_DEVICE, "reduce_rows: %s", hipGetErrorString(e));
         }
     }
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         cleanup();
-        for (int lh = 0; lh < 33; lh++) out_by_log[lh] = nullptr;
         return fail(PG_E_DEVICE, "reduce_openings: %s", hipGetErrorString(e));
     }
+    for (size_t i = 0; i < launches.size(); i++) out_by_log[launches[i].lh] = made[i];
     return PG_OK;
 }
 
@@ -1754,6 +1769,7 @@ static void run_svc_job(pg_service* sv, SvcJob* j, hipStream_t s) {
     pg_buf* lde = nullptr;
     std::shared_ptr<DevMem> scratch;
     if (rc == PG_OK) rc = lde_on_stream(ctx, s, j->field, tr, j->lb, gen_monty(j->field), 1, &lde, &scratch);
+    if (rc != PG_OK) (void)hipStreamSynchronize(s);  // the fill may still be writing the trace
     scratch.reset();
     if (tr) pg_buf_free(tr);
     j->trace = nullptr;
@@ -1907,6 +1923,36 @@ extern "C" int pg_service_open(pg_service* sv, uint64_t id, const uint64_t* idx,
         t = it->second->tree;
     }
     return pg_mmcs_open_batch(sv->ctx, t, idx, nq, rows_out, paths_out);
+}
+
+extern "C" int pg_service_cancel(pg_service* sv, uint64_t id) {
+    if (!sv) return fail(PG_E_INVALID, "null service");
+    pg_buf* trace = nullptr;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        auto it = sv->jobs.find(id);
+        if (it == sv->jobs.end()) return fail(PG_E_INVALID, "unknown job %llu", (unsigned long long)id);
+        SvcJob* j = it->second.get();
+        if (j->state != 0)
+            return fail(PG_E_INVALID, "job %llu is already %s", (unsigned long long)id,
+                        j->state == 1 ? "running" : "finished");
+        for (auto q = sv->pending.begin(); q != sv->pending.end(); ++q)
+            if (*q == id) {
+                sv->pending.erase(q);
+                break;
+            }
+        j->state = 2;
+        j->rc = PG_E_CANCELLED;
+        j->err = "job cancelled before it ran";
+        j->t_done = now_s();
+        j->footprint = 0;  // never reserved
+        trace = j->trace;
+        j->trace = nullptr;
+        sv->cv_done.notify_all();
+        sv->cv_work.notify_all();  // the queue head may have changed
+    }
+    pg_buf_free(trace);
+    return PG_OK;
 }
 
 extern "C" int pg_service_release(pg_service* sv, uint64_t id) {
@@ -2077,6 +2123,7 @@ extern "C" int pg_fri_commit_phase(pg_ctx* ctx, int field, pg_challenger* ch, co
         pg_tree* t;
         int rc = mmcs_commit_on_stream(ctx, l.s, field, {leaves}, {true}, roots_out + 8 * r, &t);
         if (rc != PG_OK) {
+            (void)hipStreamSynchronize(l.s);  // the first round's copy may still be writing it
             pg_buf_free(leaves);
             return rc;
         }
