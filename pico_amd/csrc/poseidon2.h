@@ -231,12 +231,28 @@ PG_HD uint32_t fmsub_small(uint32_t x, uint32_t S) {
     return fmadd_small<F, C>(F::P - x, S);
 }
 
+// x 2^-K mod p for x < p, where p = 1 (mod 2^K) (KoalaBear: K <= 24,
+// BabyBear: K <= 27).  With m = -x mod 2^K, x + m p = x + m = 0 (mod 2^K),
+// and x + m p < 2^K p, so (x + m p) / 2^K is the product, already below p:
+// a masked negation, one 64-bit multiply-add and a funnel shift.  The map is
+// linear, so it applies to Montgomery representatives as they are.
+template <class F, int K>
+PG_HD uint32_t fdiv_pow2(uint32_t x) {
+    static_assert(K >= 1 && K < 32 && ((F::P - 1u) & ((1u << K) - 1u)) == 0, "needs p = 1 mod 2^K");
+    const uint32_t m = (0u - x) & ((1u << K) - 1u);
+    const uint64_t v = (uint64_t)m * F::P + x;
+    return (uint32_t)(v >> K);
+}
+
 // s_i <- s_i * diag_i + S.  Both fields' diagonals (consts.rs:110-154) are
 //   -2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, then +-2^-k entries and +-c
 // with c = 127 (KoalaBear: entries 11 = -127, 15 = 127) or c = 15 (BabyBear:
 // 12 = -15, 15 = 15).  The entries -2 .. 1/2 and -1/2 use additions and a
 // halving, the small integers a 64-bit multiply-add (fmadd_small), the
-// powers of two a Montgomery product.
+// powers of two fdiv_pow2.  As integers the table holds p - (p-1)/2^k
+// (= +2^-k) and (p-1)/2^k (= -2^-k):
+//   KoalaBear  9: +2^-8  10: +2^-3  12: -2^-8  13: -2^-3  14: -2^-4
+//   BabyBear   9: +2^-8  10: +2^-2  11: +2^-3  13: -2^-8  14: -2^-4
 template <class F>
 PG_HD void internal_layer(uint32_t (&s)[16]) {
     constexpr bool KB = F::P == KoalaBear::P;
@@ -253,12 +269,18 @@ PG_HD void internal_layer(uint32_t (&s)[16]) {
     s[8] = fmsub_small<F, 4>(s[8], sum);
     if constexpr (KB) {
         s[11] = fmsub_small<F, CBIG>(s[11], sum);
-#pragma unroll
-        for (int i : {9, 10, 12, 13, 14}) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
+        s[9] = fadd<F>(fdiv_pow2<F, 8>(s[9]), sum);
+        s[10] = fadd<F>(fdiv_pow2<F, 3>(s[10]), sum);
+        s[12] = fsub<F>(sum, fdiv_pow2<F, 8>(s[12]));
+        s[13] = fsub<F>(sum, fdiv_pow2<F, 3>(s[13]));
+        s[14] = fsub<F>(sum, fdiv_pow2<F, 4>(s[14]));
     } else {
         s[12] = fmsub_small<F, CBIG>(s[12], sum);
-#pragma unroll
-        for (int i : {9, 10, 11, 13, 14}) s[i] = fadd<F>(fmul<F>(s[i], P2<F>::K.diag[i]), sum);
+        s[9] = fadd<F>(fdiv_pow2<F, 8>(s[9]), sum);
+        s[10] = fadd<F>(fdiv_pow2<F, 2>(s[10]), sum);
+        s[11] = fadd<F>(fdiv_pow2<F, 3>(s[11]), sum);
+        s[13] = fsub<F>(sum, fdiv_pow2<F, 8>(s[13]));
+        s[14] = fsub<F>(sum, fdiv_pow2<F, 4>(s[14]));
     }
     s[15] = fmadd_small<F, CBIG>(s[15], sum);
 }
