@@ -54,6 +54,7 @@ typedef struct pg_ctx pg_ctx;   /* one device + HBM budget + table caches */
 typedef struct pg_buf pg_buf;   /* device matrix (rows x cols u32) */
 typedef struct pg_tree pg_tree; /* MMCS prover data: matrices + digest layers */
 typedef struct pg_fri pg_fri;   /* FRI commit-phase prover data */
+typedef struct pg_open_proof pg_open_proof; /* everything Pcs::open returns */
 
 /* DuplexChallenger<F, Poseidon2, 16, 8> state (Montgomery values).
  * Replaces the reference's DuplexChallenger (restated in-tree at
@@ -216,6 +217,35 @@ int pg_pcs_reduce_openings(pg_ctx *ctx, const pg_tree *const *rounds, size_t nro
                            const uint32_t *npoints, const uint32_t *points_mont,
                            const uint32_t *values_mont, const uint32_t alpha_mont[4],
                            pg_buf **out_by_log);
+/* The whole of TwoAdicFriPcs::open (the reference calls it at
+ * vm/src/machine/prover.rs:588-589; the verifier replays it at
+ * vm/src/compiler/recursion/circuit/fri.rs:36-73,84-99,118-241) in one call:
+ * values of every matrix at its points, each observed on `ch` (round ->
+ * matrix -> point -> column, 4 words); alpha = ch.sample_ext(); reduced
+ * openings per LDE height; the FRI commit phase down to 2^log_blowup (the
+ * final vector must be constant and one value is observed) or, with
+ * log_final >= 0, down to 2^log_final observing all of it; the PoW grind;
+ * num_queries indices of log_max bits; every input opening and every
+ * commit-phase opening.  npoints[m] points (<= 2) for matrix m in round ->
+ * matrix order, points_mont their ext4 words in the same order.  The
+ * challenger ends in the state the verifier's replay reaches. */
+int pg_pcs_open(pg_ctx *ctx, const pg_tree *const *rounds, size_t nrounds, const uint32_t *npoints,
+                const uint32_t *points_mont, uint32_t log_blowup, int log_final, size_t num_queries,
+                int pow_bits, pg_challenger *ch, pg_open_proof **out);
+/* Sizes of the arrays pg_open_proof_read fills (any pointer may be null):
+ * values n_values words; alpha 4; commits 8 per FRI round; betas 4 per
+ * round; final_poly final_words; pow witness 1 (canonical); query_indices
+ * num_queries (u64); input_openings input_words_per_query per query (for
+ * each round in order: its rows, total-width words, then its path, 8 words
+ * per level); fri_siblings 4 per round per query; fri_paths
+ * fri_path_words_per_query per query (round 0's path first).  All other
+ * words are Montgomery. */
+int pg_open_proof_sizes(const pg_open_proof *p, size_t *n_values, int *fri_rounds, size_t *final_words,
+                        size_t *num_queries, size_t *input_words_per_query, size_t *fri_path_words_per_query);
+int pg_open_proof_read(const pg_open_proof *p, uint32_t *values, uint32_t alpha[4], uint32_t *commits,
+                       uint32_t *betas, uint32_t *final_poly, uint32_t *pow_witness, uint64_t *query_indices,
+                       uint32_t *input_openings, uint32_t *fri_siblings, uint32_t *fri_paths);
+void pg_open_proof_free(pg_open_proof *p);
 
 /* ---- batched commits ------------------------------------------------------
  * Many independent single-matrix Pcs::commit jobs (the reference proves the

@@ -86,6 +86,10 @@ SIGNATURES = {
     "pg_pcs_evaluations": (_i, [_P, _P, _sz, _sz, _P]),
     "pg_pcs_open_values": (_i, [_P, _P, _sz, _u32, _P, _sz, _P]),
     "pg_pcs_reduce_openings": (_i, [_P, _P, _sz, _P, _P, _P, _P, _P]),
+    "pg_pcs_open": (_i, [_P, _P, _sz, _P, _P, _u32, _i, _sz, _i, _P, _pp]),
+    "pg_open_proof_sizes": (_i, [_P, _P, _P, _P, _P, _P, _P]),
+    "pg_open_proof_read": (_i, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "pg_open_proof_free": (None, [_P]),
     "pg_batch_create": (_i, [_P, _i, _P, _P, _P, _sz, _i, _pp]),
     "pg_batch_run": (_i, [_P, _P, _P]),
     "pg_batch_take_tree": (_i, [_P, _sz, _pp]),

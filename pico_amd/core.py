@@ -435,41 +435,90 @@ class TwoAdicFriPcs:
         return {lh: DeviceMatrix(self.ctx, outs[lh], owned=True) for lh in range(33) if outs[lh]}
 
     def open(self, rounds, challenger, num_queries, pow_bits, log_final=None):
-        """TwoAdicFriPcs::open: rounds = [(ProverData, [points per matrix])].
-        Evaluates, has the challenger observe every value (round -> matrix ->
-        point -> column, 4 coefficients each), samples alpha, reduces, runs the
-        FRI commit phase down to 2^log_final (default log_blowup, the
-        reference's stop, final vector constant), grinds, samples the query
-        indices and answers them.  Returns (opened_values, proof dict)."""
-        opened = []
+        """TwoAdicFriPcs::open through pg_pcs_open: rounds = [(ProverData,
+        [points per matrix])].  Evaluates, has the challenger observe every
+        value (round -> matrix -> point -> column, 4 coefficients each),
+        samples alpha, reduces, runs the FRI commit phase down to 2^log_final
+        (default log_blowup, the reference's stop, final vector constant),
+        grinds, samples the query indices and answers them.  Returns
+        (opened_values, proof dict), all canonical."""
+        f = self.field
+        npts, pts, widths = [], [], []
         for d, mat_points in rounds:
-            per_mat = []
-            for k, p in enumerate(mat_points):
+            ws = [lib().pg_buf_cols(lib().pg_tree_matrix(d.handle, k)) for k in range(d.num_matrices)]
+            if len(mat_points) != len(ws):
+                raise ValueError("one list of points per committed matrix")
+            widths.append(ws)
+            for p in mat_points:
                 p = np.asarray(p, np.uint64).reshape(-1, 4)
-                per_mat.append(self.open_values(d, k, p) if len(p) else np.zeros((0, 0, 4), np.uint32))
+                npts.append(len(p))
+                pts.append(p)
+        trees = (ctypes.c_void_p * len(rounds))(*[d.handle for d, _ in rounds])
+        npts_a = np.ascontiguousarray(npts, dtype=np.uint32)
+        pts_a = np.ascontiguousarray(to_monty(f, np.concatenate(pts) if pts else np.zeros((0, 4), np.uint64)))
+        h = ctypes.c_void_p()
+        check(lib().pg_pcs_open(self.ctx.handle, trees, len(rounds), _ptr(npts_a), _ptr(pts_a), self.log_blowup,
+                                -1 if log_final is None else int(log_final), int(num_queries), int(pow_bits),
+                                ctypes.byref(challenger.c), ctypes.byref(h)))
+        try:
+            sz = [ctypes.c_size_t() for _ in range(5)]
+            nr = ctypes.c_int()
+            check(lib().pg_open_proof_sizes(h, ctypes.byref(sz[0]), ctypes.byref(nr), ctypes.byref(sz[1]),
+                                            ctypes.byref(sz[2]), ctypes.byref(sz[3]), ctypes.byref(sz[4])))
+            nvals, nfinal, nq, inw, fpw = (x.value for x in sz)
+            R = nr.value
+            values = np.zeros(max(nvals, 1), np.uint32)
+            alpha = np.zeros(4, np.uint32)
+            commits = np.zeros((max(R, 1), 8), np.uint32)
+            betas = np.zeros((max(R, 1), 4), np.uint32)
+            final = np.zeros((nfinal // 4, 4), np.uint32)
+            wit = ctypes.c_uint32()
+            qidx = np.zeros(max(nq, 1), np.uint64)
+            inputs = np.zeros(max(nq * inw, 1), np.uint32)
+            sib = np.zeros((max(nq, 1), max(R, 1), 4), np.uint32)
+            fpaths = np.zeros(max(nq * fpw, 1), np.uint32)
+            check(lib().pg_open_proof_read(h, _ptr(values), _ptr(alpha), _ptr(commits), _ptr(betas), _ptr(final),
+                                           ctypes.byref(wit), _ptr(qidx), _ptr(inputs), _ptr(sib), _ptr(fpaths)))
+        finally:
+            lib().pg_open_proof_free(h)
+        values = from_monty(f, values[:nvals])
+        opened, off, mi = [], 0, 0
+        for ws in widths:
+            per_mat = []
+            for w in ws:
+                n = npts[mi] * w * 4
+                per_mat.append(values[off:off + n].reshape(npts[mi], w, 4))
+                off += n
+                mi += 1
             opened.append(per_mat)
-        for per_mat in opened:
-            for v in per_mat:
-                challenger.observe(np.asarray(v).ravel())
-        alpha = challenger.sample_ext()
-        ro = self.reduce_openings(rounds, opened, alpha)
-        log_stop = self.log_blowup if log_final is None else log_final
-        fri = fri_commit_phase(self.ctx, self.field, challenger, ro, log_stop,
-                               final_mode=0 if log_final is None else 1)
-        witness = fri_grind(self.ctx, self.field, challenger, pow_bits)
-        log_max = max(ro)
-        idx = [challenger.sample_bits(log_max) for _ in range(num_queries)]
-        mmcs = MerkleTreeMmcs(self.ctx, self.field)
-        inputs = []
-        for i in idx:
-            per_round = []
-            for d, _ in rounds:
-                per_round.append(mmcs.open_batch(i >> (log_max - d.log_height), d))
-            inputs.append(per_round)
-        sib, paths = fri_answer_queries(fri, idx, log_max)
-        proof = {"alpha": alpha, "commits": fri.commits, "betas": fri.betas, "final_poly": fri.final_poly,
-                 "pow_witness": witness, "query_indices": idx, "input_openings": inputs,
-                 "fri_siblings": sib, "fri_paths": paths, "reduced_openings": ro, "fri_data": fri}
+        log_heights = [d.log_height for d, _ in rounds]
+        input_openings = []
+        inputs = from_monty(f, inputs[:nq * inw]).reshape(nq, inw) if nq else inputs[:0]
+        for q in range(nq):
+            per_round, o = [], 0
+            for ws, lh in zip(widths, log_heights):
+                rows = []
+                for w in ws:
+                    rows.append(inputs[q, o:o + w])
+                    o += w
+                path = inputs[q, o:o + 8 * lh].reshape(lh, 8)
+                o += 8 * lh
+                per_round.append((rows, path))
+            input_openings.append(per_round)
+        log_max = max(log_heights)  # pg_pcs_open rejects a round taller than every opened matrix
+        plens = [log_max - 1 - r for r in range(R)]
+        fpaths = from_monty(f, fpaths[:nq * fpw]).reshape(nq, fpw) if nq else fpaths[:0]
+        fri_paths = []
+        for q in range(nq):
+            lst, o = [], 0
+            for L in plens:
+                lst.append(fpaths[q, o:o + 8 * L].reshape(L, 8))
+                o += 8 * L
+            fri_paths.append(lst)
+        proof = {"alpha": from_monty(f, alpha), "commits": from_monty(f, commits[:R]),
+                 "betas": from_monty(f, betas[:R]), "final_poly": from_monty(f, final), "pow_witness": wit.value,
+                 "query_indices": [int(x) for x in qidx[:nq]], "input_openings": input_openings,
+                 "fri_siblings": from_monty(f, sib[:nq, :R]), "fri_paths": fri_paths}
         return opened, proof
 
 

@@ -2225,3 +2225,163 @@ extern "C" int pg_fri_answer_queries(pg_ctx* ctx, const pg_fri* f, const uint64_
     }
     return PG_OK;
 }
+
+// ------------------------------------------------------------ Pcs::open --
+// The whole of TwoAdicFriPcs::open in one call, in the order the verifier
+// replays it (vm/src/compiler/recursion/circuit/fri.rs:36-73, 84-99,
+// 118-241): the values of every matrix at its points (round -> matrix ->
+// point -> column), each observed; alpha; the reduced openings per height;
+// the FRI commit phase (a commitment, an observation and a beta per round,
+// the final vector observed); the proof-of-work grind; the query indices;
+// every input opening and every commit-phase opening.
+struct pg_open_proof {
+    int field = 0;
+    std::vector<uint32_t> values;  // Montgomery words, round -> matrix -> point -> column -> 4
+    uint32_t alpha[4] = {0, 0, 0, 0};
+    int fri_rounds = 0;
+    std::vector<uint32_t> commits, betas, final_poly;
+    uint32_t pow_witness = 0;
+    std::vector<uint64_t> queries;
+    size_t input_words = 0;  // per query: for each round, its rows then its path
+    std::vector<uint32_t> inputs;
+    size_t fri_path_words = 0;  // per query
+    std::vector<uint32_t> siblings, fri_paths;
+};
+
+extern "C" void pg_open_proof_free(pg_open_proof* p) { delete p; }
+
+extern "C" int pg_pcs_open(pg_ctx* ctx, const pg_tree* const* rounds, size_t nrounds, const uint32_t* npoints,
+                           const uint32_t* points_mont, uint32_t log_blowup, int log_final, size_t num_queries,
+                           int pow_bits, pg_challenger* ch, pg_open_proof** out) {
+    if (!ctx || !rounds || nrounds == 0 || !npoints || !ch || !out || !rounds[0])
+        return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    const int field = rounds[0]->field;
+    if (ch->field != field) return fail(PG_E_INVALID, "challenger field differs from the commitments'");
+    size_t nvals = 0, npts_total = 0, mi = 0;
+    for (size_t r = 0; r < nrounds; r++) {
+        if (!rounds[r] || rounds[r]->field != field) return fail(PG_E_INVALID, "round %zu: missing tree or mixed fields", r);
+        for (size_t k = 0; k < rounds[r]->mats.size(); k++, mi++) {
+            nvals += (size_t)npoints[mi] * rounds[r]->mats[k]->cols * 4;
+            npts_total += npoints[mi];
+        }
+    }
+    if (npts_total == 0) return fail(PG_E_INVALID, "no opening points");
+    if (!points_mont) return fail(PG_E_INVALID, "null points");
+    std::unique_ptr<pg_open_proof> pf(new pg_open_proof);
+    pf->field = field;
+    // 1. values, observed in order
+    pf->values.resize(nvals);
+    size_t voff = 0, poff = 0;
+    mi = 0;
+    for (size_t r = 0; r < nrounds; r++)
+        for (size_t k = 0; k < rounds[r]->mats.size(); k++, mi++) {
+            const uint32_t np = npoints[mi];
+            if (np)
+                PG_TRY(pg_pcs_open_values(ctx, rounds[r], k, log_blowup, points_mont + 4 * poff, np,
+                                          pf->values.data() + voff));
+            voff += (size_t)np * rounds[r]->mats[k]->cols * 4;
+            poff += np;
+        }
+    for (uint32_t v : pf->values) pg_challenger_observe(ch, v);
+    // 2. alpha, reduced openings per height
+    pg_challenger_sample_ext(ch, pf->alpha);
+    pg_buf* by_log[33];
+    PG_TRY(pg_pcs_reduce_openings(ctx, rounds, nrounds, npoints, points_mont, pf->values.data(), pf->alpha, by_log));
+    auto free_reduced = [&]() {
+        for (auto*& b : by_log) {
+            pg_buf_free(b);
+            b = nullptr;
+        }
+    };
+    int log_max = -1;
+    for (int lh = 0; lh < 33; lh++)
+        if (by_log[lh]) log_max = lh;
+    const int log_stop = log_final < 0 ? (int)log_blowup : log_final;
+    if (log_max < 0 || log_stop > log_max) {
+        free_reduced();
+        return fail(PG_E_INVALID, "FRI stop 2^%d above the tallest reduced opening 2^%d", log_stop, log_max);
+    }
+    for (size_t r = 0; r < nrounds; r++)
+        if (rounds[r]->log_h > log_max) {
+            free_reduced();
+            return fail(PG_E_INVALID, "round %zu is taller than every opened matrix", r);
+        }
+    // 3. FRI commit phase
+    const int max_rounds = log_max - log_stop;
+    pf->commits.resize(8 * std::max(max_rounds, 1));
+    pf->betas.resize(4 * std::max(max_rounds, 1));
+    pf->final_poly.resize(4 * ((size_t)1 << log_stop));
+    pg_fri* fri = nullptr;
+    int rc = pg_fri_commit_phase(ctx, field, ch, by_log, log_max, log_stop, log_final < 0 ? 0 : 1,
+                                 pf->commits.data(), pf->betas.data(), pf->final_poly.data(), &pf->fri_rounds, &fri);
+    free_reduced();
+    if (rc != PG_OK) return rc;
+    std::unique_ptr<pg_fri, void (*)(pg_fri*)> fri_guard(fri, pg_fri_free);
+    pf->commits.resize(8 * (size_t)pf->fri_rounds);
+    pf->betas.resize(4 * (size_t)pf->fri_rounds);
+    // 4. proof of work, query indices
+    PG_TRY(pg_fri_grind(ctx, field, ch, pow_bits, &pf->pow_witness));
+    pf->queries.resize(num_queries);
+    for (auto& q : pf->queries) q = pg_challenger_sample_bits(ch, log_max);
+    // 5. input openings, round by round, interleaved per query
+    for (size_t r = 0; r < nrounds; r++) pf->input_words += rounds[r]->total_width + 8 * (size_t)rounds[r]->log_h;
+    pf->inputs.resize(num_queries * pf->input_words);
+    size_t roff = 0;
+    std::vector<uint64_t> idx(num_queries);
+    for (size_t r = 0; r < nrounds; r++) {
+        const pg_tree* t = rounds[r];
+        const size_t tw = t->total_width, pw = 8 * (size_t)t->log_h;
+        for (size_t q = 0; q < num_queries; q++) idx[q] = pf->queries[q] >> (log_max - t->log_h);
+        std::vector<uint32_t> rows(num_queries * tw), paths(num_queries * pw);
+        PG_TRY(pg_mmcs_open_batch(ctx, t, idx.data(), num_queries, rows.data(), paths.data()));
+        for (size_t q = 0; q < num_queries; q++) {
+            uint32_t* dst = pf->inputs.data() + q * pf->input_words + roff;
+            if (tw) memcpy(dst, rows.data() + q * tw, tw * 4);
+            if (pw) memcpy(dst + tw, paths.data() + q * pw, pw * 4);
+        }
+        roff += tw + pw;
+    }
+    // 6. commit-phase openings
+    for (auto* t : fri->trees) pf->fri_path_words += 8 * (size_t)t->log_h;
+    pf->siblings.resize(num_queries * (size_t)pf->fri_rounds * 4);
+    pf->fri_paths.resize(num_queries * pf->fri_path_words);
+    PG_TRY(pg_fri_answer_queries(ctx, fri, pf->queries.data(), num_queries, pf->siblings.data(),
+                                 pf->fri_paths.data()));
+    *out = pf.release();
+    return PG_OK;
+}
+
+extern "C" int pg_open_proof_sizes(const pg_open_proof* p, size_t* n_values, int* fri_rounds, size_t* final_words,
+                                   size_t* num_queries, size_t* input_words_per_query,
+                                   size_t* fri_path_words_per_query) {
+    if (!p) return fail(PG_E_INVALID, "null proof");
+    if (n_values) *n_values = p->values.size();
+    if (fri_rounds) *fri_rounds = p->fri_rounds;
+    if (final_words) *final_words = p->final_poly.size();
+    if (num_queries) *num_queries = p->queries.size();
+    if (input_words_per_query) *input_words_per_query = p->input_words;
+    if (fri_path_words_per_query) *fri_path_words_per_query = p->fri_path_words;
+    return PG_OK;
+}
+
+extern "C" int pg_open_proof_read(const pg_open_proof* p, uint32_t* values, uint32_t alpha[4], uint32_t* commits,
+                                  uint32_t* betas, uint32_t* final_poly, uint32_t* pow_witness,
+                                  uint64_t* query_indices, uint32_t* input_openings, uint32_t* fri_siblings,
+                                  uint32_t* fri_paths) {
+    if (!p) return fail(PG_E_INVALID, "null proof");
+    auto put = [](void* dst, const void* src, size_t bytes) {
+        if (dst && bytes) memcpy(dst, src, bytes);
+    };
+    put(values, p->values.data(), p->values.size() * 4);
+    put(alpha, p->alpha, 16);
+    put(commits, p->commits.data(), p->commits.size() * 4);
+    put(betas, p->betas.data(), p->betas.size() * 4);
+    put(final_poly, p->final_poly.data(), p->final_poly.size() * 4);
+    if (pow_witness) *pow_witness = p->pow_witness;
+    put(query_indices, p->queries.data(), p->queries.size() * 8);
+    put(input_openings, p->inputs.data(), p->inputs.size() * 4);
+    put(fri_siblings, p->siblings.data(), p->siblings.size() * 4);
+    put(fri_paths, p->fri_paths.data(), p->fri_paths.size() * 4);
+    return PG_OK;
+}

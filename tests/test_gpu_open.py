@@ -72,42 +72,87 @@ def test_reduce_openings_match_oracle(ctx, oracle, f):
     quot.free()
 
 
-def test_open_flow_verifies(ctx, oracle):
-    """The whole open on a two-round commitment, then the verifier's checks at
-    every query: input openings verify, and the reduced opening recomputed
-    from the opened rows equals the folded vector's entry."""
+def _ext_inv(f, a):
+    p = pico_amd.PRIMES[f]
+    r, e, b = [1, 0, 0, 0], p ** 4 - 2, list(a)
+    while e:
+        if e & 1:
+            r = pyref.ext_mul(f, r, b)
+        b = pyref.ext_mul(f, b, b)
+        e >>= 1
+    assert pyref.ext_inv_by_search_check(f, a, r)
+    return r
+
+
+def _two_rounds(ctx, oracle, f):
+    p = pico_amd.PRIMES[f]
+    pcs, root_main, main, _ = _commit(ctx, oracle, f, [(9, 16), (8, 6), (9, 3)], 1, 1000)
+    qshift = pico_amd.GENERATORS[f] * pico_amd.two_adic_generator(f, 10) % p
+    _, root_q, quot, _ = _commit(ctx, oracle, f, [(9, 4)], 1, 1100, shift=qshift)
+    return pcs, (root_main, main), (root_q, quot)
+
+
+def _points(f, ch, log_trace):
+    p = pico_amd.PRIMES[f]
+    zeta = ch.sample_ext()
+    gz = (np.asarray(zeta, np.uint64) * pico_amd.two_adic_generator(f, log_trace) % p).astype(np.uint32)
+    return zeta, [np.stack([zeta, gz])] * 3, [np.asarray(zeta, np.uint32)[None, :]]
+
+
+def test_open_verifies(ctx, oracle):
+    """pg_pcs_open on a two-round commitment (a mixed-height main batch at
+    zeta and g zeta, a quotient-like batch on a shifted domain at zeta),
+    checked by a verifier written from circuit/fri.rs:36-436: transcript
+    replay (values observed, alpha, one beta per committed round, the final
+    value, the PoW witness, the query indices), every input opening against
+    its round root, each height's reduced opening recomputed from the opened
+    rows, and every query's fold chain through the commit-phase openings to
+    the final value."""
     f = KOALABEAR
     p = pico_amd.PRIMES[f]
-    lb = 1
-    pcs, root_main, main, _ = _commit(ctx, oracle, f, [(9, 16), (8, 6), (9, 3)], lb, 1000)
-    qshift = pico_amd.GENERATORS[f] * pico_amd.two_adic_generator(f, 10) % p
-    _, root_q, quot, _ = _commit(ctx, oracle, f, [(9, 4)], lb, 1100, shift=qshift)
+    pcs, (root_main, main), (root_q, quot) = _two_rounds(ctx, oracle, f)
     ch = pico_amd.DuplexChallenger(f)
     ch.observe(root_main)
     ch.observe(root_q)
-    zeta = ch.sample_ext()
-    gz = (np.asarray(zeta, np.uint64) * pico_amd.two_adic_generator(f, 9) % p).astype(np.uint32)
-    rounds = [(main, [np.stack([zeta, gz])] * 3), (quot, [np.asarray(zeta, np.uint32)[None, :]])]
-    opened, proof = pcs.open(rounds, ch, num_queries=12, pow_bits=6)
-    ro = proof["reduced_openings"]
-    log_max = max(ro)
-    ro_host = {lh: v.to_host(f) for lh, v in ro.items()}
-    alpha = [int(a) for a in proof["alpha"]]
+    zeta, main_pts, quot_pts = _points(f, ch, 9)
+    rounds = [(main, main_pts), (quot, quot_pts)]
+    nq, pow_bits = 12, 6
+    opened, proof = pcs.open(rounds, ch, num_queries=nq, pow_bits=pow_bits)
+    # transcript replay
+    vch = pico_amd.DuplexChallenger(f)
+    vch.observe(root_main)
+    vch.observe(root_q)
+    assert [int(x) for x in vch.sample_ext()] == [int(x) for x in zeta]
+    for per_round in opened:
+        for v in per_round:
+            vch.observe(np.asarray(v).ravel())
+    alpha = [int(a) for a in vch.sample_ext()]
+    assert alpha == [int(a) for a in proof["alpha"]]
+    R = len(proof["commits"])
+    for r in range(R):
+        vch.observe(proof["commits"][r])
+        assert [int(b) for b in vch.sample_ext()] == [int(b) for b in proof["betas"][r]]
+    final = proof["final_poly"]
+    assert np.all(final == final[0])  # the reference stops at 2^log_blowup with a constant vector
+    vch.observe(final[0])
+    assert vch.check_witness(pow_bits, proof["pow_witness"])
+    log_max = 10
+    assert R == log_max - 1
+    idx_all = [vch.sample_bits(log_max) for _ in range(nq)]
+    assert idx_all == proof["query_indices"]
     mm = pico_amd.MerkleTreeMmcs(ctx, f)
-    for qi, idx in enumerate(proof["query_indices"]):
-        # recompute every height's reduced opening at this query from the opened rows
-        acc = {}
-        pow_ctr = {}
-        for r, (d, mat_pts) in enumerate(rounds):
+    for qi, idx in enumerate(idx_all):
+        # input openings and the reduced opening of every height at this query
+        ro, pow_ctr = {}, {}
+        for r, (d, mat_points) in enumerate(rounds):
             rows, path = proof["input_openings"][qi][r]
             dims = [(m.cols, m.rows) for m in mm.get_matrices(d)]
-            ridx = idx >> (log_max - d.log_height)
-            assert mm.verify_batch(d.root(), dims, ridx, rows, path)
+            assert mm.verify_batch(d.root(), dims, idx >> (log_max - d.log_height), rows, path)
             for k, (w, n) in enumerate(dims):
                 lh = n.bit_length() - 1
                 i = idx >> (log_max - lh)
                 x = pyref.GEN[f] * pow(pyref.root_of_unity(f, lh), pyref.rev(i, lh), p) % p
-                for q, z in enumerate(np.asarray(mat_pts[k]).reshape(-1, 4)):
+                for q, z in enumerate(np.asarray(mat_points[k]).reshape(-1, 4)):
                     num = [0, 0, 0, 0]
                     for c in range(w):
                         ap = [1, 0, 0, 0]
@@ -118,24 +163,84 @@ def test_open_flow_verifies(ctx, oracle):
                         num = [(a + b) % p for a, b in zip(num, pyref.ext_mul(f, ap, diff))]
                         pow_ctr[lh] = pow_ctr.get(lh, 0) + 1
                     den = [(int(z[0]) - x) % p, int(z[1]), int(z[2]), int(z[3])]
-                    acc.setdefault(lh, []).append((num, den, i))
-        for lh, terms in acc.items():
-            i = terms[0][2]
-            got = [int(t) for t in ro_host[lh][i]]
-            # sum num_t / den_t == got  <=>  got * prod den == sum num_t prod_{u != t} den_u
-            lhs = got
-            for _, den, _ in terms:
-                lhs = pyref.ext_mul(f, lhs, den)
-            rhs = [0, 0, 0, 0]
-            for t, (num, _, _) in enumerate(terms):
-                term = num
-                for u, (_, den, _) in enumerate(terms):
-                    if u != t:
-                        term = pyref.ext_mul(f, term, den)
-                rhs = [(a + b) % p for a, b in zip(rhs, term)]
-            assert lhs == rhs, (qi, lh)
-    # the reference's final vector is constant after folding to 2^log_blowup
-    assert np.all(proof["final_poly"] == proof["final_poly"][0])
-    proof["fri_data"].free()
+                    term = pyref.ext_mul(f, num, _ext_inv(f, den))
+                    ro[lh] = [(a + b) % p for a, b in zip(ro.get(lh, [0, 0, 0, 0]), term)]
+        # fold chain through the commit-phase openings
+        cur = ro[log_max]
+        for r in range(R):
+            L = log_max - r
+            ir = idx >> r
+            s = [int(t) for t in proof["fri_siblings"][qi][r]]
+            pair = [cur, s] if ir % 2 == 0 else [s, cur]
+            leaf = np.array(pair[0] + pair[1], np.uint32)
+            assert mm.verify_batch(proof["commits"][r], [(8, 1 << (L - 1))], ir >> 1, [leaf],
+                                   proof["fri_paths"][qi][r]), (qi, r)
+            x = pow(pyref.root_of_unity(f, L), pyref.rev(ir >> 1, L - 1), p)
+            cur = pyref.fold_pair(f, pair[0], pair[1], [int(b) for b in proof["betas"][r]], x)
+            if L - 1 in ro:
+                cur = [(a + b) % p for a, b in zip(cur, ro[L - 1])]
+        assert cur == [int(t) for t in final[0]], qi
+    main.free()
+    quot.free()
+
+
+def _open_stepwise(pcs, rounds, ch, num_queries, pow_bits, log_final=None):
+    """The same open composed from the single-purpose entry points."""
+    ctx, f = pcs.ctx, pcs.field
+    opened = [[pcs.open_values(d, k, p) for k, p in enumerate(mp)] for d, mp in rounds]
+    for per in opened:
+        for v in per:
+            ch.observe(np.asarray(v).ravel())
+    alpha = ch.sample_ext()
+    ro = pcs.reduce_openings(rounds, opened, alpha)
+    log_stop = pcs.log_blowup if log_final is None else log_final
+    fri = pico_amd.fri_commit_phase(ctx, f, ch, ro, log_stop, final_mode=0 if log_final is None else 1)
+    for b in ro.values():
+        b.free()
+    w = pico_amd.fri_grind(ctx, f, ch, pow_bits)
+    log_max = max(ro)
+    idx = [ch.sample_bits(log_max) for _ in range(num_queries)]
+    mm = pico_amd.MerkleTreeMmcs(ctx, f)
+    inputs = [[mm.open_batch(i >> (log_max - d.log_height), d) for d, _ in rounds] for i in idx]
+    sib, paths = pico_amd.fri_answer_queries(fri, idx, log_max)
+    out = (opened, alpha, fri.commits, fri.betas, fri.final_poly, w, idx, inputs, sib, paths)
+    fri.free()
+    return out
+
+
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("log_final", [None, 3])
+def test_open_matches_stepwise(ctx, oracle, f, log_final):
+    pcs, (root_main, main), (root_q, quot) = _two_rounds(ctx, oracle, f)
+    chs = []
+    for _ in range(2):
+        ch = pico_amd.DuplexChallenger(f)
+        ch.observe(root_main)
+        ch.observe(root_q)
+        chs.append(ch)
+    _, main_pts, quot_pts = _points(f, chs[0], 9)
+    _points(f, chs[1], 9)
+    rounds = [(main, main_pts), (quot, quot_pts)]
+    opened, proof = pcs.open(rounds, chs[0], num_queries=9, pow_bits=5, log_final=log_final)
+    s_open, alpha, commits, betas, final, w, idx, inputs, sib, paths = _open_stepwise(pcs, rounds, chs[1], 9, 5,
+                                                                                      log_final)
+    for a, b in zip(opened, s_open):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    assert np.array_equal(proof["alpha"], alpha)
+    assert np.array_equal(proof["commits"], commits) and np.array_equal(proof["betas"], betas)
+    assert np.array_equal(proof["final_poly"], final) and proof["pow_witness"] == w
+    assert proof["query_indices"] == idx
+    for q in range(len(idx)):
+        for r in range(2):
+            rows_a, path_a = proof["input_openings"][q][r]
+            rows_b, path_b = inputs[q][r]
+            assert all(np.array_equal(x, y) for x, y in zip(rows_a, rows_b))
+            assert np.array_equal(path_a, path_b)
+        for r in range(len(commits)):
+            assert np.array_equal(proof["fri_paths"][q][r], paths[q][r])
+    assert np.array_equal(proof["fri_siblings"], sib)
+    # both challengers end in the same state
+    assert chs[0].sample_bits(20) == chs[1].sample_bits(20)
     main.free()
     quot.free()
