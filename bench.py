@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--cpu-leaf-rows", type=int, default=1 << 15)
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the shard+FRI, batched-jobs and served-stream sections")
+    ap.add_argument("--commit-only", action="store_true",
+                    help="skip the end-to-end and Pcs::open legs too (profiled runs)")
     ap.add_argument("--served-jobs", type=int, default=32)
     ap.add_argument("--rates", type=str, default="5,20,50,200")
     args = ap.parse_args()
@@ -242,84 +244,85 @@ def main():
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
         "root": [int(x) for x in root],
     }
-    # End to end: the trace starts in (pageable) host memory, is copied to the
-    # device and committed.  Reported beside `value`, which is device-resident.
-    host_trace = trace.to_host()
-    e2e = []
-    for _ in range(3):
-        sync()
-        t0 = time.perf_counter()
-        m = ctx.upload(host_trace)
-        t1 = time.perf_counter()
-        root_e2e, pd = pcs.commit([(dom, m)])
-        t2 = time.perf_counter()
+    if not args.commit_only:
+        # End to end: the trace starts in (pageable) host memory, is copied to the
+        # device and committed.  Reported beside `value`, which is device-resident.
+        host_trace = trace.to_host()
+        e2e = []
+        for _ in range(3):
+            sync()
+            t0 = time.perf_counter()
+            m = ctx.upload(host_trace)
+            t1 = time.perf_counter()
+            root_e2e, pd = pcs.commit([(dom, m)])
+            t2 = time.perf_counter()
+            pd.free()
+            m.free()
+            e2e.append((t1 - t0, t2 - t0))
+        h2d_s, tot_s = min(e2e, key=lambda x: x[1])
+        out["end_to_end"] = {"ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "h2d_ms": h2d_s * 1e3,
+                             "h2d_GB_per_s": host_trace.nbytes / h2d_s / 1e9, "host_memory": "pageable",
+                             "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
+        # the same with the trace in page-locked memory (pg_host_alloc)
+        pinned = ctx.host_matrix(h, w)
+        pinned.array[:] = host_trace
+        e2e = []
+        for _ in range(3):
+            sync()
+            t0 = time.perf_counter()
+            m = ctx.upload(pinned.array)
+            t1 = time.perf_counter()
+            root_e2e, pd = pcs.commit([(dom, m)])
+            t2 = time.perf_counter()
+            pd.free()
+            m.free()
+            e2e.append((t1 - t0, t2 - t0))
+        h2d_s, tot_s = min(e2e, key=lambda x: x[1])
+        out["end_to_end_pinned"] = {"ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "h2d_ms": h2d_s * 1e3,
+                                    "h2d_GB_per_s": host_trace.nbytes / h2d_s / 1e9, "host_memory": "page-locked",
+                                    "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
+        # pg_pcs_commit_host from page-locked memory: column blocks uploaded,
+        # extended and absorbed behind one another
+        e2e = []
+        for _ in range(3):
+            sync()
+            t0 = time.perf_counter()
+            root_e2e, pd = pcs.commit_host([(dom, pinned.array)], canonical=False)
+            e2e.append(time.perf_counter() - t0)
+            pd.free()
+        tot_s = min(e2e)
+        out["end_to_end_pinned_overlapped"] = {
+            "ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "host_memory": "page-locked",
+            "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
+        pinned.free()
+        progress("end-to-end legs done")
+        del host_trace
+        # Pcs::open's device part on the same commitment: values at zeta and
+        # g zeta, then the reduced opening over all 2^21 x 256 LDE cells.
+        _, pd = pcs.commit([(dom, trace)])
+        p = pico_amd.PRIMES[f]
+        zeta = np.array([123456789, 987654321, 55555, 77777], np.uint64) % p
+        pts = np.stack([zeta, zeta * pico_amd.two_adic_generator(f, args.log_h) % p])
+        alpha = np.array([31, 41, 59, 26], np.uint64)
+        for rep in range(3):
+            ctx.profile(rep == 2)
+            ctx.profile_reset()
+            t0 = time.perf_counter()
+            vals = pcs.open_values(pd, 0, pts)
+            t1 = time.perf_counter()
+            ro = pcs.reduce_openings([(pd, [pts])], [[vals]], alpha)
+            t2 = time.perf_counter()
+            for buf in ro.values():
+                buf.free()
+        ctx.profile(False)
+        red_ms = ctx.profile_get("reduce_openings")[0]
+        val_ms = ctx.profile_get("open_values")[0]
+        red_bytes = n_rows * w * 4 + n_rows * 16
+        out["pcs_open"] = {"points": 2, "open_values_ms": (t1 - t0) * 1e3, "open_values_kernel_ms": val_ms,
+                           "reduce_openings_ms": (t2 - t1) * 1e3, "reduce_kernel_ms": red_ms,
+                           "reduce_bytes": red_bytes, "reduce_GB_per_s": red_bytes / (red_ms / 1e3) / 1e9,
+                           "reduce_frac_hbm": red_bytes / (red_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
         pd.free()
-        m.free()
-        e2e.append((t1 - t0, t2 - t0))
-    h2d_s, tot_s = min(e2e, key=lambda x: x[1])
-    out["end_to_end"] = {"ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "h2d_ms": h2d_s * 1e3,
-                         "h2d_GB_per_s": host_trace.nbytes / h2d_s / 1e9, "host_memory": "pageable",
-                         "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
-    # the same with the trace in page-locked memory (pg_host_alloc)
-    pinned = ctx.host_matrix(h, w)
-    pinned.array[:] = host_trace
-    e2e = []
-    for _ in range(3):
-        sync()
-        t0 = time.perf_counter()
-        m = ctx.upload(pinned.array)
-        t1 = time.perf_counter()
-        root_e2e, pd = pcs.commit([(dom, m)])
-        t2 = time.perf_counter()
-        pd.free()
-        m.free()
-        e2e.append((t1 - t0, t2 - t0))
-    h2d_s, tot_s = min(e2e, key=lambda x: x[1])
-    out["end_to_end_pinned"] = {"ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "h2d_ms": h2d_s * 1e3,
-                                "h2d_GB_per_s": host_trace.nbytes / h2d_s / 1e9, "host_memory": "page-locked",
-                                "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
-    # pg_pcs_commit_host from page-locked memory: column blocks uploaded,
-    # extended and absorbed behind one another
-    e2e = []
-    for _ in range(3):
-        sync()
-        t0 = time.perf_counter()
-        root_e2e, pd = pcs.commit_host([(dom, pinned.array)], canonical=False)
-        e2e.append(time.perf_counter() - t0)
-        pd.free()
-    tot_s = min(e2e)
-    out["end_to_end_pinned_overlapped"] = {
-        "ms": tot_s * 1e3, "cells_per_s": cells / tot_s, "host_memory": "page-locked",
-        "root_matches": bool(all(int(a) == int(b) for a, b in zip(root_e2e, root)))}
-    pinned.free()
-    progress("end-to-end legs done")
-    del host_trace
-    # Pcs::open's device part on the same commitment: values at zeta and
-    # g zeta, then the reduced opening over all 2^21 x 256 LDE cells.
-    _, pd = pcs.commit([(dom, trace)])
-    p = pico_amd.PRIMES[f]
-    zeta = np.array([123456789, 987654321, 55555, 77777], np.uint64) % p
-    pts = np.stack([zeta, zeta * pico_amd.two_adic_generator(f, args.log_h) % p])
-    alpha = np.array([31, 41, 59, 26], np.uint64)
-    for rep in range(3):
-        ctx.profile(rep == 2)
-        ctx.profile_reset()
-        t0 = time.perf_counter()
-        vals = pcs.open_values(pd, 0, pts)
-        t1 = time.perf_counter()
-        ro = pcs.reduce_openings([(pd, [pts])], [[vals]], alpha)
-        t2 = time.perf_counter()
-        for buf in ro.values():
-            buf.free()
-    ctx.profile(False)
-    red_ms = ctx.profile_get("reduce_openings")[0]
-    val_ms = ctx.profile_get("open_values")[0]
-    red_bytes = n_rows * w * 4 + n_rows * 16
-    out["pcs_open"] = {"points": 2, "open_values_ms": (t1 - t0) * 1e3, "open_values_kernel_ms": val_ms,
-                       "reduce_openings_ms": (t2 - t1) * 1e3, "reduce_kernel_ms": red_ms,
-                       "reduce_bytes": red_bytes, "reduce_GB_per_s": red_bytes / (red_ms / 1e3) / 1e9,
-                       "reduce_frac_hbm": red_bytes / (red_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
-    pd.free()
     trace.free()
     if not args.no_workloads:
         from pico_amd import workloads
