@@ -523,7 +523,7 @@ int or_mmcs_verify(int f, const size_t *heights, const size_t *widths, size_t n,
     size_t *start = (size_t *)malloc((n + 1) * sizeof(size_t));
     start[0] = 0;
     for (size_t k = 0; k < n; k++) start[k + 1] = start[k] + widths[k];
-    uint32_t *buf = (uint32_t *)malloc((wtot + 8) * sizeof(uint32_t));
+    uint32_t *buf = (uint32_t *)calloc(wtot + 8, sizeof(uint32_t));
     size_t len = 0;
     for (size_t k = 0; k < n; k++)
         if (heights[k] == H) {
