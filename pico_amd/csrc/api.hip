@@ -1229,14 +1229,19 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
     std::shared_ptr<DevMem> tw;
     PG_TRY(get_twiddles(ctx, s, t->field, log_h, &tw));
     // about 2^20 (chunk, column) threads, at least 16 rows per chunk
+    const uint64_t col_threads = open_values_vec4(m->ptr, m->cols, (uint32_t)w, (int)std::min<size_t>(npts, 4))
+                                     ? w / 4 : w;
     uint32_t chunk_rows = 16;
-    while ((uint64_t)chunk_rows * (1u << 20) < (uint64_t)h * w) chunk_rows <<= 1;
+    while ((uint64_t)chunk_rows * (1u << 20) < (uint64_t)h * col_threads) chunk_rows <<= 1;
     const uint32_t nchunks = (uint32_t)((h + chunk_rows - 1) / chunk_rows);
     const size_t per = 4;  // points per launch
     const size_t q_max = std::min(npts, per);
-    const size_t b_w = q_max * h * 16, b_p = q_max * nchunks * w * 16, b_o = npts * w * 16;
+    // column sums in slices of about 64 chunks (at most 64 slices)
+    const uint32_t nslices = std::max<uint32_t>(1, std::min<uint32_t>(64, nchunks / 64));
+    const size_t b_w = q_max * h * 16, b_p = q_max * nchunks * w * 16, b_s = q_max * nslices * w * 16,
+                 b_o = npts * w * 16;
     int rc;
-    auto scratch = make_devmem(ctx, b_w + b_p + b_o, &rc);
+    auto scratch = make_devmem(ctx, b_w + b_p + b_s + b_o, &rc);
     if (!scratch) return rc;
     uint8_t* base = (uint8_t*)scratch->ptr;
     const uint32_t s_mont = to_monty<F>(F::GEN);
@@ -1251,9 +1256,10 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
         TIMED(kt, "open_values", s,
               launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, (const uint32_t*)tw->ptr, z, fac, nq,
                                  (Ext4*)base, (Ext4*)(base + b_w), nchunks, chunk_rows,
-                                 (Ext4*)(base + b_w + b_p + q0 * w * 16), s));
+                                 (Ext4*)(base + b_w + b_p), nslices,
+                                 (Ext4*)(base + b_w + b_p + b_s + q0 * w * 16), s));
     }
-    HIP_TRY(hipMemcpyAsync(out, base + b_w + b_p, b_o, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out, base + b_w + b_p + b_s, b_o, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return PG_OK;
 }

@@ -22,6 +22,8 @@
 
 namespace pg {
 
+bool open_values_vec4(const uint32_t* m, uint64_t ld, uint32_t w, int npts);
+
 // Ext4 values in global memory are stored as native 4-vectors (copyable
 // across address spaces); these convert at the load / store.
 __device__ __forceinline__ Ext4 ext_of(u32x4 v) { return Ext4{{v.x, v.y, v.z, v.w}}; }
@@ -54,39 +56,158 @@ __global__ void __launch_bounds__(256) column_dot_kernel(gcptr __restrict__ m, u
     if (t >= nthreads) return;
     const uint32_t col = (uint32_t)(t % w), chunk = (uint32_t)(t / w);
     const uint32_t r0 = chunk * chunk_rows, r1 = r0 + chunk_rows < rows ? r0 + chunk_rows : rows;
-    uint32_t acc[NP][4] = {};
-    for (uint32_t r = r0; r < r1; r++) {
+    LazyDot<F> acc[NP][4];
+    auto fold_all = [&]() {
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) acc[q][e].fold();
+    };
+    // Rows go in groups of UR whose loads (values and weights) are all
+    // issued before the products that use them, so each thread keeps UR
+    // memory round trips in flight instead of one.  A fold follows every
+    // third product of an accumulator.
+    constexpr int UR = 4;
+    uint32_t r = r0;
+    for (; r + UR <= r1; r += UR) {
+        uint32_t v[UR];
+        Ext4 cr[UR][NP];
+#pragma unroll
+        for (int u = 0; u < UR; u++) {
+            v[u] = m[(uint64_t)(r + u) * ld + col];
+#pragma unroll
+            for (int q = 0; q < NP; q++) cr[u][q] = ext_of(c[(uint64_t)q * rows + r + u]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UR; u++) {
+#pragma unroll
+            for (int q = 0; q < NP; q++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[q][e].add(v[u], cr[u][q].c[e]);
+            if (u % 3 == 2 || u == UR - 1) fold_all();
+        }
+    }
+    for (uint32_t k = 0; r < r1; r++, k++) {  // fewer than UR rows left (short matrices)
         const uint32_t v = m[(uint64_t)r * ld + col];
 #pragma unroll
         for (int q = 0; q < NP; q++) {
             const Ext4 cr = ext_of(c[(uint64_t)q * rows + r]);
 #pragma unroll
-            for (int e = 0; e < 4; e++) acc[q][e] = fadd<F>(acc[q][e], fmul<F>(v, cr.c[e]));
+            for (int e = 0; e < 4; e++) acc[q][e].add(v, cr.c[e]);
         }
+        if (k % 3 == 2) fold_all();
     }
 #pragma unroll
-    for (int q = 0; q < NP; q++) partial[(uint64_t)q * nthreads + t] = u32x4{acc[q][0], acc[q][1], acc[q][2], acc[q][3]};
+    for (int q = 0; q < NP; q++)
+        partial[(uint64_t)q * nthreads + t] =
+            u32x4{acc[q][0].value(), acc[q][1].value(), acc[q][2].value(), acc[q][3].value()};
 }
 
-// out[col] = factor * sum over chunks of partial[chunk][col]: one 256-thread
-// block per column, a strided pass then a tree in LDS
-template <class F>
-__global__ void __launch_bounds__(256) column_sum_kernel(gcext __restrict__ partial, uint32_t nchunks, uint32_t w,
-                                                         Ext4 factor, gext __restrict__ out) {
-    __shared__ uint32_t red[4][256];
-    const uint32_t col = blockIdx.x, t = threadIdx.x;
-    Ext4 s{{0, 0, 0, 0}};
-    for (uint32_t k = t; k < nchunks; k += 256) s = eadd<F>(s, ext_of(partial[(uint64_t)k * w + col]));
+// The same for four adjacent columns per thread (16-byte loads; each weight
+// load serves four columns), for matrices whose rows are 16-byte aligned
+// (w % 4 == 0, aligned base and pitch) and at most two points.
+template <class F, int NP>
+__global__ void __launch_bounds__(256) column_dot4_kernel(gcptr __restrict__ m, uint64_t ld, uint32_t w,
+                                                          gcext __restrict__ c, uint32_t rows,
+                                                          uint32_t chunk_rows, uint64_t nthreads4,
+                                                          uint64_t nthreads, gext __restrict__ partial) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nthreads4) return;
+    const uint32_t w4 = w / 4;
+    const uint32_t col = (uint32_t)(t % w4) * 4, chunk = (uint32_t)(t / w4);
+    const uint32_t r0 = chunk * chunk_rows, r1 = r0 + chunk_rows < rows ? r0 + chunk_rows : rows;
+    LazyDot<F> acc[NP][4][4];  // point, column, coefficient
+    auto fold_all = [&]() {
 #pragma unroll
-    for (int q = 0; q < 4; q++) red[q][t] = s.c[q];
-    __syncthreads();
-    for (uint32_t half = 128; half > 0; half >>= 1) {
-        if (t < half)
+        for (int q = 0; q < NP; q++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) red[q][t] = fadd<F>(red[q][t], red[q][t + half]);
-        __syncthreads();
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[q][j][e].fold();
+    };
+    constexpr int UR = 2;
+    uint32_t r = r0;
+    for (; r + UR <= r1; r += UR) {
+        u32x4 v[UR];
+        Ext4 cr[UR][NP];
+#pragma unroll
+        for (int u = 0; u < UR; u++) {
+            v[u] = *reinterpret_cast<const GLOBAL u32x4*>(m + (uint64_t)(r + u) * ld + col);
+#pragma unroll
+            for (int q = 0; q < NP; q++) cr[u][q] = ext_of(c[(uint64_t)q * rows + r + u]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UR; u++) {
+            const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int q = 0; q < NP; q++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) acc[q][j][e].add(vv[j], cr[u][q].c[e]);
+        }
+        fold_all();  // two products since the last fold
     }
-    if (t == 0) out[col] = v4_of(emul<F>(Ext4{{red[0][0], red[1][0], red[2][0], red[3][0]}}, factor));
+    if (r < r1) {  // one row left
+        const u32x4 v = *reinterpret_cast<const GLOBAL u32x4*>(m + (uint64_t)r * ld + col);
+        const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const Ext4 cr = ext_of(c[(uint64_t)q * rows + r]);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[q][j][e].add(vv[j], cr.c[e]);
+        }
+    }
+    const uint64_t out_t = (uint64_t)chunk * w + col;  // the scalar kernel's partial index
+#pragma unroll
+    for (int q = 0; q < NP; q++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            partial[(uint64_t)q * nthreads + out_t + j] = u32x4{acc[q][j][0].value(), acc[q][j][1].value(),
+                                                                acc[q][j][2].value(), acc[q][j][3].value()};
+}
+
+// Column sums of the partials in two stages.  Stage 1: block (x, y) adds,
+// for the 64 columns x*64 .. x*64+63, the chunks of slice y; its 256 threads
+// are 64 columns by 4 interleaved chunk lanes, so each wave reads 64
+// adjacent columns' partials (1 KB contiguous) per step; the 4 lanes meet in
+// LDS.  Stage 2: per column, the slices' sums times the barycentric factor.
+template <class F>
+__global__ void __launch_bounds__(256) column_slice_sum_kernel(gcext __restrict__ partial, uint32_t nchunks,
+                                                               uint32_t w, uint32_t nslices,
+                                                               gext __restrict__ sums) {
+    __shared__ uint32_t red[4][4][64];
+    const uint32_t cl = threadIdx.x % 64, kl = threadIdx.x / 64;
+    const uint32_t col = blockIdx.x * 64 + cl, y = blockIdx.y;
+    const uint32_t per = (nchunks + nslices - 1) / nslices;
+    const uint32_t k0 = y * per, k1 = k0 + per < nchunks ? k0 + per : nchunks;
+    Ext4 s{{0, 0, 0, 0}};
+    if (col < w)
+        for (uint32_t k = k0 + kl; k < k1; k += 4) s = eadd<F>(s, ext_of(partial[(uint64_t)k * w + col]));
+#pragma unroll
+    for (int e = 0; e < 4; e++) red[kl][e][cl] = s.c[e];
+    __syncthreads();
+    if (kl == 0 && col < w) {
+#pragma unroll
+        for (int u = 1; u < 4; u++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) s.c[e] = fadd<F>(s.c[e], red[u][e][cl]);
+        sums[(uint64_t)y * w + col] = v4_of(s);
+    }
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) column_finish_kernel(gcext __restrict__ sums, uint32_t nslices, uint32_t w,
+                                                            Ext4 factor, gext __restrict__ out) {
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= w) return;
+    Ext4 s{{0, 0, 0, 0}};
+    for (uint32_t y = 0; y < nslices; y++) s = eadd<F>(s, ext_of(sums[(uint64_t)y * w + col]));
+    out[col] = v4_of(emul<F>(s, factor));
 }
 
 // Reduced opening of one log-height: a 256-thread block owns 256 rows,
@@ -126,26 +247,44 @@ __global__ void __launch_bounds__(RO_ROWS) reduce_rows_kernel(const RoMat* __res
         const uint32_t width = mk->width;
         const uint64_t ld = mk->ld;
         const gcptr m = (gcptr)mk->data;
-        uint32_t acc[4] = {0, 0, 0, 0};
-        for (uint32_t c0 = 0; c0 < width; c0 += RO_COLS) {
-            const uint32_t col = c0 + lc;
+        LazyDot<F> acc[4];
+        uint32_t since_fold = 0;
+        // the next tile's values are loaded into registers while the current
+        // one is reduced, so a tile's HBM round trip overlaps the arithmetic
+        constexpr int LPT = RO_ROWS / (RO_ROWS / RO_COLS);  // loads per thread per tile
+        uint32_t nxt[LPT];
+        auto load_tile = [&](uint32_t c0) {
+            const uint32_t col = c0 + lc < width ? c0 + lc : width - 1;  // clamped; masked at the store
 #pragma unroll
-            for (int kk = 0; kk < RO_ROWS / (RO_ROWS / RO_COLS); kk++) {
+            for (int kk = 0; kk < LPT; kk++) {
                 const uint32_t r = lr + (RO_ROWS / RO_COLS) * kk;
                 const uint32_t row = base + r < n ? base + r : n - 1;
-                tile[r * (RO_COLS + 1) + lc] = col < width ? m[(uint64_t)row * ld + col] : 0u;
+                nxt[kk] = m[(uint64_t)row * ld + col];
             }
+        };
+        if (width) load_tile(0);
+        for (uint32_t c0 = 0; c0 < width; c0 += RO_COLS) {
+            const bool col_live = c0 + lc < width;
+#pragma unroll
+            for (int kk = 0; kk < LPT; kk++)
+                tile[(lr + (RO_ROWS / RO_COLS) * kk) * (RO_COLS + 1) + lc] = col_live ? nxt[kk] : 0u;
             __syncthreads();
+            if (c0 + RO_COLS < width) load_tile(c0 + RO_COLS);
             const uint32_t cn = width - c0 < (uint32_t)RO_COLS ? width - c0 : (uint32_t)RO_COLS;
             for (uint32_t c = 0; c < cn; c++) {
                 const uint32_t v = tile[t * (RO_COLS + 1) + c];
                 const Ext4 ap = ext_of(alpha_pows[c0 + c]);
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[q] = fadd<F>(acc[q], fmul<F>(v, ap.c[q]));
+                for (int q = 0; q < 4; q++) acc[q].add(v, ap.c[q]);
+                if (++since_fold == 3) {  // the same in every lane
+                    since_fold = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[q].fold();
+                }
             }
             __syncthreads();
         }
-        const Ext4 r = Ext4{{acc[0], acc[1], acc[2], acc[3]}};
+        const Ext4 r = Ext4{{acc[0].value(), acc[1].value(), acc[2].value(), acc[3].value()}};
         const int np = mk->npts;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
@@ -169,13 +308,22 @@ static unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 template <class F>
 static hipError_t open_values_t(const uint32_t* m, uint64_t ld, uint32_t w, int log_h, const uint32_t* tw,
                                 const Ext4* z, const Ext4* factor, int npts, Ext4* weights, Ext4* partial,
-                                uint32_t nchunks, uint32_t chunk_rows, Ext4* out, hipStream_t s) {
+                                uint32_t nchunks, uint32_t chunk_rows, Ext4* sums, uint32_t nslices, Ext4* out,
+                                hipStream_t s) {
     const uint32_t h = 1u << log_h;
     for (int q = 0; q < npts; q++)
         bary_weights_kernel<F><<<blocks_for(h, 256), 256, 0, s>>>(z[q], to_monty<F>(F::GEN), log_h, (gcptr)tw,
                                                                     (gext)(weights + (uint64_t)q * h));
     const uint64_t nt = (uint64_t)nchunks * w;
-    switch (npts) {
+    if (open_values_vec4(m, ld, w, npts)) {
+        const uint64_t nt4 = nt / 4;
+        if (npts == 1)
+            column_dot4_kernel<F, 1><<<blocks_for(nt4, 256), 256, 0, s>>>((gcptr)m, ld, w, (gcext)weights, h,
+                                                                         chunk_rows, nt4, nt, (gext)partial);
+        else
+            column_dot4_kernel<F, 2><<<blocks_for(nt4, 256), 256, 0, s>>>((gcptr)m, ld, w, (gcext)weights, h,
+                                                                         chunk_rows, nt4, nt, (gext)partial);
+    } else switch (npts) {
         case 1: column_dot_kernel<F, 1><<<blocks_for(nt, 256), 256, 0, s>>>((gcptr)m, ld, w, (gcext)weights, h,
                                                                           chunk_rows, nt, (gext)partial); break;
         case 2: column_dot_kernel<F, 2><<<blocks_for(nt, 256), 256, 0, s>>>((gcptr)m, ld, w, (gcext)weights, h,
@@ -185,23 +333,32 @@ static hipError_t open_values_t(const uint32_t* m, uint64_t ld, uint32_t w, int 
         default: column_dot_kernel<F, 4><<<blocks_for(nt, 256), 256, 0, s>>>((gcptr)m, ld, w, (gcext)weights, h,
                                                                            chunk_rows, nt, (gext)partial); break;
     }
-    for (int q = 0; q < npts; q++)
-        column_sum_kernel<F><<<w, 256, 0, s>>>((gcext)(partial + (uint64_t)q * nt), nchunks, w, factor[q],
-                                               (gext)(out + (uint64_t)q * w));
+    for (int q = 0; q < npts; q++) {
+        Ext4* sq = sums + (uint64_t)q * nslices * w;
+        column_slice_sum_kernel<F><<<dim3(blocks_for(w, 64), nslices), 256, 0, s>>>(
+            (gcext)(partial + (uint64_t)q * nt), nchunks, w, nslices, (gext)sq);
+        column_finish_kernel<F><<<blocks_for(w, 256), 256, 0, s>>>((gcext)sq, nslices, w, factor[q],
+                                                                    (gext)(out + (uint64_t)q * w));
+    }
     return hipGetLastError();
+}
+
+bool open_values_vec4(const uint32_t* m, uint64_t ld, uint32_t w, int npts) {
+    return npts <= 2 && w % 4 == 0 && ld % 4 == 0 && (uintptr_t)m % 16 == 0;
 }
 
 hipError_t launch_open_values(int field, const uint32_t* m, uint64_t ld, uint32_t w, int log_h,
                               const uint32_t* tw, const Ext4* z, const Ext4* factor, int npts, Ext4* weights,
-                              Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* out,
-                              hipStream_t stream) {
+                              Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* sums, uint32_t nslices,
+                              Ext4* out, hipStream_t stream) {
     if (w == 0 || nchunks == 0 || npts <= 0) return hipSuccess;
+    if (nslices == 0 || nslices > nchunks || nslices > 65535) return hipErrorInvalidValue;
     if (npts > OV_MAX_POINTS) return hipErrorInvalidValue;
     return field == FIELD_KOALABEAR
                ? open_values_t<KoalaBear>(m, ld, w, log_h, tw, z, factor, npts, weights, partial, nchunks,
-                                          chunk_rows, out, stream)
+                                          chunk_rows, sums, nslices, out, stream)
                : open_values_t<BabyBear>(m, ld, w, log_h, tw, z, factor, npts, weights, partial, nchunks,
-                                         chunk_rows, out, stream);
+                                         chunk_rows, sums, nslices, out, stream);
 }
 
 hipError_t launch_reduce_rows(int field, const RoMat* mats, int nmats, const Ext4* pts, int npts,
