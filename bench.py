@@ -242,6 +242,7 @@ def main():
         "roofline": roofline,
         "roofline_ntt": ntt,
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
+        "lde_tuning": os.environ.get("PICO_AMD_LDE_TUNING", os.path.relpath(pico_amd.TUNING_FILE, ROOT)),
         "root": [int(x) for x in root],
     }
     if not args.commit_only:

@@ -94,6 +94,28 @@ int pg_profile_get(pg_ctx *ctx, const char *family, double *total_ms, uint64_t *
  * the error and leave no device memory behind. */
 int pg_debug_fail_alloc(pg_ctx *ctx, long long after);
 
+/* ---- LDE tile-shape autotuning ------------------------------------------------
+ * Each LDE pass (single, A, BC, D) of a size-2^s transform runs one of up to
+ * two tile shapes (register-DFT size x threads per tile).  The choice is kept
+ * per (field, pass, s, width bucket: w <= 8, <= 32, <= 128, larger) in a
+ * process-wide table read by every LDE launch (batches fix theirs at
+ * pg_batch_create); variant 0 is the shape measured best on MI355X when the
+ * library was written.  pg_autotune times both shapes of every pass of LDEs
+ * of heights 2^log_h_min .. 2^log_h_max and the given widths (synthetic
+ * inputs, `reps` timed runs after one warm-up) and keeps the faster per
+ * entry; `report` (may be null) receives one line per decision.  Run it
+ * before other work: the table is shared by every context and thread.
+ * Export/import persist the table as text lines
+ * "field kind log_s width_bucket variant" (kind: 0 single, 1 A, 2 BC, 3 D).
+ * Results do not depend on the table: every shape computes the same LDE.
+ * No counterpart in the reference (its CPU DFT has no tile shapes). */
+int pg_autotune(pg_ctx *ctx, int field, int log_h_min, int log_h_max, const uint32_t *widths, size_t n_widths,
+                uint32_t log_blowup, int reps, char *report, size_t report_len);
+int pg_tune_set(int field, int kind, int log_s, int width_bucket, int variant);
+int pg_tune_get(int field, int kind, int log_s, uint32_t width, int *variant);
+int pg_tune_export(char *buf, size_t len, size_t *needed);
+int pg_tune_import(const char *text);
+
 /* ---- device matrices ------------------------------------------------------ */
 int pg_buf_alloc(pg_ctx *ctx, size_t rows, size_t cols, pg_buf **out);
 /* non-owning view of caller memory (e.g. a torch tensor) */

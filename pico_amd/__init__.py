@@ -10,3 +10,4 @@ from ._lib import (BABYBEAR, KOALABEAR, AdmissionError, PgError, lib)  # noqa: F
 from .core import *  # noqa: F401,F403
 
 lib()  # load now: a missing library is an import error, not a silent fallback
+load_tuning()  # the measured LDE tile-shape table (host-side; no GPU touched)  # noqa: F405

@@ -11,6 +11,7 @@ device and the ABI use happens at the boundary.  Errors raise ``PgError``
 where the reference panics.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -120,6 +121,58 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().pg_buf_wrap(self.handle, ctypes.c_void_p(device_ptr), rows, cols, ctypes.byref(h)))
         return DeviceMatrix(self, h, owned=True)  # the view object itself is freed; memory is not
+
+
+# ------------------------------------------------------------- autotune --
+LDE_PASSES = {"single": 0, "a": 1, "bc": 2, "d": 3}
+
+
+def autotune(ctx, field, log_h_min, log_h_max, widths, log_blowup=1, reps=3):
+    """Time both tile shapes of every LDE pass for these heights and widths
+    and keep the faster (process-wide); returns the decision report."""
+    w = np.ascontiguousarray(widths, dtype=np.uint32)
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().pg_autotune(ctx.handle, field, log_h_min, log_h_max, _ptr(w), w.size, log_blowup, reps, buf,
+                            len(buf)))
+    return buf.value.decode()
+
+
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune", "lde_shapes.txt")
+
+
+def load_tuning(path=None):
+    """Import a tile-shape table (default: the shipped MI355X table, or the
+    file named by PICO_AMD_LDE_TUNING; "none" keeps the built-in shapes).
+    Host-side only; returns the path used or None."""
+    path = path or os.environ.get("PICO_AMD_LDE_TUNING", TUNING_FILE)
+    if path == "none" or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tune_import(fh.read())
+    return path
+
+
+def tune_export():
+    """The tile-shape table as text ("field kind log_s width_bucket variant" lines)."""
+    n = ctypes.c_size_t()
+    check(lib().pg_tune_export(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    check(lib().pg_tune_export(buf, n.value, ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def tune_import(text):
+    check(lib().pg_tune_import(text.encode()))
+
+
+def tune_set(field, kind, log_s, width_bucket, variant):
+    check(lib().pg_tune_set(field, kind, log_s, width_bucket, variant))
+
+
+def tune_get(field, kind, log_s, width):
+    v = ctypes.c_int()
+    check(lib().pg_tune_get(field, kind, log_s, width, ctypes.byref(v)))
+    return v.value
 
 
 class PinnedMatrix:
@@ -748,5 +801,7 @@ __all__ = [
     "Radix2DitParallel", "MerkleTreeMmcs", "ProverData", "TwoAdicFriPcs",
     "TwoAdicMultiplicativeCoset", "DuplexChallenger", "fri_fold", "fri_commit_phase",
     "fri_grind", "fri_answer_queries", "to_monty", "from_monty", "two_adic_generator", "PgError",
-    "device_synchronize", "CommitBatch", "CommitService",
+    "device_synchronize", "CommitBatch", "CommitService", "PinnedMatrix",
+    "autotune", "tune_export", "tune_import", "tune_set", "tune_get", "LDE_PASSES", "load_tuning",
+    "TUNING_FILE",
 ]

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -1538,7 +1539,7 @@ extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* eval
     // descriptors
     Staging st;
     struct PassRec {
-        int kind, log_s, njobs, max_b;
+        int kind, log_s, variant, njobs, max_b;
         size_t off_args, off_prefix;
         uint64_t total;
     };
@@ -1583,13 +1584,16 @@ extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* eval
         for (int i = 0; i < (Lg <= 11 ? 1 : 3); i++) {
             const int kind = Lg <= 11 ? KIND_SINGLE : kinds3[i];
             const int log_s = (kind == KIND_BC || kind == KIND_SINGLE) ? k2 : k1;
-            const int C = lde_pass_cols(kind, log_s);
+            uint32_t wmax = 0;  // one tile shape per launch: the widest job's
+            for (auto& a : args) wmax = std::max(wmax, a.w);
+            const int variant = lde_variant(field, kind, log_s, wmax);
+            const int C = lde_pass_cols(kind, log_s, variant);
             std::vector<uint64_t> prefix(1, 0);
             for (auto& a : args)
                 prefix.push_back(prefix.back() + lde_pass_row_tiles(kind, Lg, a.b) * ((a.w + C - 1) / C));
             int max_b = 0;
             for (auto& a : args) max_b = std::max(max_b, a.b);
-            PassRec pr{kind, log_s, (int)args.size(), max_b, st.put(args.data(), args.size() * sizeof(LdeArgs)),
+            PassRec pr{kind, log_s, variant, (int)args.size(), max_b, st.put(args.data(), args.size() * sizeof(LdeArgs)),
                        st.put(prefix.data(), prefix.size() * 8), prefix.back()};
             passes.push_back(pr);
         }
@@ -1645,7 +1649,8 @@ extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* eval
         const LdeArgs* jobs = (const LdeArgs*)(dd + pr.off_args);
         const uint64_t* pf = (const uint64_t*)(dd + pr.off_prefix);
         b->launches.push_back([=](hipStream_t s) {
-            return launch_lde_pass_batch(field, pr.kind, pr.log_s, jobs, pf, pr.njobs, pr.total, pr.max_b, s);
+            return launch_lde_pass_batch(field, pr.kind, pr.log_s, pr.variant, jobs, pf, pr.njobs, pr.total, pr.max_b,
+                                         s);
         });
     }
     {
@@ -2389,5 +2394,188 @@ extern "C" int pg_open_proof_read(const pg_open_proof* p, uint32_t* values, uint
     put(input_openings, p->inputs.data(), p->inputs.size() * 4);
     put(fri_siblings, p->siblings.data(), p->siblings.size() * 4);
     put(fri_paths, p->fri_paths.data(), p->fri_paths.size() * 4);
+    return PG_OK;
+}
+
+// ------------------------------------------------------------- autotune --
+// Per-pass durations of one LDE, by pass name (HIP events around each launch).
+struct PassTimer : KernelTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    std::string cur;
+    std::map<std::string, double> ms;
+    PassTimer() {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+    }
+    ~PassTimer() override {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    void begin(const char* name, hipStream_t s) override {
+        cur = name;
+        (void)hipEventRecord(a, s);
+    }
+    void end(hipStream_t s) override {
+        (void)hipEventRecord(b, s);
+        if (hipEventSynchronize(b) != hipSuccess) return;
+        float t = 0;
+        if (hipEventElapsedTime(&t, a, b) == hipSuccess) ms[cur] += t;
+    }
+};
+
+extern "C" int pg_autotune(pg_ctx* ctx, int field, int log_h_min, int log_h_max, const uint32_t* widths, size_t nw,
+                           uint32_t log_blowup, int reps, char* report, size_t report_len) {
+    if (!ctx || !field_ok(field) || !widths || nw == 0 || log_h_min < 1 || log_h_max < log_h_min || reps < 1)
+        return fail(PG_E_INVALID, "bad argument");
+    for (size_t i = 0; i < nw; i++)
+        if (widths[i] == 0) return fail(PG_E_INVALID, "width zero");
+    if (log_h_max > lde_max_log_h()) return fail(PG_E_UNSUPPORTED, "log height %d > %d", log_h_max, lde_max_log_h());
+    if (log_h_max + (int)log_blowup > two_adicity(field) || log_blowup > 8)
+        return fail(PG_E_INVALID, "LDE beyond the two-adic subgroup");
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    static const char* const names[4] = {"lde_single", "lde_pass_a", "lde_pass_bc", "lde_pass_d"};
+    // total ms per (kind, log_s, bucket, variant) over every tuned shape
+    std::map<std::tuple<int, int, int>, double> total[2];
+    for (int L = log_h_min; L <= log_h_max; L++)
+        for (size_t wi = 0; wi < nw; wi++) {
+            const uint32_t w = widths[wi];
+            const int bucket = lde_width_bucket(w);
+            int k1, k2;
+            lde_split(L, &k1, &k2);
+            std::vector<std::pair<int, int>> passes;  // (kind, log_s)
+            if (L <= 11)
+                passes = {{KIND_SINGLE, L}};
+            else
+                passes = {{KIND_A, k1}, {KIND_BC, k2}, {KIND_D, k1}};
+            pg_buf* in = nullptr;
+            int rc = buf_new(ctx, (size_t)1 << L, w, &in);
+            if (rc != PG_OK) return rc;
+            std::unique_ptr<pg_buf, void (*)(pg_buf*)> in_guard(in, pg_buf_free);
+            HIP_TRY(launch_fill_random(field, 0x7475 + (uint64_t)L * 4096 + w, in->ptr, in->rows * in->cols, 1, l.s));
+            std::shared_ptr<DevMem> scratch;
+            for (int v = 0; v < 2; v++) {
+                // every pass of this LDE in variant v where it has one
+                std::vector<int> saved;
+                for (auto& ps : passes) {
+                    saved.push_back(lde_variant(field, ps.first, ps.second, w));
+                    lde_set_variant(field, ps.first, ps.second, bucket, v < lde_num_variants(ps.first, ps.second) ? v : 0);
+                }
+                PassTimer pt;
+                for (int r = 0; r <= reps; r++) {  // r = 0 warms up
+                    pg_buf* o = nullptr;
+                    pt.ms.clear();
+                    const uint32_t shift = gen_monty(field);
+                    int L2;
+                    rc = lde_check(field, in->rows, in->cols, log_blowup, &L2);
+                    std::shared_ptr<DevMem> tw, sc;
+                    if (rc == PG_OK) rc = get_twiddles(ctx, l.s, field, L, &tw);
+                    if (rc == PG_OK) rc = get_scale(ctx, l.s, field, L, (int)log_blowup, shift, &sc);
+                    if (rc == PG_OK) rc = buf_new(ctx, in->rows << log_blowup, w, &o);
+                    if (rc == PG_OK && L > 11 && (!scratch || scratch->bytes < o->bytes)) {
+                        scratch.reset();
+                        scratch = make_devmem(ctx, o->bytes, &rc);
+                    }
+                    hipError_t e = hipSuccess;
+                    if (rc == PG_OK)
+                        e = launch_coset_lde(field, in->ptr, w, L, w, (int)log_blowup, (const uint32_t*)tw->ptr,
+                                             (const uint32_t*)sc->ptr, L > 11 ? (uint32_t*)scratch->ptr : nullptr,
+                                             o->ptr, w, 1, l.s, &pt);
+                    if (e == hipSuccess && rc == PG_OK) e = hipStreamSynchronize(l.s);
+                    pg_buf_free(o);
+                    if (rc == PG_OK && e != hipSuccess) rc = fail(PG_E_DEVICE, "autotune LDE: %s", hipGetErrorString(e));
+                    if (rc != PG_OK) {
+                        for (size_t i = 0; i < passes.size(); i++)
+                            lde_set_variant(field, passes[i].first, passes[i].second, bucket, saved[i]);
+                        return rc;
+                    }
+                    if (r == 0) continue;
+                    for (auto& ps : passes)
+                        if (v < lde_num_variants(ps.first, ps.second))
+                            total[v][std::make_tuple(ps.first, ps.second, bucket)] += pt.ms[names[ps.first]];
+                }
+                for (size_t i = 0; i < passes.size(); i++)
+                    lde_set_variant(field, passes[i].first, passes[i].second, bucket, saved[i]);
+            }
+        }
+    // the faster variant wherever both were timed
+    std::string rep = "# field kind log_s width_bucket ms_default ms_alternative -> variant\n";
+    for (auto& kv : total[1]) {
+        auto it = total[0].find(kv.first);
+        if (it == total[0].end()) continue;
+        const int kind = std::get<0>(kv.first), log_s = std::get<1>(kv.first), bucket = std::get<2>(kv.first);
+        const int choice = kv.second < it->second ? 1 : 0;
+        lde_set_variant(field, kind, log_s, bucket, choice);
+        char line[160];
+        snprintf(line, sizeof(line), "%d %s %d %d %.4f %.4f -> %d\n", field, names[kind], log_s, bucket,
+                 it->second / reps, kv.second / reps, choice);
+        rep += line;
+    }
+    if (report && report_len) {
+        strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_tune_set(int field, int kind, int log_s, int width_bucket, int variant) {
+    if (lde_set_variant(field, kind, log_s, width_bucket, variant) != 0)
+        return fail(PG_E_INVALID, "no tile-shape variant %d for field %d kind %d log size %d bucket %d", variant, field,
+                    kind, log_s, width_bucket);
+    return PG_OK;
+}
+
+extern "C" int pg_tune_get(int field, int kind, int log_s, uint32_t width, int* variant) {
+    if (!variant || !field_ok(field) || kind < 0 || kind > 3 || log_s < 0 || log_s > 11)
+        return fail(PG_E_INVALID, "bad argument");
+    *variant = lde_variant(field, kind, log_s, width);
+    return PG_OK;
+}
+
+extern "C" int pg_tune_export(char* buf, size_t len, size_t* needed) {
+    std::string s;
+    for (int f = 0; f < 2; f++)
+        for (int k = 0; k < 4; k++)
+            for (int ls = 0; ls < 12; ls++) {
+                if (lde_num_variants(k, ls) < 2) continue;
+                for (int b = 0; b < LDE_WIDTH_BUCKETS; b++) {
+                    char line[64];
+                    snprintf(line, sizeof(line), "%d %d %d %d %d\n", f, k, ls, b,
+                             lde_variant(f, k, ls, b == 0 ? 1u : b == 1 ? 9u : b == 2 ? 33u : 129u));
+                    s += line;
+                }
+            }
+    if (needed) *needed = s.size() + 1;
+    if (buf && len) {
+        if (len < s.size() + 1) return fail(PG_E_INVALID, "buffer of %zu bytes, %zu needed", len, s.size() + 1);
+        memcpy(buf, s.c_str(), s.size() + 1);
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_tune_import(const char* text) {
+    if (!text) return fail(PG_E_INVALID, "null text");
+    // validate every line before applying any
+    std::vector<std::array<int, 5>> rows;
+    const char* p = text;
+    int lineno = 0;
+    while (*p) {
+        const char* e = strchr(p, '\n');
+        std::string line(p, e ? (size_t)(e - p) : strlen(p));
+        p = e ? e + 1 : p + line.size();
+        lineno++;
+        const size_t c = line.find('#');
+        if (c != std::string::npos) line.resize(c);
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::array<int, 5> v{};
+        char tail = 0;
+        if (sscanf(line.c_str(), "%d %d %d %d %d %c", &v[0], &v[1], &v[2], &v[3], &v[4], &tail) != 5)
+            return fail(PG_E_INVALID, "line %d: expected 'field kind log_s bucket variant'", lineno);
+        if (v[0] < 0 || v[0] > 1 || v[1] < 0 || v[1] > 3 || v[2] < 0 || v[2] > 11 || v[3] < 0 ||
+            v[3] >= LDE_WIDTH_BUCKETS || v[4] < 0 || v[4] >= lde_num_variants(v[1], v[2]))
+            return fail(PG_E_INVALID, "line %d: no such entry or variant", lineno);
+        rows.push_back(v);
+    }
+    for (auto& v : rows) lde_set_variant(v[0], v[1], v[2], v[3], v[4]);
     return PG_OK;
 }

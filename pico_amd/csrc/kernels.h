@@ -100,14 +100,22 @@ enum LdeKind { KIND_SINGLE = 0, KIND_A = 1, KIND_BC = 2, KIND_D = 3 };
 // Split of a size-2^L transform: L <= 11 is one pass (k1 = 0, k2 = L);
 // otherwise k1 = ceil(L/2) strided bits, k2 = L - k1 block bits.
 void lde_split(int L, int* k1, int* k2);
-// Columns per tile of the kernel that runs pass `kind` with tile size 2^log_s.
-int lde_pass_cols(int kind, int log_s);
+// Columns per tile of the kernel that runs pass `kind` with tile size 2^log_s
+// in tile-shape variant `variant`.
+int lde_pass_cols(int kind, int log_s, int variant);
+// Tile-shape variants: 1 or 2 per (pass, log size); the choice is kept per
+// (field, pass, log size, width bucket) and read by every LDE launch.
+constexpr int LDE_WIDTH_BUCKETS = 4;  // widths <= 8, <= 32, <= 128, larger
+int lde_width_bucket(uint32_t w);
+int lde_num_variants(int kind, int log_s);
+int lde_variant(int field, int kind, int log_s, uint32_t w);
+int lde_set_variant(int field, int kind, int log_s, int bucket, int variant);
 // Row tiles (before the column-block factor) of pass `kind` for one job.
 uint64_t lde_pass_row_tiles(int kind, int L, int b);
 // One pass over a device table of jobs (all with the same L); prefix holds
 // each job's first tile (row tiles x column blocks), prefix[njobs] = total;
 // max_b = the largest added_bits among the jobs.
-hipError_t launch_lde_pass_batch(int field, int kind, int log_s, const LdeArgs* jobs,
+hipError_t launch_lde_pass_batch(int field, int kind, int log_s, int variant, const LdeArgs* jobs,
                                  const uint64_t* prefix, int njobs, uint64_t total_tiles, int max_b,
                                  hipStream_t stream, KernelTimer* kt = nullptr);
 // tw[e] = w_{2^log_n}^e, e < 2^log_n (Montgomery)

@@ -32,6 +32,8 @@
 // thread still holds only E values (tile_dif3 / tile_dit3).  Lanes map to
 // adjacent columns so every global access is a run of 8-16 consecutive
 // words of one row.
+#include <atomic>
+
 #include "kernels.h"
 #include "field.h"
 
@@ -359,6 +361,18 @@ constexpr int shape_log_e(int log_s, int kind) {
            : log_s == 10 ? 4
            : (kind == KIND_BC || kind == KIND_SINGLE) ? 4 : 6;
 }
+// A second tile shape per transform size (-1: none), for the autotuner:
+// 2^7 8x16 three-level, 2^8 32x8, 2^9 16x32 three-level, 2^10 32x32, 2^11
+// 64x32 for the coset-looping passes and 16x128 three-level for the strided
+// ones (each the other side of the measured default).
+constexpr int shape_alt_log_e(int log_s, int kind) {
+    return log_s == 7 ? 3
+           : log_s == 8 ? 5
+           : log_s == 9 ? 4
+           : log_s == 10 ? 5
+           : log_s == 11 ? ((kind == KIND_BC || kind == KIND_SINGLE) ? 6 : 4)
+                         : -1;
+}
 
 // Columns per tile: 16 (64-byte row runs) up to 1024 threads per workgroup;
 // 8 for the coset-looping passes' three-level tiles (two 512-thread
@@ -642,9 +656,21 @@ static void run(const PassLaunch& pl, hipStream_t s, KernelTimer* kt) {
     if (kt) kt->end(s);
 }
 
-// tile shapes (LOG_E, LOG_P) per log S from shape_log_e
+// tile shape (LOG_E, LOG_P) for size 2^LS: the default, or the alternative
+template <class F, int KIND, int LS>
+static void run_ls(int variant, const PassLaunch& a, hipStream_t s, KernelTimer* kt) {
+    constexpr int E0 = shape_log_e(LS, KIND), E1 = shape_alt_log_e(LS, KIND);
+    if constexpr (E1 >= 0) {
+        if (variant) {
+            run<F, E1, LS - E1, KIND>(a, s, kt);
+            return;
+        }
+    }
+    run<F, E0, LS - E0, KIND>(a, s, kt);
+}
+
 template <class F, int KIND>
-static bool run_logs(int log_s, const PassLaunch& a, hipStream_t s, KernelTimer* kt) {
+static bool run_logs(int log_s, int variant, const PassLaunch& a, hipStream_t s, KernelTimer* kt) {
     switch (log_s) {
         case 0: if constexpr (KIND == KIND_SINGLE) { run<F, 0, 0, KIND>(a, s, kt); return true; } break;
         case 1: if constexpr (KIND == KIND_SINGLE) { run<F, 1, 0, KIND>(a, s, kt); return true; } break;
@@ -652,12 +678,12 @@ static bool run_logs(int log_s, const PassLaunch& a, hipStream_t s, KernelTimer*
         case 3: if constexpr (KIND == KIND_SINGLE) { run<F, 3, 0, KIND>(a, s, kt); return true; } break;
         case 4: if constexpr (KIND == KIND_SINGLE) { run<F, 4, 0, KIND>(a, s, kt); return true; } break;
         case 5: if constexpr (KIND == KIND_SINGLE) { run<F, 5, 0, KIND>(a, s, kt); return true; } break;
-        case 6: run<F, shape_log_e(6, KIND), 6 - shape_log_e(6, KIND), KIND>(a, s, kt); return true;
-        case 7: run<F, shape_log_e(7, KIND), 7 - shape_log_e(7, KIND), KIND>(a, s, kt); return true;
-        case 8: run<F, shape_log_e(8, KIND), 8 - shape_log_e(8, KIND), KIND>(a, s, kt); return true;
-        case 9: run<F, shape_log_e(9, KIND), 9 - shape_log_e(9, KIND), KIND>(a, s, kt); return true;
-        case 10: run<F, shape_log_e(10, KIND), 10 - shape_log_e(10, KIND), KIND>(a, s, kt); return true;
-        case 11: run<F, shape_log_e(11, KIND), 11 - shape_log_e(11, KIND), KIND>(a, s, kt); return true;
+        case 6: run_ls<F, KIND, 6>(variant, a, s, kt); return true;
+        case 7: run_ls<F, KIND, 7>(variant, a, s, kt); return true;
+        case 8: run_ls<F, KIND, 8>(variant, a, s, kt); return true;
+        case 9: run_ls<F, KIND, 9>(variant, a, s, kt); return true;
+        case 10: run_ls<F, KIND, 10>(variant, a, s, kt); return true;
+        case 11: run_ls<F, KIND, 11>(variant, a, s, kt); return true;
         default: break;
     }
     return false;
@@ -668,8 +694,29 @@ void lde_split(int L, int* k1, int* k2) {
     *k2 = L - *k1;
 }
 
-int lde_pass_cols(int kind, int log_s) {
-    return cols_for(shape_log_e(log_s, kind), log_s - shape_log_e(log_s, kind), kind);
+int lde_pass_cols(int kind, int log_s, int variant) {
+    const int e = variant && shape_alt_log_e(log_s, kind) >= 0 ? shape_alt_log_e(log_s, kind) : shape_log_e(log_s, kind);
+    return cols_for(e, log_s - e, kind);
+}
+
+// Tile-shape choice per (field, pass, log size, width bucket): 0 = the
+// default shape, 1 = the alternative.  Process-wide; set by the autotuner.
+static std::atomic<uint8_t> g_variant[2][4][12][LDE_WIDTH_BUCKETS];
+
+int lde_width_bucket(uint32_t w) { return w <= 8 ? 0 : w <= 32 ? 1 : w <= 128 ? 2 : 3; }
+int lde_num_variants(int kind, int log_s) {
+    return log_s >= 0 && log_s < 12 && shape_alt_log_e(log_s, kind) >= 0 ? 2 : 1;
+}
+int lde_variant(int field, int kind, int log_s, uint32_t w) {
+    if (field < 0 || field > 1 || kind < 0 || kind > 3 || log_s < 0 || log_s >= 12) return 0;
+    return g_variant[field][kind][log_s][lde_width_bucket(w)].load(std::memory_order_relaxed);
+}
+int lde_set_variant(int field, int kind, int log_s, int bucket, int variant) {
+    if (field < 0 || field > 1 || kind < 0 || kind > 3 || log_s < 0 || log_s >= 12 || bucket < 0 ||
+        bucket >= LDE_WIDTH_BUCKETS || variant < 0 || variant >= lde_num_variants(kind, log_s))
+        return -1;
+    g_variant[field][kind][log_s][bucket].store((uint8_t)variant, std::memory_order_relaxed);
+    return 0;
 }
 
 uint64_t lde_pass_row_tiles(int kind, int L, int b) {
@@ -684,23 +731,24 @@ uint64_t lde_pass_row_tiles(int kind, int L, int b) {
 }
 
 template <class F>
-static bool pass_t(int kind, int log_s, const PassLaunch& pl, hipStream_t s, KernelTimer* kt) {
+static bool pass_t(int kind, int log_s, int variant, const PassLaunch& pl, hipStream_t s, KernelTimer* kt) {
     switch (kind) {
-        case KIND_SINGLE: return run_logs<F, KIND_SINGLE>(log_s, pl, s, kt);
-        case KIND_A: return run_logs<F, KIND_A>(log_s, pl, s, kt);
-        case KIND_BC: return run_logs<F, KIND_BC>(log_s, pl, s, kt);
-        default: return run_logs<F, KIND_D>(log_s, pl, s, kt);
+        case KIND_SINGLE: return run_logs<F, KIND_SINGLE>(log_s, variant, pl, s, kt);
+        case KIND_A: return run_logs<F, KIND_A>(log_s, variant, pl, s, kt);
+        case KIND_BC: return run_logs<F, KIND_BC>(log_s, variant, pl, s, kt);
+        default: return run_logs<F, KIND_D>(log_s, variant, pl, s, kt);
     }
 }
 
-hipError_t launch_lde_pass_batch(int field, int kind, int log_s, const LdeArgs* jobs, const uint64_t* prefix,
-                                 int njobs, uint64_t total_tiles, int max_b, hipStream_t s, KernelTimer* kt) {
+hipError_t launch_lde_pass_batch(int field, int kind, int log_s, int variant, const LdeArgs* jobs,
+                                 const uint64_t* prefix, int njobs, uint64_t total_tiles, int max_b, hipStream_t s,
+                                 KernelTimer* kt) {
     if (njobs <= 0) return hipErrorInvalidValue;
     LdeArgs a0{};
     a0.b = max_b;  // coset-loop bound shared by every tile group
     const PassLaunch pl{a0, jobs, prefix, njobs, total_tiles};
-    const bool ok = field == FIELD_KOALABEAR ? pass_t<KoalaBear>(kind, log_s, pl, s, kt)
-                                             : pass_t<BabyBear>(kind, log_s, pl, s, kt);
+    const bool ok = field == FIELD_KOALABEAR ? pass_t<KoalaBear>(kind, log_s, variant, pl, s, kt)
+                                             : pass_t<BabyBear>(kind, log_s, variant, pl, s, kt);
     return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
@@ -727,7 +775,8 @@ static hipError_t coset_lde_t(const uint32_t* in, size_t ld_in, int L, uint32_t 
         a.ntiles = lde_pass_row_tiles(kind, L, b);
         const int log_s = kind == KIND_BC || kind == KIND_SINGLE ? a.k2 : a.k1;
         const PassLaunch pl{a, nullptr, nullptr, 0, 0};
-        if (!pass_t<F>(kind, log_s, pl, s, kt)) return hipErrorInvalidValue;
+        const int variant = lde_variant(F::P == KoalaBear::P ? FIELD_KOALABEAR : FIELD_BABYBEAR, kind, log_s, w);
+        if (!pass_t<F>(kind, log_s, variant, pl, s, kt)) return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
