@@ -146,10 +146,13 @@ size_t pg_buf_ld(const pg_buf *buf);
  * side holds canonical values of `field`, converted on the device. */
 int pg_upload(pg_ctx *ctx, pg_buf *buf, const uint32_t *host, int field, int canonical);
 /* Page-locked host memory (hipHostMalloc) for traces produced on the host:
- * pg_upload from it runs at the link's rate, from pageable memory it goes
- * through the runtime's staging copies (measured on MI355X, 1 GiB:
- * 58 GB/s page-locked, 29-32 GB/s pageable).  Trace generation writes rows straight into it; no function
- * here registers memory that already exists. */
+ * uploads from it are DMA'd directly (measured on MI355X, 1 GiB: 57.5 GB/s).
+ * Uploads of 4 MiB or more from pageable memory go through the library's
+ * page-locked double buffers, filled 16 MiB at a time by a pool of host
+ * threads while the previous chunk's DMA runs (54 GB/s, against 26-30 GB/s
+ * through the runtime's own staging).  Trace generation writes rows
+ * straight into this memory; no function here registers memory that
+ * already exists. */
 int pg_host_alloc(pg_ctx *ctx, size_t bytes, void **out);
 void pg_host_free(void *p);
 int pg_download(pg_ctx *ctx, const pg_buf *buf, size_t row0, size_t nrows, uint32_t *host,
