@@ -427,6 +427,51 @@ static bool host_is_locked(const void* p) {
     return false;
 }
 
+// `bytes` contiguous bytes from device memory src to pageable host memory
+// dst through a stage: the DMA of chunk k+1 runs while the host threads copy
+// chunk k out of the other buffer.  Work queued on s before the call is
+// ordered before the copies; returns when dst holds everything.
+static int stage_download(pg_ctx* ctx, hipStream_t s, void* dst, const void* src, size_t bytes) {
+    Stage* st;
+    PG_TRY(stage_get(ctx, &st));
+    HostPool& pool = host_pool();
+    const int nt = pool.size();
+    const size_t nchunks = (bytes + Stage::CHUNK - 1) / Stage::CHUNK;
+    hipError_t e = hipSuccess;
+    auto drain = [&](size_t k) {  // chunk k's DMA is done: copy it out
+        const size_t off = k * Stage::CHUNK, n = std::min(Stage::CHUNK, bytes - off);
+        const char* from = (const char*)st->buf[k & 1];
+        char* to = (char*)dst + off;
+        const size_t per = (n + nt - 1) / nt;
+        pool.run(nt, [&](int i) {
+            const size_t a = (size_t)i * per, z = std::min(n, a + per);
+            if (a < z) memcpy(to + a, from + a, z - a);
+        });
+    };
+    for (size_t k = 0; k < nchunks && e == hipSuccess; k++) {
+        const size_t off = k * Stage::CHUNK, n = std::min(Stage::CHUNK, bytes - off);
+        // chunk k goes into buffer k & 1, whose previous chunk (k - 2) was
+        // drained in the previous iteration
+        e = hipMemcpyAsync(st->buf[k & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(st->ev[k & 1], s);
+        if (e == hipSuccess && k > 0) {
+            e = hipEventSynchronize(st->ev[(k - 1) & 1]);
+            if (e == hipSuccess) drain(k - 1);
+        }
+    }
+    if (e == hipSuccess && nchunks) {
+        e = hipEventSynchronize(st->ev[(nchunks - 1) & 1]);
+        if (e == hipSuccess) drain(nchunks - 1);
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        stage_destroy(st);
+        return fail(PG_E_DEVICE, "staged download: %s", hipGetErrorString(e));
+    }
+    stage_put(ctx, st);
+    return PG_OK;
+}
+
 // `rows` rows of `width` bytes from src (pitch spitch, pageable host memory)
 // to dst (device, pitch dpitch), queued on s through a stage: whole chunks
 // when both sides are contiguous, else groups of rows gathered on the host
@@ -763,16 +808,28 @@ extern "C" int pg_download(pg_ctx* ctx, const pg_buf* b, size_t row0, size_t nro
         return b->ld == b->cols ? hipMemcpyAsync(dst, src, n * 4, kind, l.s)
                                 : hipMemcpy2DAsync(dst, b->cols * 4, src, b->ld * 4, b->cols * 4, nrows, kind, l.s);
     };
+    const bool staged = n * 4 >= STAGE_MIN_BYTES && !host_is_locked(host);
     if (canonical) {
         if (!field_ok(field)) return fail(PG_E_INVALID, "unknown field %d", field);
         void* t;
         PG_TRY(ctx_alloc(ctx, n * 4, &t));
         hipError_t e = copy_rows(t, hipMemcpyDeviceToDevice);
         if (e == hipSuccess) e = launch_convert(field, (uint32_t*)t, n, 0, l.s);
-        if (e == hipSuccess) e = hipMemcpyAsync(host, t, n * 4, hipMemcpyDeviceToHost, l.s);
+        int rc = PG_OK;
+        if (e == hipSuccess) {
+            if (staged)
+                rc = stage_download(ctx, l.s, host, t, n * 4);
+            else
+                e = hipMemcpyAsync(host, t, n * 4, hipMemcpyDeviceToHost, l.s);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(l.s);
         ctx_free(ctx, t, n * 4);
+        if (rc != PG_OK) return rc;
         HIP_TRY(e);
+        return PG_OK;
+    }
+    if (staged && b->ld == b->cols) {
+        PG_TRY(stage_download(ctx, l.s, host, src, n * 4));
         return PG_OK;
     }
     HIP_TRY(copy_rows(host, hipMemcpyDeviceToHost));
@@ -1359,12 +1416,35 @@ extern "C" int pg_pcs_evaluations(pg_ctx* ctx, const pg_tree* t, size_t k, size_
     if (!ctx || !t || !host || k >= t->mats.size()) return fail(PG_E_INVALID, "bad argument");
     const pg_buf* m = t->mats[k];
     if (!is_pow2(rows) || rows > m->rows) return fail(PG_E_INVALID, "domain size %zu invalid", rows);
-    std::vector<uint32_t> tmp(rows * m->cols);
-    PG_TRY(pg_download(ctx, m, 0, rows, tmp.data(), t->field, 0));
-    const int q = log2_of(rows);
-    for (size_t j = 0; j < rows; j++)
-        memcpy(host + j * m->cols, tmp.data() + (size_t)bit_rev((uint32_t)j, q) * m->cols, m->cols * 4);
-    return PG_OK;
+    // natural order gathered on the device a block of rows at a time (at most
+    // 256 MiB of scratch), each block downloaded as one contiguous run
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    const size_t w = m->cols, q = (size_t)log2_of(rows);
+    if (w == 0 || rows == 0) return PG_OK;
+    const size_t block = std::max<size_t>(1, std::min(rows, (size_t)(256u << 20) / (w * 4)));
+    void* tmp;
+    PG_TRY(ctx_alloc(ctx, block * w * 4, &tmp));
+    int rc = PG_OK;
+    for (size_t r0 = 0; r0 < rows && rc == PG_OK; r0 += block) {
+        const size_t nr = std::min(block, rows - r0);
+        hipError_t e = launch_rows_bitrev(m->ptr, m->ld, (uint32_t)w, (int)q, r0, nr, (uint32_t*)tmp, l.s);
+        if (e != hipSuccess) {
+            rc = fail(PG_E_DEVICE, "row gather: %s", hipGetErrorString(e));
+            break;
+        }
+        uint32_t* dst = host + r0 * w;
+        if (nr * w * 4 >= STAGE_MIN_BYTES && !host_is_locked(host)) {
+            rc = stage_download(ctx, l.s, dst, tmp, nr * w * 4);
+        } else {
+            e = hipMemcpyAsync(dst, tmp, nr * w * 4, hipMemcpyDeviceToHost, l.s);
+            if (e == hipSuccess) e = hipStreamSynchronize(l.s);
+            if (e != hipSuccess) rc = fail(PG_E_DEVICE, "download: %s", hipGetErrorString(e));
+        }
+    }
+    (void)hipStreamSynchronize(l.s);
+    ctx_free(ctx, tmp, block * w * 4);
+    return rc;
 }
 
 // ------------------------------------------------------------ PCS open --

@@ -113,6 +113,26 @@ hipError_t launch_fri_fold(int field, const uint32_t* v, size_t n_out, const uin
     return hipGetLastError();
 }
 
+// dst row r (r < nrows) = src row rev_q(r0 + r), w words per row: a run of
+// natural-order rows of a matrix stored in bit-reversed row order.  One
+// thread per word, adjacent threads on adjacent columns (coalesced).
+__global__ void rows_bitrev_kernel(const uint32_t* __restrict__ src, uint64_t ld, uint32_t w, int q, uint64_t r0,
+                                   uint64_t nrows, uint32_t* __restrict__ dst) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows * w) return;
+    const uint64_t r = t / w, c = t % w;
+    const uint64_t j = r0 + r;
+    const uint64_t sr = q ? (uint64_t)(__brevll(j) >> (64 - q)) : 0;
+    dst[t] = ((gcptr)src)[sr * ld + c];
+}
+
+hipError_t launch_rows_bitrev(const uint32_t* src, uint64_t ld, uint32_t w, int q, uint64_t r0, uint64_t nrows,
+                              uint32_t* dst, hipStream_t stream) {
+    if (nrows == 0 || w == 0) return hipSuccess;
+    rows_bitrev_kernel<<<blocks(nrows * w), 256, 0, stream>>>(src, ld, w, q, r0, nrows, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_field_add(int field, uint32_t* out, const uint32_t* add, size_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     if (field == FIELD_KOALABEAR)

@@ -180,6 +180,9 @@ hipError_t launch_pow_search(int field, const uint32_t* state, const uint32_t* i
 
 // ---- misc (api.hip) ------------------------------------------------------
 hipError_t launch_convert(int field, uint32_t* data, size_t n, int to_monty, hipStream_t stream);
+// dst row r = src row rev_q(r0 + r) for r < nrows (w words per row, src pitch ld)
+hipError_t launch_rows_bitrev(const uint32_t* src, uint64_t ld, uint32_t w, int q, uint64_t r0, uint64_t nrows,
+                              uint32_t* dst, hipStream_t stream);
 hipError_t launch_fill_random(int field, uint64_t seed, uint32_t* out, size_t n, int monty,
                               hipStream_t stream);
 

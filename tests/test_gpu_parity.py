@@ -8,6 +8,8 @@ import pytest
 
 import pico_amd
 from pico_amd import BABYBEAR, KOALABEAR
+from pico_amd._lib import check, lib
+from pico_amd.core import _ptr
 
 pytestmark = pytest.mark.gpu
 FIELDS = [KOALABEAR, BABYBEAR]
@@ -256,3 +258,29 @@ def test_fri_reference_final_mode(ctx, oracle):
     with pytest.raises(pico_amd.PgError):
         pico_amd.fri_commit_phase(ctx, f, pico_amd.DuplexChallenger(f),
                                   {ln: up(ctx, f, oracle.random_matrix(f, 99, 1 << ln, 4))}, lb, 0)
+
+
+def test_large_downloads_and_evaluations(ctx, oracle):
+    """Downloads big enough to go through the page-locked stage (several
+    16 MiB chunks, a ragged last one): raw and canonical words, the natural-
+    order evaluations of get_evaluations_on_domain on the full LDE domain and
+    on a sub-coset, and the same into a page-locked destination."""
+    f = KOALABEAR
+    a = oracle.random_matrix(f, 4242, 1 << 16, 40)
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    root, pd = pcs.commit([(pcs.natural_domain_for_degree(1 << 16), up(ctx, f, a))])
+    lde = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd)[0]
+    ref = oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f])
+    assert np.array_equal(lde.to_host(f), ref)
+    assert np.array_equal(pico_amd.from_monty(f, lde.to_host()), ref)
+    assert np.array_equal(lde.to_host(f, row0=5, nrows=(1 << 17) - 9), ref[5:-4])
+    nat = oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f], bit_reverse=False)
+    g = pico_amd.GENERATORS[f]
+    ev = pcs.get_evaluations_on_domain(pd, 0, pico_amd.TwoAdicMultiplicativeCoset(17, g))
+    assert np.array_equal(ev, nat)
+    ev = pcs.get_evaluations_on_domain(pd, 0, pico_amd.TwoAdicMultiplicativeCoset(15, g))
+    assert np.array_equal(ev, nat[::4])
+    pinned = ctx.host_matrix(1 << 17, 40)
+    check(lib().pg_pcs_evaluations(ctx.handle, pd.handle, 0, 1 << 17, _ptr(pinned.array)))
+    assert np.array_equal(pico_amd.from_monty(f, pinned.array), nat)
+    pinned.free()
