@@ -336,6 +336,20 @@ def main():
                            "reduce_openings_ms": (t2 - t1) * 1e3, "reduce_kernel_ms": red_ms,
                            "reduce_bytes": red_bytes, "reduce_GB_per_s": red_bytes / (red_ms / 1e3) / 1e9,
                            "reduce_frac_hbm": red_bytes / (red_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        # get_evaluations_on_domain over the whole 2^21-row LDE domain (the
+        # quotient domain at quotient degree 2): natural-order Montgomery
+        # words into a pageable host array, as the ABI hands them to Rust
+        from pico_amd._lib import check, lib
+        from pico_amd.core import _ptr
+        ev = np.zeros((n_rows, w), np.uint32)
+        ev_s = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            check(lib().pg_pcs_evaluations(ctx.handle, pd.handle, 0, n_rows, _ptr(ev)))
+            ev_s.append(time.perf_counter() - t0)
+        out["evaluations_on_domain"] = {"rows": n_rows, "cols": w, "ms": min(ev_s) * 1e3,
+                                        "GB_per_s": ev.nbytes / min(ev_s) / 1e9, "host_memory": "pageable"}
+        del ev
         pd.free()
     trace.free()
     if not args.no_workloads:

@@ -212,8 +212,11 @@ int pg_pcs_commit(pg_ctx *ctx, int field, const pg_buf *const *evals,
 int pg_pcs_commit_host(pg_ctx *ctx, int field, const uint32_t *const *host, const size_t *rows,
                        const size_t *cols, const uint32_t *domain_shifts_mont, size_t n,
                        uint32_t log_blowup, int canonical, uint32_t root[8], pg_tree **out);
-/* get_evaluations_on_domain: the first `rows` committed rows of matrix k,
- * returned in natural order (rows * width words). */
+/* get_evaluations_on_domain (p3 TwoAdicFriPcs, prover.rs:467-490 is the
+ * caller): the first `rows` committed rows of matrix k, i.e. its evaluations
+ * on the sub-coset of `rows` points, returned in natural order (rows * width
+ * Montgomery words).  The rows are put in natural order on the device; a
+ * pageable destination is filled through the page-locked stage. */
 int pg_pcs_evaluations(pg_ctx *ctx, const pg_tree *tree, size_t k, size_t rows, uint32_t *host);
 
 /* ---- Pcs::open (device part) -----------------------------------------------
