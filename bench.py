@@ -210,6 +210,10 @@ def main():
                     "textbook_instr_per_perm": TEXTBOOK_OPS_PER_PERM,
                     "algorithmic_bytes": n_rows * w * 4 + n_rows * 32,
                     "frac_of_measured_opcode_rate_floor": round(floor_s / avg_s, 4)}
+        # the same kernel against the HBM roof, to show it is not the bound
+        hbm_gbs = roofline["algorithmic_bytes"] / avg_s / 1e9
+        roofline["hbm_GB_per_s"] = round(hbm_gbs, 1)
+        roofline["hbm_frac"] = round(hbm_gbs / HBM_PEAK_GBS, 4)
     # HBM view of the NTT passes: algorithmic bytes per launch / duration
     by = 4 * w
     pass_bytes = {"lde_pass_a": 2 * h * by, "lde_pass_bc": h * by + n_rows * by,

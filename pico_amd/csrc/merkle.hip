@@ -45,6 +45,115 @@ __device__ __forceinline__ void absorb_rows(uint32_t (&st)[16], const MatRef* ma
     }
 }
 
+// ---- one permutation over 16 lanes ---------------------------------------
+// A layer with few nodes cannot fill the GPU, so with one state per thread it
+// costs one permutation's latency (about 5200 dependent VALU instructions,
+// ~11 us on MI355X) however small it is: the top ~15 layers of every tree
+// and the small FRI rounds.  For those layers the 16 lanes of a DPP row
+// (lanes 16r .. 16r+15 of a wave) hold one state, lane l holding s_l: the
+// S-boxes of a full round run side by side, and the linear layers take their
+// cross-lane terms from DPP moves (quad_perm inside groups of 4 lanes,
+// row_ror across the groups), about 30 dependent instructions per round.
+// All arithmetic is on reduced values, so the result is the same permutation
+// word for word.
+static __constant__ P2Const<KoalaBear> kLaneKb = make_p2const<KoalaBear>();
+static __constant__ P2Const<BabyBear> kLaneBb = make_p2const<BabyBear>();
+
+template <class F>
+__device__ __forceinline__ const P2Const<F>& lane_consts() {
+    if constexpr (F::P == KoalaBear::P)
+        return kLaneKb;
+    else
+        return kLaneBb;
+}
+
+// v from lane CTRL selects, within the lane's DPP row (every lane active)
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_from(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+constexpr int DPP_QUAD_SWAP1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_SWAP2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int DPP_QUAD_NEXT = 0x39;   // quad_perm [1,2,3,0]: lane i of a quad reads lane i+1
+constexpr int DPP_ROW_ROR4 = 0x124;
+constexpr int DPP_ROW_ROR8 = 0x128;
+
+// Sum over the lane's group of 4, then over the 4 groups: both orbits are
+// complete whichever way the moves rotate.
+template <class F>
+__device__ __forceinline__ uint32_t quad_sum(uint32_t x) {
+    const uint32_t a = fadd<F>(x, dpp_from<DPP_QUAD_SWAP1>(x));
+    return fadd<F>(a, dpp_from<DPP_QUAD_SWAP2>(a));
+}
+template <class F>
+__device__ __forceinline__ uint32_t across_quads(uint32_t y) {
+    const uint32_t u = fadd<F>(y, dpp_from<DPP_ROW_ROR4>(y));
+    return fadd<F>(u, dpp_from<DPP_ROW_ROR8>(u));
+}
+
+// External layer: M4 row i is (2, 3, 1, 1) rotated by i, i.e.
+// y_i = (x_0 + x_1 + x_2 + x_3) + x_i + 2 x_{i+1 mod 4}; then
+// s_i = y_i + sum over the four groups of y at position i mod 4.
+template <class F>
+__device__ __forceinline__ uint32_t external_lanes(uint32_t x) {
+    const uint32_t nx = dpp_from<DPP_QUAD_NEXT>(x);
+    const uint32_t y = fadd<F>(fadd<F>(quad_sum<F>(x), x), fadd<F>(nx, nx));
+    return fadd<F>(y, across_quads<F>(y));
+}
+
+// Internal layer: s_i = s_i diag_i + sum of all 16
+template <class F>
+__device__ __forceinline__ uint32_t internal_lanes(uint32_t x, uint32_t diag) {
+    return fadd<F>(fmul<F>(x, diag), across_quads<F>(quad_sum<F>(x)));
+}
+
+// x = s_l in, P(s)_l out; l = lane index within the DPP row.
+template <class F>
+__device__ __forceinline__ uint32_t permute_lanes(uint32_t x, int l) {
+    const P2Const<F>& K = lane_consts<F>();
+    uint32_t rc[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) rc[r] = K.ext_rc[r][l];
+    const uint32_t diag = K.diag[l];
+    x = external_lanes<F>(x);
+#pragma unroll
+    for (int r = 0; r < 4; r++) x = external_lanes<F>(sbox<F>(fadd<F>(x, rc[r])));
+#pragma unroll
+    for (int r = 0; r < F::ROUNDS_P; r++) {
+        const uint32_t v = sbox<F>(fadd<F>(x, P2<F>::K.int_rc[r]));
+        x = internal_lanes<F>(l == 0 ? v : x, diag);
+    }
+#pragma unroll
+    for (int r = 4; r < 8; r++) x = external_lanes<F>(sbox<F>(fadd<F>(x, rc[r])));
+    return x;
+}
+
+// Layers of at most this many nodes (or leaves) take the 16-lane kernels:
+// above it the 16x larger thread count costs more than the latency saved.
+constexpr uint64_t LANES_MAX_NODES = 1u << 14;
+
+// next[i] = C(prev[2i], prev[2i+1]): the two child digests are the state.
+// Thread 16 i + l is lane l of node i; a node's 16 lanes leave together.
+template <class F>
+__global__ void __launch_bounds__(256) compress_lanes_kernel(const uint32_t* prev, uint32_t* next, uint64_t n_out) {
+    const uint64_t node = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = (int)(threadIdx.x & 15u);
+    if (node >= n_out) return;
+    const uint32_t x = permute_lanes<F>(((gcptr)prev)[16 * node + l], l);
+    if (l < 8) ((gptr)next)[8 * node + l] = x;
+}
+
+// Rows of exactly 8 words: one sponge chunk, st = (row, 0^8).
+template <class F>
+__global__ void __launch_bounds__(256) leaf8_lanes_kernel(const uint32_t* data, uint64_t stride, uint64_t rows,
+                                                          uint32_t* digests) {
+    const uint64_t row = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = (int)(threadIdx.x & 15u);
+    if (row >= rows) return;
+    const uint32_t x = permute_lanes<F>(l < 8 ? ((gcptr)data)[row * stride + l] : 0u, l);
+    if (l < 8) ((gptr)digests)[8 * row + l] = x;
+}
+
 // General leaf hash: any number of matrices, any widths.
 template <class F>
 __global__ void __launch_bounds__(256) leaf_hash_kernel(const MatRef* mats, int n, uint32_t total,
@@ -270,6 +379,20 @@ __global__ void __launch_bounds__(256) compress_batch_kernel(const CompressJob* 
     o[1] = u32x4{st[4], st[5], st[6], st[7]};
 }
 
+template <class F>
+__global__ void __launch_bounds__(256) compress_batch_lanes_kernel(const CompressJob* __restrict__ jobs,
+                                                                   const uint64_t* __restrict__ prefix,
+                                                                   int njobs, uint64_t total) {
+    const uint64_t gi = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = (int)(threadIdx.x & 15u);
+    if (gi >= total) return;
+    const int j = find_job(prefix, njobs, gi);
+    const CompressJob jb = jobs[j];
+    const uint64_t i = gi - prefix[j];
+    const uint32_t x = permute_lanes<F>(((gcptr)jb.prev)[16 * i + l], l);
+    if (l < 8) ((gptr)jb.next)[8 * i + l] = x;
+}
+
 __global__ void gather_roots_kernel(const uint32_t* const* __restrict__ src, uint32_t* __restrict__ dst, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n * 8) dst[i] = src[i >> 3][i & 7];
@@ -296,6 +419,14 @@ hipError_t launch_leaf_hash_batch(int field, const LeafJob* jobs, const uint64_t
 hipError_t launch_compress_batch(int field, const CompressJob* jobs, const uint64_t* prefix, int njobs,
                                  uint64_t total, hipStream_t stream) {
     if (total == 0) return hipSuccess;
+    if (total <= LANES_MAX_NODES) {
+        const unsigned g = grid_for(total * 16, 256);
+        if (field == FIELD_KOALABEAR)
+            compress_batch_lanes_kernel<KoalaBear><<<g, 256, 0, stream>>>(jobs, prefix, njobs, total);
+        else
+            compress_batch_lanes_kernel<BabyBear><<<g, 256, 0, stream>>>(jobs, prefix, njobs, total);
+        return hipGetLastError();
+    }
     if (field == FIELD_KOALABEAR)
         compress_batch_kernel<KoalaBear><<<grid_for(total, 256), 256, 0, stream>>>(jobs, prefix, njobs, total);
     else
@@ -341,6 +472,14 @@ hipError_t launch_absorb_segment(int field, const uint32_t* data, uint64_t strid
 hipError_t launch_leaf_hash_w8(int field, const uint32_t* data, uint64_t stride, uint32_t width,
                                size_t rows, uint32_t* digests, hipStream_t stream) {
     if (rows == 0) return hipSuccess;
+    if (width == 8 && rows <= LANES_MAX_NODES) {
+        const unsigned g = grid_for(rows * 16, 256);
+        if (field == FIELD_KOALABEAR)
+            leaf8_lanes_kernel<KoalaBear><<<g, 256, 0, stream>>>(data, stride, rows, digests);
+        else
+            leaf8_lanes_kernel<BabyBear><<<g, 256, 0, stream>>>(data, stride, rows, digests);
+        return hipGetLastError();
+    }
     const unsigned g = grid_for(rows, 256);
     if (field == FIELD_KOALABEAR)
         leaf_hash_w8_kernel<KoalaBear><<<g, 256, 0, stream>>>(data, stride, width / 8, rows, digests);
@@ -352,6 +491,14 @@ hipError_t launch_leaf_hash_w8(int field, const uint32_t* data, uint64_t stride,
 hipError_t launch_compress(int field, const uint32_t* prev, uint32_t* next, size_t n_out,
                            const MatRef* inj, int n_inj, uint32_t inj_width, hipStream_t stream) {
     if (n_out == 0) return hipSuccess;
+    if (n_inj == 0 && n_out <= LANES_MAX_NODES) {
+        const unsigned g = grid_for(n_out * 16, 256);
+        if (field == FIELD_KOALABEAR)
+            compress_lanes_kernel<KoalaBear><<<g, 256, 0, stream>>>(prev, next, n_out);
+        else
+            compress_lanes_kernel<BabyBear><<<g, 256, 0, stream>>>(prev, next, n_out);
+        return hipGetLastError();
+    }
     const unsigned g = grid_for(n_out, 256);
     if (field == FIELD_KOALABEAR)
         compress_kernel<KoalaBear><<<g, 256, 0, stream>>>(prev, next, n_out, inj, n_inj, inj_width);
