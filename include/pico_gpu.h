@@ -93,6 +93,11 @@ int pg_profile_get(pg_ctx *ctx, const char *family, double *total_ms, uint64_t *
  * (which panics on allocation failure); every entry point must then return
  * the error and leave no device memory behind. */
 int pg_debug_fail_alloc(pg_ctx *ctx, long long after);
+/* Column-block sizing of pg_pcs_commit_host, for tests that exercise the
+ * blocked path on small matrices: blocks of about `block_bytes`, at least
+ * `min_cols` columns wide (0 restores the defaults, 256 MiB and 64).  No
+ * counterpart in the reference. */
+int pg_debug_host_blocks(pg_ctx *ctx, size_t block_bytes, size_t min_cols);
 
 /* ---- LDE tile-shape autotuning ------------------------------------------------
  * Each LDE pass (single, A, BC, D) of a size-2^s transform runs one of up to

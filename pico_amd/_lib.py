@@ -57,6 +57,7 @@ SIGNATURES = {
     "pg_profile_get": (_i, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_uint64)]),
     "pg_debug_fail_alloc": (_i, [_P, ctypes.c_longlong]),
+    "pg_debug_host_blocks": (_i, [_P, ctypes.c_size_t, ctypes.c_size_t]),
     "pg_autotune": (_i, [_P, _i, _i, _i, _P, _sz, _u32, _i, ctypes.c_char_p, _sz]),
     "pg_tune_set": (_i, [_i, _i, _i, _i, _i]),
     "pg_tune_get": (_i, [_i, _i, _i, _u32, ctypes.POINTER(ctypes.c_int)]),

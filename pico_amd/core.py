@@ -77,6 +77,11 @@ class Context:
         with PG_E_OOM (negative: disarm).  For tests of error paths."""
         check(lib().pg_debug_fail_alloc(self.handle, int(after)))
 
+    def debug_host_blocks(self, block_bytes=0, min_cols=0):
+        """Column-block sizing of commit_host (0: the defaults).  For tests
+        of the blocked path on small matrices."""
+        check(lib().pg_debug_host_blocks(self.handle, int(block_bytes), int(min_cols)))
+
     def profile_reset(self):
         check(lib().pg_profile_reset(self.handle))
 

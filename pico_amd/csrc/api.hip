@@ -80,6 +80,9 @@ struct DevMem {  // device allocation accounted against the context budget
     size_t bytes = 0;
 };
 
+// column blocks of pg_pcs_commit_host (see there)
+static constexpr size_t HOST_BLOCK_BYTES = 256ull << 20, HOST_BLOCK_MIN_COLS = 64;
+
 struct pg_ctx {
     int device = 0;
     size_t budget = 0;
@@ -113,6 +116,8 @@ struct pg_ctx {
     // fault injection: the allocation this many allocations from now fails
     // once with PG_E_OOM; negative = off
     std::atomic<long long> fail_countdown{-1};
+    // column-block sizing of pg_pcs_commit_host (pg_debug_host_blocks)
+    std::atomic<size_t> host_block_bytes{HOST_BLOCK_BYTES}, host_block_min_cols{HOST_BLOCK_MIN_COLS};
     // page-locked double buffers for uploads from pageable memory (struct
     // Stage), created on first use and freed by pg_destroy
     std::mutex stage_mu;
@@ -513,7 +518,10 @@ static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, co
                 for (size_t r = (size_t)i * per; r < std::min(n, (size_t)(i + 1) * per); r++)
                     memcpy(buf + r * width, (const char*)src + (done + r) * spitch, width);
             });
-            e = hipMemcpy2DAsync((char*)dst + done * dpitch, dpitch, buf, width, width, n, hipMemcpyHostToDevice, s);
+            e = dpitch == width
+                    ? hipMemcpyAsync((char*)dst + done * width, buf, n * width, hipMemcpyHostToDevice, s)
+                    : hipMemcpy2DAsync((char*)dst + done * dpitch, dpitch, buf, width, width, n, hipMemcpyHostToDevice,
+                                       s);
         }
         if (e == hipSuccess) e = hipEventRecord(st->ev[b], s);
         used[b] = e == hipSuccess;
@@ -607,6 +615,13 @@ extern "C" int pg_profile_get(pg_ctx* ctx, const char* name, double* total_ms, u
 extern "C" int pg_debug_fail_alloc(pg_ctx* ctx, long long after) {
     if (!ctx) return fail(PG_E_INVALID, "null context");
     ctx->fail_countdown.store(after < 0 ? -1 : after);
+    return PG_OK;
+}
+
+extern "C" int pg_debug_host_blocks(pg_ctx* ctx, size_t block_bytes, size_t min_cols) {
+    if (!ctx) return fail(PG_E_INVALID, "null context");
+    ctx->host_block_bytes.store(block_bytes ? block_bytes : HOST_BLOCK_BYTES);
+    ctx->host_block_min_cols.store(min_cols ? min_cols : HOST_BLOCK_MIN_COLS);
     return PG_OK;
 }
 
@@ -1328,18 +1343,29 @@ extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const*
     auto kt = make_timer(ctx);
     // The tallest matrices are cut into column blocks (Montgomery input only:
     // the canonical conversion works on whole matrices): each block is
-    // uploaded as a pitched copy (from pageable memory its rows are gathered
-    // into a stage first), extended into its columns of the LDE and absorbed
-    // into every row's sponge, so the leaf hash runs behind the upload
-    // instead of after it.  512-byte block rows keep pitched copies at the
-    // link's rate.
-    constexpr size_t CB = 128, SPLIT_BYTES = 64ull << 20;
+    // uploaded, extended into its columns of the LDE and absorbed into every
+    // row's sponge, so the leaf hash runs behind the upload instead of after
+    // it.  On the device a block is stored dense (rows x cb words at word
+    // rows * c0 of the input buffer), so its upload is one contiguous DMA
+    // from a stage into which host threads gather the block's row segments
+    // (a DMA that reads the pitched rows itself is slower, from page-locked
+    // memory too).  Blocks of about HOST_BLOCK_BYTES keep the part of the work
+    // that cannot overlap an upload (the last block's LDE and absorb) short;
+    // at least HOST_BLOCK_MIN_COLS columns wide, as every block costs a pass
+    // over the sponge states (128 bytes per LDE row) and narrow LDE tiles
+    // leave lanes idle.  Measured on MI355X, 2^20 x 256 at blowup 2: 25.4 ms
+    // from page-locked and 25.3 ms from pageable memory with 64-column
+    // blocks, 26.8 / 28.6 ms with 128-column blocks written by pitched DMAs.
+    constexpr size_t SPLIT_BYTES = 64ull << 20;
+    const size_t block_bytes = ctx->host_block_bytes.load(), min_cols = ctx->host_block_min_cols.load();
     size_t Hmax = 0, segments = 0;
     for (size_t k = 0; k < n; k++) Hmax = std::max(Hmax, rows[k]);
     std::vector<bool> locked(n, false);
     for (size_t k = 0; k < n; k++) locked[k] = host_is_locked(host[k]);
     auto block_cols = [&](size_t k) {
-        return !canonical && rows[k] * cols[k] * 4 >= SPLIT_BYTES ? std::min(CB, cols[k]) : cols[k];
+        if (canonical || rows[k] * cols[k] * 4 < SPLIT_BYTES) return cols[k];
+        const size_t cb = std::max<size_t>(min_cols, block_bytes / (rows[k] * 4) / 8 * 8);
+        return std::min(cb, cols[k]);
     };
     for (size_t k = 0; k < n; k++)
         if (rows[k] == Hmax) segments += (cols[k] + block_cols(k) - 1) / block_cols(k);
@@ -1351,18 +1377,16 @@ extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const*
             return rc;
         }
     }
+    // block (k, c0): dense at word rows * c0 of in[k] (the whole matrix, in
+    // its own layout, when it is not split)
+    auto block_ptr = [&](size_t k, size_t c0) { return in[k]->ptr + rows[k] * c0; };
     auto upload_wait = [&](size_t k, size_t c0, size_t cb) -> int {
-        if (!locked[k] && in[k]->bytes >= STAGE_MIN_BYTES) {  // pageable: through a stage
-            if (cb == cols[k])
-                PG_TRY(stage_upload(ctx, lc.s, in[k]->ptr, in[k]->bytes, host[k], in[k]->bytes, in[k]->bytes, 1));
-            else
-                PG_TRY(stage_upload(ctx, lc.s, in[k]->ptr + c0, cols[k] * 4, host[k] + c0, cols[k] * 4, cb * 4,
-                                    rows[k]));
-        } else if (cb == cols[k]) {
-            HIP_TRY(hipMemcpyAsync(in[k]->ptr, host[k], in[k]->bytes, hipMemcpyHostToDevice, lc.s));
+        if (cb != cols[k]) {  // row segments gathered into a stage
+            PG_TRY(stage_upload(ctx, lc.s, block_ptr(k, c0), cb * 4, host[k] + c0, cols[k] * 4, cb * 4, rows[k]));
+        } else if (!locked[k] && in[k]->bytes >= STAGE_MIN_BYTES) {  // pageable: through a stage
+            PG_TRY(stage_upload(ctx, lc.s, in[k]->ptr, in[k]->bytes, host[k], in[k]->bytes, in[k]->bytes, 1));
         } else {
-            HIP_TRY(hipMemcpy2DAsync(in[k]->ptr + c0, cols[k] * 4, host[k] + c0, cols[k] * 4, cb * 4, rows[k],
-                                     hipMemcpyHostToDevice, lc.s));
+            HIP_TRY(hipMemcpyAsync(in[k]->ptr, host[k], in[k]->bytes, hipMemcpyHostToDevice, lc.s));
         }
         if (canonical) HIP_TRY(launch_convert(field, in[k]->ptr, rows[k] * cols[k], 1, lc.s));
         hipEvent_t e;
@@ -1373,7 +1397,7 @@ extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const*
         return PG_OK;
     };
     auto lde_cols = [&](size_t k, size_t c0, size_t cb) -> hipError_t {
-        return launch_coset_lde(field, in[k]->ptr + c0, cols[k], L[k], (uint32_t)cb, (int)log_blowup,
+        return launch_coset_lde(field, block_ptr(k, c0), cb, L[k], (uint32_t)cb, (int)log_blowup,
                                 (const uint32_t*)tw[k]->ptr, (const uint32_t*)sc[k]->ptr,
                                 L[k] > 11 ? (uint32_t*)scratch->ptr : nullptr, ldes[k]->ptr + c0, cols[k], 1, l.s,
                                 kt.get());

@@ -198,32 +198,61 @@ def test_commit_host_rejects_bad_shapes_without_leaking(ctx):
 
 
 def test_commit_host_column_blocks_match_oracle(ctx, oracle):
-    """Tallest matrices large enough to be cut into column blocks (uploaded,
-    extended and absorbed block by block): widths 300 (blocks 128, 128, 44)
-    and 200 (128, 72, starting mid-chunk at input 300 = 37 * 8 + 4), an
-    injected shorter matrix, Montgomery words in page-locked memory."""
+    """Tallest matrices cut into column blocks (uploaded through the stage,
+    extended and absorbed block by block), with the block size forced down
+    to 64 MiB / 8 columns so small matrices split: widths 300 (blocks 128,
+    128, 44) and 200 (128, 72, starting mid-chunk at input 300 = 37 * 8 + 4),
+    an injected shorter matrix; Montgomery words in page-locked and in
+    pageable memory."""
     f = KOALABEAR
-    p = pico_amd.PRIMES[f]
     shapes = [(17, 300), (15, 9), (17, 200)]
     hosts = [oracle.random_matrix(f, 6000 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
     pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
     doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
-    pinned = []  # column blocks are cut only from page-locked memory
+    pinned = []
     for a in hosts:
         pm = ctx.host_matrix(*a.shape)
         pm.array[:] = _to_monty(f, a)
         pinned.append(pm)
-    root_h, pd_h = pcs.commit_host([(d, pm.array) for d, pm in zip(doms, pinned)], canonical=False)
+    ctx.debug_host_blocks(64 << 20, 8)
+    try:
+        root_h, pd_h = pcs.commit_host([(d, pm.array) for d, pm in zip(doms, pinned)], canonical=False)
+        root_p, pd_p = pcs.commit_host([(d, pm.array.copy()) for d, pm in zip(doms, pinned)], canonical=False)
+    finally:
+        ctx.debug_host_blocks()
     root_d, pd_d = pcs.commit([(d, ctx.upload(a, field=f)) for d, a in zip(doms, hosts)])
     assert np.array_equal(root_h, root_d)
+    assert np.array_equal(root_p, root_d)
     ldes = [oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f]) for a in hosts]
     assert np.array_equal(root_h, oracle.MerkleTree(f, ldes).root)
-    mats = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd_h)
-    for k in (0, 2):
-        assert np.array_equal(mats[k].to_host(f, row0=1000, nrows=64), ldes[k][1000:1064]), k
-    # the same from pageable memory (whole matrices)
+    for pd in (pd_h, pd_p):
+        mats = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd)
+        for k in (0, 2):
+            assert np.array_equal(mats[k].to_host(f, row0=1000, nrows=64), ldes[k][1000:1064]), k
+    for x in (pd_h, pd_d, pd_p):
+        x.free()
+    for pm in pinned:
+        pm.free()
+
+
+def test_commit_host_default_blocks(ctx):
+    """Default block sizing (256 MiB, at least 64 columns): a 2^20 x 72
+    matrix is cut into blocks of 64 and 8 columns; an injected shorter
+    matrix rides along.  Roots equal upload-then-commit (itself pinned to the
+    oracle by the parity tests), from page-locked and pageable memory."""
+    f = KOALABEAR
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    shapes = [(20, 72), (18, 20)]
+    devs = [ctx.random_matrix(f, 6100 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    root_d, pd_d = pcs.commit(list(zip(doms, devs)))
+    pinned = [ctx.host_matrix(1 << L, w) for L, w in shapes]
+    for pm, d in zip(pinned, devs):
+        pm.array[:] = d.to_host()
+    root_h, pd_h = pcs.commit_host([(d, pm.array) for d, pm in zip(doms, pinned)], canonical=False)
     root_p, pd_p = pcs.commit_host([(d, pm.array.copy()) for d, pm in zip(doms, pinned)], canonical=False)
-    assert np.array_equal(root_p, root_h)
+    assert np.array_equal(root_h, root_d)
+    assert np.array_equal(root_p, root_d)
     for x in (pd_h, pd_d, pd_p):
         x.free()
     for pm in pinned:
