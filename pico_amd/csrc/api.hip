@@ -1052,9 +1052,13 @@ static int tree_prepare(pg_ctx* ctx, hipStream_t s, int field, const std::vector
 // wait for it), so a commit can start hashing tall matrices while shorter
 // ones are still arriving.  For the tallest run it also gets the leaf
 // digest layer; returning STAGE_HASHED says it has queued those digests
-// itself (and the leaf hash is skipped).
+// itself (and the leaf hash is skipped).  For a shorter run it may set *inj
+// to the digests of the run's concatenated rows, queued on s already; the
+// layer then only compresses.  `sorted` is the device descriptor array of
+// the matrices in height order.
 static constexpr int STAGE_HASHED = 1;
-using StageFn = std::function<int(const std::vector<size_t>& order, size_t g0, size_t g1, uint32_t* leaves)>;
+using StageFn = std::function<int(const std::vector<size_t>& order, size_t g0, size_t g1, uint32_t* leaves,
+                                  const MatRef* sorted, const uint32_t** inj)>;
 
 static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const std::vector<pg_buf*>& mats,
                                  const std::vector<bool>& owned, uint32_t root[8], pg_tree** out,
@@ -1074,7 +1078,7 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
     auto kt = make_timer(ctx);
     bool hashed = false;
     if (stage) {
-        const int r = stage(order, g0, g1, t->digests);
+        const int r = stage(order, g0, g1, t->digests, d_sorted, nullptr);
         if (r < 0) return r;
         hashed = r == STAGE_HASHED;
     }
@@ -1091,10 +1095,15 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
         size_t gj = gi;
         uint32_t iw = 0;
         while (gj < n && mats[order[gj]]->rows == nl) iw += (uint32_t)mats[order[gj++]]->cols;
-        if (stage && gj > gi) PG_TRY(stage(order, gi, gj, nullptr));
-        TIMED(kt, gj > gi ? "compress_inject" : "compress", s,
-              launch_compress(field, t->digests + 8 * off, t->digests + 8 * (off + len), nl,
-                              gj > gi ? d_sorted + gi : nullptr, (int)(gj - gi), iw, s));
+        const uint32_t* inj = nullptr;
+        if (stage && gj > gi) PG_TRY(stage(order, gi, gj, nullptr, d_sorted, &inj));
+        if (inj)
+            TIMED(kt, "compress_digest", s,
+                  launch_compress_digest(field, t->digests + 8 * off, t->digests + 8 * (off + len), nl, inj, s));
+        else
+            TIMED(kt, gj > gi ? "compress_inject" : "compress", s,
+                  launch_compress(field, t->digests + 8 * off, t->digests + 8 * (off + len), nl,
+                                  gj > gi ? d_sorted + gi : nullptr, (int)(gj - gi), iw, s));
         off += len;
         len = nl;
         gi = gj;
@@ -1402,7 +1411,41 @@ extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const*
                                 L[k] > 11 ? (uint32_t*)scratch->ptr : nullptr, ldes[k]->ptr + c0, cols[k], 1, l.s,
                                 kt.get());
     };
-    StageFn stage = [&](const std::vector<size_t>& order, size_t g0, size_t g1, uint32_t* leaves) -> int {
+    // The matrices below the tallest height are uploaded, extended and
+    // row-hashed first: the upload of the tallest ones bounds the call, and
+    // the GPU has time to spare meanwhile.  Left to their level, their
+    // uploads, LDEs and the hashing of their rows (2 + w/8 permutations per
+    // node at the injection layers) would all run after that upload.  Their
+    // digests (32 bytes per LDE row) wait in HBM for their layer.
+    std::map<size_t, std::shared_ptr<DevMem>> run_digests;  // run start -> digests of its rows
+    auto shorter_first = [&](const std::vector<size_t>& order, size_t g1, const MatRef* sorted) -> int {
+        for (size_t a = g1, b; a < order.size(); a = b) {
+            uint64_t wsum = 0;
+            for (b = a; b < order.size() && rows[order[b]] == rows[order[a]]; b++) {
+                const size_t k = order[b];
+                PG_TRY(upload_wait(k, 0, cols[k]));
+                HIP_TRY(lde_cols(k, 0, cols[k]));
+                wsum += cols[k];
+            }
+            if (wsum > 0xffffffffull) return fail(PG_E_INVALID, "total width too large");
+            const size_t nl = rows[order[a]] << log_blowup;
+            int mrc = PG_OK;
+            auto d = make_devmem(ctx, nl * 32, &mrc);
+            if (!d) return mrc;
+            TIMED(kt, "leaf_hash", l.s,
+                  launch_leaf_hash(field, sorted + a, (int)(b - a), (uint32_t)wsum, nl, (uint32_t*)d->ptr, l.s));
+            run_digests[a] = d;
+        }
+        return PG_OK;
+    };
+    StageFn stage = [&](const std::vector<size_t>& order, size_t g0, size_t g1, uint32_t* leaves,
+                        const MatRef* sorted, const uint32_t** inj) -> int {
+        if (leaves) PG_TRY(shorter_first(order, g1, sorted));
+        auto it = run_digests.find(g0);
+        if (!leaves && it != run_digests.end()) {
+            *inj = (const uint32_t*)it->second->ptr;
+            return PG_OK;
+        }
         if (leaves && segments > 1) {
             uint64_t wsum = 0, done = 0;
             for (size_t j = g0; j < g1; j++) wsum += cols[order[j]];

@@ -51,6 +51,10 @@ hipError_t launch_leaf_hash(int field, const MatRef* mats, int n, uint32_t total
 // next[i] = C(next[i], sponge(concat_k row i of inj[k])), for i < n_out.
 hipError_t launch_compress(int field, const uint32_t* prev, uint32_t* next, size_t n_out,
                            const MatRef* inj, int n_inj, uint32_t inj_width, hipStream_t stream);
+// The same with the injected rows already hashed: next[i] = C(C(prev[2i],
+// prev[2i+1]), inj[i]) for 8-word digests inj[i].
+hipError_t launch_compress_digest(int field, const uint32_t* prev, uint32_t* next, size_t n_out,
+                                  const uint32_t* inj, hipStream_t stream);
 // Opening gather: for each query q < nq and matrix k, copy row
 // (idx[q] >> shift[k]) of mats[k] to rows_out[q * total_width + col_off[k] ..];
 // for each layer j < log_h, copy digest layer_j[(idx[q] >> j) ^ 1] to

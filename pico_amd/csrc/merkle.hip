@@ -143,6 +143,21 @@ __global__ void __launch_bounds__(256) compress_lanes_kernel(const uint32_t* pre
     if (l < 8) ((gptr)next)[8 * node + l] = x;
 }
 
+// next[i] = C(C(prev[2i], prev[2i+1]), inj[i]) with the digests inj[i] of the
+// injected rows computed beforehand: the second state is the first one's
+// output with its upper half replaced by inj[i].
+template <class F>
+__global__ void __launch_bounds__(256) compress_digest_lanes_kernel(const uint32_t* prev, const uint32_t* inj,
+                                                                    uint32_t* next, uint64_t n_out) {
+    const uint64_t node = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = (int)(threadIdx.x & 15u);
+    if (node >= n_out) return;
+    const uint32_t d = l < 8 ? 0u : ((gcptr)inj)[8 * node + (l - 8)];
+    uint32_t x = permute_lanes<F>(((gcptr)prev)[16 * node + l], l);
+    x = permute_lanes<F>(l < 8 ? x : d, l);
+    if (l < 8) ((gptr)next)[8 * node + l] = x;
+}
+
 // Rows of exactly 8 words: one sponge chunk, st = (row, 0^8).
 template <class F>
 __global__ void __launch_bounds__(256) leaf8_lanes_kernel(const uint32_t* data, uint64_t stride, uint64_t rows,
@@ -303,6 +318,29 @@ __global__ void open_gather_kernel(const MatRef* mats, const uint32_t* shift,
         const uint64_t sib = (index >> j) ^ 1u;
         paths_out[((uint64_t)q * log_h + j) * 8 + e] = digests[(layer_off[j] + sib) * 8 + e];
     }
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) compress_digest_kernel(const uint32_t* prev, const uint32_t* inj,
+                                                              uint32_t* next, uint64_t n_out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const GLOBAL u32x4* p = reinterpret_cast<const GLOBAL u32x4*>((gcptr)prev + 16 * i);
+    uint32_t st[16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const u32x4 v = p[q];
+        st[4 * q] = v.x; st[4 * q + 1] = v.y; st[4 * q + 2] = v.z; st[4 * q + 3] = v.w;
+    }
+    permute<F>(st);
+    const GLOBAL u32x4* d = reinterpret_cast<const GLOBAL u32x4*>((gcptr)inj + 8 * i);
+    const u32x4 d0 = d[0], d1 = d[1];
+    st[8] = d0.x; st[9] = d0.y; st[10] = d0.z; st[11] = d0.w;
+    st[12] = d1.x; st[13] = d1.y; st[14] = d1.z; st[15] = d1.w;
+    permute<F>(st);
+    GLOBAL u32x4* o = reinterpret_cast<GLOBAL u32x4*>((gptr)next + 8 * i);
+    o[0] = u32x4{st[0], st[1], st[2], st[3]};
+    o[1] = u32x4{st[4], st[5], st[6], st[7]};
 }
 
 // ---- batched single-matrix commits --------------------------------------
@@ -504,6 +542,25 @@ hipError_t launch_compress(int field, const uint32_t* prev, uint32_t* next, size
         compress_kernel<KoalaBear><<<g, 256, 0, stream>>>(prev, next, n_out, inj, n_inj, inj_width);
     else
         compress_kernel<BabyBear><<<g, 256, 0, stream>>>(prev, next, n_out, inj, n_inj, inj_width);
+    return hipGetLastError();
+}
+
+hipError_t launch_compress_digest(int field, const uint32_t* prev, uint32_t* next, size_t n_out,
+                                  const uint32_t* inj, hipStream_t stream) {
+    if (n_out == 0) return hipSuccess;
+    if (n_out <= LANES_MAX_NODES) {
+        const unsigned g = grid_for(n_out * 16, 256);
+        if (field == FIELD_KOALABEAR)
+            compress_digest_lanes_kernel<KoalaBear><<<g, 256, 0, stream>>>(prev, inj, next, n_out);
+        else
+            compress_digest_lanes_kernel<BabyBear><<<g, 256, 0, stream>>>(prev, inj, next, n_out);
+        return hipGetLastError();
+    }
+    const unsigned g = grid_for(n_out, 256);
+    if (field == FIELD_KOALABEAR)
+        compress_digest_kernel<KoalaBear><<<g, 256, 0, stream>>>(prev, inj, next, n_out);
+    else
+        compress_digest_kernel<BabyBear><<<g, 256, 0, stream>>>(prev, inj, next, n_out);
     return hipGetLastError();
 }
 
