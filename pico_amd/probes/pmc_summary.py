@@ -26,8 +26,14 @@ def family(kernel):
     m = re.search(r"lde_kernel<.*?, (\d+), (\d+), (\d+)>\(", kernel)
     if m:
         return KIND_NAMES[int(m.group(3))]
+    # the 16-lane forms count with their one-thread families, as in bench.py
+    lanes = {"compress_lanes_kernel": "compress", "compress_batch_lanes_kernel": "compress_batch",
+             "compress_digest_lanes_kernel": "compress_digest", "leaf8_lanes_kernel": "leaf_hash_w8"}
+    for name, fam in lanes.items():
+        if name + "<" in kernel or name + "(" in kernel:
+            return fam
     for name in ("leaf_hash_w8_kernel", "leaf_hash_batch_kernel", "leaf_hash_kernel", "compress_batch_kernel",
-                 "compress_kernel", "copy_kernel"):
+                 "compress_digest_kernel", "compress_kernel", "copy_kernel"):
         if name + "<" in kernel or name + "(" in kernel:
             return name[: -len("_kernel")]
     return None
