@@ -257,3 +257,30 @@ def test_commit_host_default_blocks(ctx):
         x.free()
     for pm in pinned:
         pm.free()
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_commit_host_split_tail_canonical(ctx, oracle, f):
+    """Several matrices of the tallest height: the last one's columns are
+    uploaded, converted from canonical form and absorbed in narrower pieces
+    (24 columns -> 8 + 16); roots equal the oracle's, pageable and
+    page-locked."""
+    shapes = [(12, 40), (10, 5), (12, 24)]
+    hosts = [oracle.random_matrix(f, 6200 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    ldes = [oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f]) for a in hosts]
+    want = oracle.MerkleTree(f, ldes).root
+    root, pd = pcs.commit_host(list(zip(doms, hosts)))
+    assert np.array_equal(root, want)
+    pd.free()
+    pinned = []
+    for a in hosts:
+        pm = ctx.host_matrix(*a.shape)
+        pm.array[:] = a
+        pinned.append(pm)
+    root, pd = pcs.commit_host([(d, pm.array) for d, pm in zip(doms, pinned)])
+    assert np.array_equal(root, want)
+    pd.free()
+    for pm in pinned:
+        pm.free()
