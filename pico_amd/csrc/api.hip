@@ -333,10 +333,11 @@ static int bind_device(pg_ctx* ctx) {
 }
 
 // --------------------------------------------------------------- staging --
-// Uploads from pageable host memory.  The runtime stages those itself at
-// ~26 GB/s on MI355X hosts; copying 16 MiB chunks with 8 host threads into a
+// Copies between device memory and pageable host memory, and gathers of
+// column blocks.  The runtime stages pageable copies itself at ~26-30 GB/s
+// on MI355X hosts; copying 16 MiB chunks with 8 host threads through a
 // page-locked double buffer, each chunk's DMA overlapping the copy of the
-// next, reaches ~49 GB/s of the link's 57.6.
+// next, reaches ~54 GB/s up and ~51 GB/s down of the link's 57.6.
 class HostPool {  // persistent threads running the pieces of one copy at a time
   public:
     explicit HostPool(int nt) {
