@@ -131,8 +131,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
-                          "part of it (see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "only one GPU is supported: the commitment path runs on one GPU's "
+                          "share (a whole job or matrix); sharding across GPUs is not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
