@@ -284,3 +284,21 @@ def test_commit_host_split_tail_canonical(ctx, oracle, f):
     pd.free()
     for pm in pinned:
         pm.free()
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_commit_host_single_tall_matrix_with_shorter_runs(ctx, oracle, f):
+    """One matrix of the tallest height (hashed by the tree's own leaf pass)
+    and two shorter runs, one of two matrices: the shorter runs are uploaded,
+    extended and row-hashed first and their layers only compress."""
+    shapes = [(9, 9), (11, 20), (9, 7), (4, 3)]
+    hosts = [oracle.random_matrix(f, 6300 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 2)
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    ldes = [oracle.coset_lde(f, a, 2, pico_amd.GENERATORS[f]) for a in hosts]
+    root, pd = pcs.commit_host(list(zip(doms, hosts)))
+    assert np.array_equal(root, oracle.MerkleTree(f, ldes).root)
+    mats = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd)
+    for k in range(len(shapes)):
+        assert np.array_equal(mats[k].to_host(f), ldes[k]), k
+    pd.free()
