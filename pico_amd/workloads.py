@@ -136,11 +136,14 @@ def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_
     return t
 
 
-def shard_ingest(ctx, shapes=None, reps=2):
+def shard_ingest(ctx, shapes=None, reps=3):
     """The shard's main commit from traces in host memory: upload every
     matrix then commit, against pg_pcs_commit_host (uploads overlapped with
     the commit of the taller height groups), from page-locked and from
-    pageable memory.  Host clock, best of `reps`; roots must agree."""
+    pageable memory.  Host clock, best of `reps` (single calls of
+    pg_pcs_commit_host vary by up to a third: the HIP runtime now and then
+    holds an asynchronous copy call on the host for milliseconds); roots
+    must agree."""
     f = core.KOALABEAR
     shapes = shapes or shard_shapes()
     pcs = core.TwoAdicFriPcs(ctx, f, 1)
