@@ -302,3 +302,57 @@ def test_commit_host_single_tall_matrix_with_shorter_runs(ctx, oracle, f):
     for k in range(len(shapes)):
         assert np.array_equal(mats[k].to_host(f), ldes[k]), k
     pd.free()
+
+
+def test_commit_host_concurrent_threads_share_the_staging_pool(ctx):
+    """Two threads commit pageable host matrices at the same time (their
+    staged copies share the process's host copy threads) and a third
+    downloads through the stage meanwhile: every root equals its solo
+    commit, the download equals its solo download."""
+    import threading
+
+    f = KOALABEAR
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    jobs = [[(16, 72), (14, 9)], [(17, 40), (15, 16)]]
+    hosts = []
+    for j, shapes in enumerate(jobs):
+        mats = [ctx.random_matrix(f, 6400 + 10 * j + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+        hosts.append([m.to_host() for m in mats])
+        for m in mats:
+            m.free()
+    doms = [[pcs.natural_domain_for_degree(1 << L) for L, _ in shapes] for shapes in jobs]
+    solo = []
+    for d, h in zip(doms, hosts):
+        r, pd = pcs.commit_host(list(zip(d, h)), canonical=False)
+        solo.append(r)
+        pd.free()
+    big = ctx.random_matrix(f, 6500, 1 << 18, 64)
+    big_solo = big.to_host()
+    got, errors = [None] * 3, []
+
+    def run(j):
+        try:
+            for _ in range(3):
+                if j < 2:
+                    r, pd = pcs.commit_host(list(zip(doms[j], hosts[j])), canonical=False)
+                    pd.free()
+                    if not np.array_equal(r, solo[j]):
+                        got[j] = r
+                        return
+                    got[j] = r
+                else:
+                    got[j] = big.to_host()
+                    if not np.array_equal(got[j], big_solo):
+                        return
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(j,)) for j in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    assert np.array_equal(got[0], solo[0]) and np.array_equal(got[1], solo[1])
+    assert np.array_equal(got[2], big_solo)
+    big.free()
