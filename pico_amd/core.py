@@ -790,15 +790,14 @@ def fri_answer_queries(data, indices, log_max):
     check(lib().pg_fri_answer_queries(data.ctx.handle, data.handle, _ptr(idx), nq, _ptr(sib),
                                       _ptr(paths)))
     sib = from_monty(data.field, sib[:, :R])
-    paths = from_monty(data.field, paths[:nq * per_q]).reshape(nq, per_q) if nq else paths[:0]
-    out = []
-    for q in range(nq):
-        lst, off = [], 0
-        for L in plens:
-            lst.append(paths[q, off:off + 8 * L].reshape(L, 8))
-            off += 8 * L
-        out.append(lst)
-    return sib, out
+    if not nq:
+        return sib, []
+    paths = from_monty(data.field, paths[:nq * per_q]).reshape(nq, per_q)
+    per_round, off = [], 0  # one (nq, L, 8) view per round
+    for L in plens:
+        per_round.append(paths[:, off:off + 8 * L].reshape(nq, L, 8))
+        off += 8 * L
+    return sib, [[pr[q] for pr in per_round] for q in range(nq)]
 
 
 __all__ = [
