@@ -12,6 +12,7 @@ where the reference panics.
 """
 import ctypes
 import os
+import time
 
 import numpy as np
 
@@ -515,9 +516,12 @@ class TwoAdicFriPcs:
         npts_a = np.ascontiguousarray(npts, dtype=np.uint32)
         pts_a = np.ascontiguousarray(to_monty(f, np.concatenate(pts) if pts else np.zeros((0, 4), np.uint64)))
         h = ctypes.c_void_p()
+        t0 = time.perf_counter()
         check(lib().pg_pcs_open(self.ctx.handle, trees, len(rounds), _ptr(npts_a), _ptr(pts_a), self.log_blowup,
                                 -1 if log_final is None else int(log_final), int(num_queries), int(pow_bits),
                                 ctypes.byref(challenger.c), ctypes.byref(h)))
+        # duration of the native call alone (the unpacking below is Python's)
+        self.last_open_seconds = time.perf_counter() - t0
         try:
             sz = [ctypes.c_size_t() for _ in range(5)]
             nr = ctypes.c_int()

@@ -129,10 +129,22 @@ def shard_commit_fri(ctx, shapes=None, log_fri=22, log_stop=6, queries=100, pow_
         "reduced_heights": sorted(ro),
         "fri_rounds": fri.num_rounds, "pow_witness": witness,
     })
+    # The whole TwoAdicFriPcs::open of the main round as one pg_pcs_open call
+    # (what a Rust adapter makes): values of the 14 matrices at zeta and
+    # g zeta, observed; alpha; reduced openings; FRI down to the blowup;
+    # PoW; the queries and every opening they need.
+    fri.free()
+    fri = None
+    ch2 = core.DuplexChallenger(f)
+    ch2.observe(root)
+    t6 = time.perf_counter()
+    _, proof = pcs.open([(main, [pts2] * len(shapes))], ch2, queries, pow_bits)
+    t["pcs_open_main_round_ms"] = pcs.last_open_seconds * 1e3
+    t["pcs_open_main_round_with_unpacking_ms"] = (time.perf_counter() - t6) * 1e3
+    t["pcs_open_fri_rounds"] = len(proof["commits"])
     main.free()
     quot.free()
     q.free()
-    fri.free()
     return t
 
 
