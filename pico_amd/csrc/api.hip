@@ -1558,9 +1558,17 @@ static Ext4 bary_factor(const Ext4& z, uint32_t s_mont, int log_h) {
     return escale<F>(zh, finv<F>(den));
 }
 
+// Barycentric weights depend only on the trace height and the points: the
+// matrices of one Pcs::open that share both (every matrix of a round's
+// height, opened at zeta and its own next point) share one computation.
+// Key: log height, point count, the points' words.
+struct WeightCache {
+    std::map<std::vector<uint32_t>, std::shared_ptr<DevMem>> bufs;
+};
+
 template <class F>
 static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k, uint32_t log_blowup,
-                         const uint32_t* points, size_t npts, uint32_t* out) {
+                         const uint32_t* points, size_t npts, uint32_t* out, WeightCache* wc = nullptr) {
     const pg_buf* m = t->mats[k];
     if (!is_pow2(m->rows) || (m->rows >> log_blowup) == 0 || log2_of(m->rows) < (int)log_blowup)
         return fail(PG_E_INVALID, "matrix %zu: height %zu has no trace coset at log_blowup %u", k, m->rows,
@@ -1580,7 +1588,7 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
     const size_t q_max = std::min(npts, per);
     // column sums in slices of about 64 chunks (at most 64 slices)
     const uint32_t nslices = std::max<uint32_t>(1, std::min<uint32_t>(64, nchunks / 64));
-    const size_t b_w = q_max * h * 16, b_p = q_max * nchunks * w * 16, b_s = q_max * nslices * w * 16,
+    const size_t b_w = wc ? 0 : q_max * h * 16, b_p = q_max * nchunks * w * 16, b_s = q_max * nslices * w * 16,
                  b_o = npts * w * 16;
     int rc;
     auto scratch = make_devmem(ctx, b_w + b_p + b_s + b_o, &rc);
@@ -1595,9 +1603,25 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
             z[q] = ext_from<F>(points + 4 * (q0 + q));
             fac[q] = bary_factor<F>(z[q], s_mont, log_h);
         }
+        Ext4* weights = (Ext4*)base;
+        int ready = 0;
+        if (wc) {
+            std::vector<uint32_t> key{(uint32_t)log_h, (uint32_t)nq};
+            for (int q = 0; q < nq; q++) key.insert(key.end(), z[q].c, z[q].c + 4);
+            auto it = wc->bufs.find(key);
+            if (it != wc->bufs.end()) {
+                weights = (Ext4*)it->second->ptr;
+                ready = 1;
+            } else {
+                auto wb = make_devmem(ctx, (size_t)nq * h * 16, &rc);
+                if (!wb) return rc;
+                weights = (Ext4*)wb->ptr;
+                wc->bufs.emplace(std::move(key), std::move(wb));
+            }
+        }
         TIMED(kt, "open_values", s,
               launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, (const uint32_t*)tw->ptr, z, fac, nq,
-                                 (Ext4*)base, (Ext4*)(base + b_w), nchunks, chunk_rows,
+                                 weights, ready, (Ext4*)(base + b_w), nchunks, chunk_rows,
                                  (Ext4*)(base + b_w + b_p), nslices,
                                  (Ext4*)(base + b_w + b_p + b_s + q0 * w * 16), s));
     }
@@ -1606,14 +1630,20 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
     return PG_OK;
 }
 
-extern "C" int pg_pcs_open_values(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t log_blowup,
-                                  const uint32_t* points_mont, size_t npts, uint32_t* out_mont) {
+static int open_values_cached(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t log_blowup,
+                              const uint32_t* points_mont, size_t npts, uint32_t* out_mont, WeightCache* wc) {
     if (!ctx || !t || (npts && (!points_mont || !out_mont)) || k >= t->mats.size())
         return fail(PG_E_INVALID, "bad argument");
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
-    return t->field == PG_KOALABEAR ? open_values_t<KoalaBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont)
-                                    : open_values_t<BabyBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont);
+    return t->field == PG_KOALABEAR
+               ? open_values_t<KoalaBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont, wc)
+               : open_values_t<BabyBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont, wc);
+}
+
+extern "C" int pg_pcs_open_values(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t log_blowup,
+                                  const uint32_t* points_mont, size_t npts, uint32_t* out_mont) {
+    return open_values_cached(ctx, t, k, log_blowup, points_mont, npts, out_mont, nullptr);
 }
 
 template <class F>
@@ -2626,15 +2656,18 @@ extern "C" int pg_pcs_open(pg_ctx* ctx, const pg_tree* const* rounds, size_t nro
     pf->values.resize(nvals);
     size_t voff = 0, poff = 0;
     mi = 0;
-    for (size_t r = 0; r < nrounds; r++)
-        for (size_t k = 0; k < rounds[r]->mats.size(); k++, mi++) {
-            const uint32_t np = npoints[mi];
-            if (np)
-                PG_TRY(pg_pcs_open_values(ctx, rounds[r], k, log_blowup, points_mont + 4 * poff, np,
-                                          pf->values.data() + voff));
-            voff += (size_t)np * rounds[r]->mats[k]->cols * 4;
-            poff += np;
-        }
+    {
+        WeightCache wc;  // freed once every value is on the host (each call synchronises)
+        for (size_t r = 0; r < nrounds; r++)
+            for (size_t k = 0; k < rounds[r]->mats.size(); k++, mi++) {
+                const uint32_t np = npoints[mi];
+                if (np)
+                    PG_TRY(open_values_cached(ctx, rounds[r], k, log_blowup, points_mont + 4 * poff, np,
+                                              pf->values.data() + voff, &wc));
+                voff += (size_t)np * rounds[r]->mats[k]->cols * 4;
+                poff += np;
+            }
+    }
     for (uint32_t v : pf->values) pg_challenger_observe(ch, v);
     // 2. alpha, reduced openings per height
     pg_challenger_sample_ext(ch, pf->alpha);

@@ -147,8 +147,8 @@ int lde_max_log_h();
 bool open_values_vec4(const uint32_t* m, uint64_t ld, uint32_t w, int npts);
 hipError_t launch_open_values(int field, const uint32_t* m, uint64_t ld, uint32_t w, int log_h,
                               const uint32_t* tw, const Ext4* z, const Ext4* factor, int npts, Ext4* weights,
-                              Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* sums, uint32_t nslices,
-                              Ext4* out, hipStream_t stream);
+                              int weights_ready, Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* sums,
+                              uint32_t nslices, Ext4* out, hipStream_t stream);
 // Reduced opening of one LDE log-height (see open.hip).  mats / pts are
 // device tables; alpha_pows[k] = alpha^k for k < the widest matrix.
 constexpr int RO_MAX_POINTS = 4;  // distinct points per log-height

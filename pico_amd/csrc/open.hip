@@ -335,13 +335,14 @@ static unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 
 template <class F>
 static hipError_t open_values_t(const uint32_t* m, uint64_t ld, uint32_t w, int log_h, const uint32_t* tw,
-                                const Ext4* z, const Ext4* factor, int npts, Ext4* weights, Ext4* partial,
-                                uint32_t nchunks, uint32_t chunk_rows, Ext4* sums, uint32_t nslices, Ext4* out,
-                                hipStream_t s) {
+                                const Ext4* z, const Ext4* factor, int npts, Ext4* weights, int weights_ready,
+                                Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* sums, uint32_t nslices,
+                                Ext4* out, hipStream_t s) {
     const uint32_t h = 1u << log_h;
-    for (int q = 0; q < npts; q++)
-        bary_weights_kernel<F><<<blocks_for(h, 256), 256, 0, s>>>(z[q], to_monty<F>(F::GEN), log_h, (gcptr)tw,
-                                                                    (gext)(weights + (uint64_t)q * h));
+    if (!weights_ready)  // else computed for these points and this height by an earlier call
+        for (int q = 0; q < npts; q++)
+            bary_weights_kernel<F><<<blocks_for(h, 256), 256, 0, s>>>(z[q], to_monty<F>(F::GEN), log_h, (gcptr)tw,
+                                                                        (gext)(weights + (uint64_t)q * h));
     const uint64_t nt = (uint64_t)nchunks * w;
     if (open_values_vec4(m, ld, w, npts)) {
         const uint64_t nt4 = nt / 4;
@@ -377,16 +378,16 @@ bool open_values_vec4(const uint32_t* m, uint64_t ld, uint32_t w, int npts) {
 
 hipError_t launch_open_values(int field, const uint32_t* m, uint64_t ld, uint32_t w, int log_h,
                               const uint32_t* tw, const Ext4* z, const Ext4* factor, int npts, Ext4* weights,
-                              Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* sums, uint32_t nslices,
-                              Ext4* out, hipStream_t stream) {
+                              int weights_ready, Ext4* partial, uint32_t nchunks, uint32_t chunk_rows, Ext4* sums,
+                              uint32_t nslices, Ext4* out, hipStream_t stream) {
     if (w == 0 || nchunks == 0 || npts <= 0) return hipSuccess;
     if (nslices == 0 || nslices > nchunks || nslices > 65535) return hipErrorInvalidValue;
     if (npts > OV_MAX_POINTS) return hipErrorInvalidValue;
     return field == FIELD_KOALABEAR
-               ? open_values_t<KoalaBear>(m, ld, w, log_h, tw, z, factor, npts, weights, partial, nchunks,
-                                          chunk_rows, sums, nslices, out, stream)
-               : open_values_t<BabyBear>(m, ld, w, log_h, tw, z, factor, npts, weights, partial, nchunks,
-                                         chunk_rows, sums, nslices, out, stream);
+               ? open_values_t<KoalaBear>(m, ld, w, log_h, tw, z, factor, npts, weights, weights_ready, partial,
+                                          nchunks, chunk_rows, sums, nslices, out, stream)
+               : open_values_t<BabyBear>(m, ld, w, log_h, tw, z, factor, npts, weights, weights_ready, partial,
+                                         nchunks, chunk_rows, sums, nslices, out, stream);
 }
 
 hipError_t launch_reduce_rows(int field, const RoMat* mats, int nmats, const Ext4* pts, int npts,
