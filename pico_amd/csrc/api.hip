@@ -12,6 +12,9 @@
 //                      reduced opening of the new length (circuit/fri.rs:36-73,
 //                      :288-387); PoW check_witness (circuit/challenger.rs:210-222)
 #include <hip/hip_runtime.h>
+#ifndef __HIP_DEVICE_COMPILE__
+#include <immintrin.h>
+#endif
 
 #include <algorithm>
 #include <array>
@@ -1183,6 +1186,125 @@ extern "C" int pg_tree_digest_layer(pg_ctx* ctx, const pg_tree* t, int layer, ui
     return PG_OK;
 }
 
+// ---------------------------------------------- host Poseidon2 (AVX2) --
+// The challenger duplexes on the host (every opened value of Pcs::open is
+// observed: ~2200 permutations for a 14-matrix shard), and host verification
+// hashes too.  One width-16 state is two 8-lane vectors: s0..s7 and s8..s15,
+// each 128-bit half one block of the external layer.  Montgomery products
+// take the even and odd lanes through vpmuludq; M4 is in-lane shuffles
+// (y_i = (x_0+x_1+x_2+x_3) + x_i + 2 x_{i+1}), the sum over blocks one
+// 128-bit swap.  All values stay reduced, so the result is the permutation
+// word for word (the CPU tests compare the challenger and verify_batch with
+// the oracle).  Used when the CPU has AVX2, else the scalar permute.
+// Measured per permutation: 0.49 against 0.54 us on an MI355X host's CPU,
+// 0.92 against 1.67 us on a slower build host; the 20 partial rounds, a
+// dependent chain through lane 0, bound both.
+#ifndef __HIP_DEVICE_COMPILE__
+namespace hostp2 {
+#define PG_AVX2 __attribute__((target("avx2"))) static inline
+template <class F>
+struct C {
+    __m256i p, mu, diag_lo, diag_hi, rc_lo[8], rc_hi[8];
+    __attribute__((target("avx2"))) void init() {
+        p = _mm256_set1_epi32((int)F::P);
+        mu = _mm256_set1_epi32((int)F::MU);  // -p^-1 mod 2^32
+        diag_lo = _mm256_loadu_si256((const __m256i*)&P2<F>::K.diag[0]);
+        diag_hi = _mm256_loadu_si256((const __m256i*)&P2<F>::K.diag[8]);
+        for (int r = 0; r < 8; r++) {
+            rc_lo[r] = _mm256_loadu_si256((const __m256i*)&P2<F>::K.ext_rc[r][0]);
+            rc_hi[r] = _mm256_loadu_si256((const __m256i*)&P2<F>::K.ext_rc[r][8]);
+        }
+    }
+};
+PG_AVX2 __m256i add(__m256i a, __m256i b, __m256i p) {
+    const __m256i r = _mm256_add_epi32(a, b);
+    return _mm256_min_epu32(r, _mm256_sub_epi32(r, p));
+}
+// a b 2^-32 mod p for a, b < p: t = a b; m = t (-p^-1) mod 2^32; (t + m p) / 2^32 < 2p
+PG_AVX2 __m256i mul(__m256i a, __m256i b, __m256i p, __m256i neg_pinv) {
+    __m256i te = _mm256_mul_epu32(a, b);
+    __m256i to = _mm256_mul_epu32(_mm256_srli_epi64(a, 32), _mm256_srli_epi64(b, 32));
+    const __m256i me = _mm256_mul_epu32(_mm256_mul_epu32(te, neg_pinv), p);
+    const __m256i mo = _mm256_mul_epu32(_mm256_mul_epu32(to, neg_pinv), p);
+    te = _mm256_srli_epi64(_mm256_add_epi64(te, me), 32);
+    to = _mm256_add_epi64(to, mo);
+    const __m256i r = _mm256_blend_epi32(te, to, 0xAA);
+    return _mm256_min_epu32(r, _mm256_sub_epi32(r, p));
+}
+template <class F>
+PG_AVX2 __m256i sbox(__m256i x, const C<F>& c) {
+    const __m256i x2 = mul(x, x, c.p, c.mu), x3 = mul(x2, x, c.p, c.mu);
+    if constexpr (F::SBOX == 3) return x3;
+    return mul(mul(x3, x3, c.p, c.mu), x, c.p, c.mu);
+}
+template <class F>
+PG_AVX2 __m256i m4(__m256i x, const C<F>& c) {
+    const __m256i a = add(x, _mm256_shuffle_epi32(x, _MM_SHUFFLE(2, 3, 0, 1)), c.p);
+    const __m256i s4 = add(a, _mm256_shuffle_epi32(a, _MM_SHUFFLE(1, 0, 3, 2)), c.p);
+    const __m256i nx = _mm256_shuffle_epi32(x, _MM_SHUFFLE(0, 3, 2, 1));
+    return add(add(s4, x, c.p), add(nx, nx, c.p), c.p);
+}
+template <class F>
+PG_AVX2 void external(__m256i& lo, __m256i& hi, const C<F>& c) {
+    lo = m4<F>(lo, c);
+    hi = m4<F>(hi, c);
+    __m256i t = add(lo, hi, c.p);
+    t = add(t, _mm256_permute2x128_si256(t, t, 0x01), c.p);
+    lo = add(lo, t, c.p);
+    hi = add(hi, t, c.p);
+}
+template <class F>
+PG_AVX2 void internal(__m256i& lo, __m256i& hi, const C<F>& c) {
+    __m256i t = add(lo, hi, c.p);
+    t = add(t, _mm256_shuffle_epi32(t, _MM_SHUFFLE(2, 3, 0, 1)), c.p);
+    t = add(t, _mm256_shuffle_epi32(t, _MM_SHUFFLE(1, 0, 3, 2)), c.p);
+    t = add(t, _mm256_permute2x128_si256(t, t, 0x01), c.p);
+    lo = add(mul(lo, c.diag_lo, c.p, c.mu), t, c.p);
+    hi = add(mul(hi, c.diag_hi, c.p, c.mu), t, c.p);
+}
+template <class F>
+__attribute__((target("avx2"))) static void permute_avx2(uint32_t st[16]) {
+    static const C<F> c = [] {
+        C<F> v;
+        v.init();
+        return v;
+    }();
+    __m256i lo = _mm256_loadu_si256((const __m256i*)st), hi = _mm256_loadu_si256((const __m256i*)(st + 8));
+    external<F>(lo, hi, c);
+    for (int r = 0; r < 4; r++) {
+        lo = sbox<F>(add(lo, c.rc_lo[r], c.p), c);
+        hi = sbox<F>(add(hi, c.rc_hi[r], c.p), c);
+        external<F>(lo, hi, c);
+    }
+    for (int r = 0; r < F::ROUNDS_P; r++) {
+        uint32_t s0 = (uint32_t)_mm256_cvtsi256_si32(lo);
+        s0 = fadd<F>(s0, P2<F>::K.int_rc[r]);
+        s0 = F::SBOX == 3 ? fmul<F>(fmul<F>(s0, s0), s0)
+                          : fmul<F>(fmul<F>(fmul<F>(s0, s0), s0), fmul<F>(fmul<F>(fmul<F>(s0, s0), s0), s0));
+        lo = _mm256_blend_epi32(lo, _mm256_castsi128_si256(_mm_cvtsi32_si128((int)s0)), 0x01);
+        internal<F>(lo, hi, c);
+    }
+    for (int r = 4; r < 8; r++) {
+        lo = sbox<F>(add(lo, c.rc_lo[r], c.p), c);
+        hi = sbox<F>(add(hi, c.rc_hi[r], c.p), c);
+        external<F>(lo, hi, c);
+    }
+    _mm256_storeu_si256((__m256i*)st, lo);
+    _mm256_storeu_si256((__m256i*)(st + 8), hi);
+}
+#undef PG_AVX2
+}  // namespace hostp2
+#endif
+
+template <class F>
+static void host_permute(uint32_t (&st)[16]) {
+#ifndef __HIP_DEVICE_COMPILE__
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    if (avx2) return hostp2::permute_avx2<F>(st);
+#endif
+    permute<F>(st);
+}
+
 // host sponge / compress for verify_batch and the challenger
 template <class F>
 static void host_sponge(const uint32_t* in, size_t n, uint32_t out[8]) {
@@ -1190,7 +1312,7 @@ static void host_sponge(const uint32_t* in, size_t n, uint32_t out[8]) {
     for (size_t o = 0; o < n; o += 8) {
         const size_t len = std::min<size_t>(8, n - o);
         memcpy(st, in + o, len * 4);
-        permute<F>(st);
+        host_permute<F>(st);
     }
     memcpy(out, st, 32);
 }
@@ -1199,7 +1321,7 @@ static void host_compress(const uint32_t* l, const uint32_t* r, uint32_t out[8])
     uint32_t st[16];
     memcpy(st, l, 32);
     memcpy(st + 8, r, 32);
-    permute<F>(st);
+    host_permute<F>(st);
     memcpy(out, st, 32);
 }
 
@@ -2389,7 +2511,7 @@ template <class F>
 static void ch_duplex(pg_challenger* c) {
     for (int i = 0; i < c->n_input; i++) c->state[i] = c->input[i];
     c->n_input = 0;
-    permute<F>(c->state);
+    host_permute<F>(c->state);
     memcpy(c->output, c->state, 32);
     c->n_output = 8;
 }
