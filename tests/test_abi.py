@@ -62,6 +62,22 @@ def test_host_challenger_matches_oracle(oracle, f):
 
 
 @pytest.mark.parametrize("f", [0, 1])
+def test_host_challenger_long_stream_matches_oracle(oracle, f):
+    """About 1300 host permutations over states the observed values make
+    random: every sample equals the oracle's (the host permutation runs in
+    AVX2 where the CPU has it)."""
+    ch = pico_amd.DuplexChallenger(f)
+    oc = oracle.Challenger(f)
+    rng = np.random.default_rng(17 + f)
+    p = pico_amd.PRIMES[f]
+    for step in range(200):
+        for v in rng.integers(0, p, int(rng.integers(1, 60))):
+            ch.observe(int(v))
+            oc.observe(int(v))
+        assert list(ch.sample_ext()) == list(oc.sample_ext()), step
+
+
+@pytest.mark.parametrize("f", [0, 1])
 def test_host_verify_batch_matches_oracle(oracle, f):
     shape = [(8, 3), (8, 9), (4, 8), (1, 2)]
     mats = [oracle.random_matrix(f, 300 + i, h, w) for i, (h, w) in enumerate(shape)]
