@@ -486,22 +486,35 @@ static int stage_download(pg_ctx* ctx, hipStream_t s, void* dst, const void* src
 // when both sides are contiguous, else groups of rows gathered on the host
 // and written with a pitched DMA.  Returns once the last DMA has read its
 // buffer (the copies into dst are complete by then too).
+// A stage kept in use across several uploads on one stream: the buffers keep
+// rotating and nothing drains between the uploads, so the gather of the next
+// upload's first chunk overlaps the DMA of the previous upload's last one
+// (pg_pcs_commit_host uploads a matrix as several column blocks).
+// stage_run_finish waits for the last DMAs and returns the stage.
+struct StageRun {
+    Stage* st = nullptr;
+    size_t k = 0;
+    bool used[2] = {false, false};
+};
+
 static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, const void* src, size_t spitch,
-                        size_t width, size_t rows) {
+                        size_t width, size_t rows, StageRun* run = nullptr) {
     const bool dense = dpitch == width && spitch == width;
     if (!dense && width > Stage::CHUNK) {
         HIP_TRY(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyHostToDevice, s));
         return PG_OK;
     }
-    Stage* st;
-    PG_TRY(stage_get(ctx, &st));
+    StageRun local;
+    StageRun& R = run ? *run : local;
+    if (!R.st) PG_TRY(stage_get(ctx, &R.st));
+    Stage* st = R.st;
     HostPool& pool = host_pool();
     const int nt = pool.size();
     const size_t total = dense ? width * rows : rows;  // bytes, or rows
     const size_t step = dense ? Stage::CHUNK : Stage::CHUNK / width;
-    bool used[2] = {false, false};
+    bool* used = R.used;
     hipError_t e = hipSuccess;
-    size_t k = 0;
+    size_t k = R.k;
     for (size_t done = 0; done < total && e == hipSuccess; done += step, k++) {
         const int b = (int)(k & 1);
         if (used[b]) e = hipEventSynchronize(st->ev[b]);  // its previous DMA is done
@@ -530,11 +543,14 @@ static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, co
         if (e == hipSuccess) e = hipEventRecord(st->ev[b], s);
         used[b] = e == hipSuccess;
     }
+    R.k = k;
+    if (e == hipSuccess && run) return PG_OK;  // the run goes on; its owner finishes it
     for (int b = 0; b < 2; b++)
         if (used[b]) {
             const hipError_t e2 = hipEventSynchronize(st->ev[b]);
             if (e == hipSuccess) e = e2;
         }
+    R.st = nullptr;
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(s);  // nothing may still read the buffers
         stage_destroy(st);
@@ -542,6 +558,20 @@ static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, co
     }
     stage_put(ctx, st);
     return PG_OK;
+}
+
+// Waits for a run's last DMAs and returns its stage (destroyed when a wait
+// fails).  Safe on a run that never started or already ended.
+static void stage_run_finish(pg_ctx* ctx, StageRun* run) {
+    if (!run->st) return;
+    bool ok = true;
+    for (int b = 0; b < 2; b++)
+        if (run->used[b] && hipEventSynchronize(run->st->ev[b]) != hipSuccess) ok = false;
+    if (ok)
+        stage_put(ctx, run->st);
+    else
+        stage_destroy(run->st);
+    run->st = nullptr;
 }
 
 extern "C" int pg_create(int device, size_t budget, pg_ctx** out) {
@@ -1447,9 +1477,11 @@ extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const*
     std::shared_ptr<DevMem> scratch;
     std::vector<hipEvent_t> events;
     bool committed = false;
+    StageRun run;  // every staged upload of this call, on lc.s
     auto cleanup = [&]() {  // after an error some work may still be queued
         (void)hipStreamSynchronize(lc.s);
         (void)hipStreamSynchronize(l.s);
+        stage_run_finish(ctx, &run);
         for (auto* b : in) pg_buf_free(b);
         if (!committed)
             for (auto* b : ldes) pg_buf_free(b);
@@ -1514,9 +1546,10 @@ extern "C" int pg_pcs_commit_host(pg_ctx* ctx, int field, const uint32_t* const*
     auto block_ptr = [&](size_t k, size_t c0) { return in[k]->ptr + rows[k] * c0; };
     auto upload_wait = [&](size_t k, size_t c0, size_t cb) -> int {
         if (cb != cols[k]) {  // row segments gathered into a stage
-            PG_TRY(stage_upload(ctx, lc.s, block_ptr(k, c0), cb * 4, host[k] + c0, cols[k] * 4, cb * 4, rows[k]));
+            PG_TRY(stage_upload(ctx, lc.s, block_ptr(k, c0), cb * 4, host[k] + c0, cols[k] * 4, cb * 4, rows[k],
+                                &run));
         } else if (!locked[k] && in[k]->bytes >= STAGE_MIN_BYTES) {  // pageable: through a stage
-            PG_TRY(stage_upload(ctx, lc.s, in[k]->ptr, in[k]->bytes, host[k], in[k]->bytes, in[k]->bytes, 1));
+            PG_TRY(stage_upload(ctx, lc.s, in[k]->ptr, in[k]->bytes, host[k], in[k]->bytes, in[k]->bytes, 1, &run));
         } else {
             HIP_TRY(hipMemcpyAsync(in[k]->ptr, host[k], in[k]->bytes, hipMemcpyHostToDevice, lc.s));
         }
