@@ -388,13 +388,17 @@ static HostPool& host_pool() {  // lives until the process ends
 
 struct Stage {
     static constexpr size_t CHUNK = 16u << 20;
-    void* buf[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    // uploads rotate through NB buffers, downloads use two; 3 or 4 measured
+    // no faster than 2 (2^20 x 256 from host memory: 22.7-24.8 ms for each)
+    static constexpr int NB = 2;
+    static_assert(NB >= 2, "double buffering at least");
+    void* buf[NB] = {};
+    hipEvent_t ev[NB] = {};
 };
 static constexpr size_t STAGE_MIN_BYTES = 4u << 20;  // smaller uploads go direct
 
 static void stage_destroy(Stage* st) {
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < Stage::NB; b++) {
         if (st->buf[b]) (void)hipHostFree(st->buf[b]);
         if (st->ev[b]) (void)hipEventDestroy(st->ev[b]);
     }
@@ -411,7 +415,7 @@ static int stage_get(pg_ctx* ctx, Stage** out) {
         }
     }
     Stage* st = new Stage;
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < Stage::NB; b++) {
         hipError_t e = hipHostMalloc(&st->buf[b], Stage::CHUNK, hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&st->ev[b], hipEventDisableTiming);
         if (e != hipSuccess) {
@@ -494,7 +498,7 @@ static int stage_download(pg_ctx* ctx, hipStream_t s, void* dst, const void* src
 struct StageRun {
     Stage* st = nullptr;
     size_t k = 0;
-    bool used[2] = {false, false};
+    bool used[Stage::NB] = {};
 };
 
 static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, const void* src, size_t spitch,
@@ -516,7 +520,7 @@ static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, co
     hipError_t e = hipSuccess;
     size_t k = R.k;
     for (size_t done = 0; done < total && e == hipSuccess; done += step, k++) {
-        const int b = (int)(k & 1);
+        const int b = (int)(k % Stage::NB);
         if (used[b]) e = hipEventSynchronize(st->ev[b]);  // its previous DMA is done
         if (e != hipSuccess) break;
         const size_t n = std::min(step, total - done);
@@ -545,7 +549,7 @@ static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, co
     }
     R.k = k;
     if (e == hipSuccess && run) return PG_OK;  // the run goes on; its owner finishes it
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < Stage::NB; b++)
         if (used[b]) {
             const hipError_t e2 = hipEventSynchronize(st->ev[b]);
             if (e == hipSuccess) e = e2;
@@ -565,7 +569,7 @@ static int stage_upload(pg_ctx* ctx, hipStream_t s, void* dst, size_t dpitch, co
 static void stage_run_finish(pg_ctx* ctx, StageRun* run) {
     if (!run->st) return;
     bool ok = true;
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < Stage::NB; b++)
         if (run->used[b] && hipEventSynchronize(run->st->ev[b]) != hipSuccess) ok = false;
     if (ok)
         stage_put(ctx, run->st);
