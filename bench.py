@@ -28,10 +28,11 @@ TEXTBOOK_OPS_PER_PERM = 616 * 5 + 1416 * 3
 # 39.3 T lane-slots/s (the published 157.3 TFLOP/s FP32 vector peak is this
 # times 2 for FMA times 2 for packed FP32, which integer code cannot use).
 # v_add_u32 / v_sub_u32 / v_mov_b32 / v_and_b32 issue up to twice per slot and
-# are priced at half a slot (pico_amd/probes/valu_rates.hip).
+# are priced at half a slot (measured below).
 VALU_PEAK_TSLOTS = 256 * 4 * 16 * 2.4e9 / 1e12
-# per-opcode rates measured by pico_amd/probes/valu_rates.hip on MI355X
-# (T lane-ops/s); an opcode not listed is taken at the single-issue rate
+# per-opcode rates measured on MI355X by a throughput microbenchmark (long
+# independent chains of one opcode over a full grid; T lane-ops/s); an
+# opcode not listed is taken at the single-issue rate
 MEASURED_RATES = {"v_add_u32": 64.2, "v_sub_u32": 63.7, "v_mov_b32": 69.0, "v_min_u32": 37.7,
                   "v_alignbit_b32": 38.4, "v_lshlrev_b32": 38.4, "v_and_b32": 69.5, "v_add3_u32": 37.9,
                   "v_mad_u32_u24": 38.1, "v_mul_lo_u32": 36.2, "v_mul_hi_u32": 38.5, "v_mad_u64_u32": 35.2,
@@ -104,6 +105,19 @@ def cpu_baseline(log_h, w, lb, budget_cols, leaf_rows):
                    f"all {perms_full} permutations priced at the measured {perm_rate:.3g} perm/s"),
         "est_job_seconds": t_total,
     }
+
+
+def golden_root_check(h, w, lb, root):
+    """True/False against the oracle's root of the same seeded job
+    (tests/golden/headline_kb_2p20x256.json); None for another shape."""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "headline_kb_2p20x256.json")) as fh:
+            g = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if (g["rows"], g["cols"], g["log_blowup"], g["seed"]) != (h, w, lb, (2 << 32) | 0):
+        return None
+    return [int(x) for x in root] == g["root"]
 
 
 def progress(msg):
@@ -248,6 +262,7 @@ def main():
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
         "lde_tuning": os.environ.get("PICO_AMD_LDE_TUNING", os.path.relpath(pico_amd.TUNING_FILE, ROOT)),
         "root": [int(x) for x in root],
+        "root_matches_golden": golden_root_check(h, w, lb, root),
     }
     if not args.commit_only:
         # End to end: the trace starts in (pageable) host memory, is copied to the

@@ -48,13 +48,16 @@ uint32_t or_sub(int f, uint32_t a, uint32_t b) { return fsub(P_OF(f), a, b); }
 uint32_t or_mul(int f, uint32_t a, uint32_t b) { return fmul(P_OF(f), a, b); }
 uint32_t or_pow(int f, uint32_t a, uint64_t e) { return fpow(P_OF(f), a, e); }
 uint32_t or_inv(int f, uint32_t a) { return fpow(P_OF(f), a, P_OF(f) - 2); }
-/* Multiplicative generators: the smallest primitive roots, 3 (KoalaBear) and
- * 31 (BabyBear).  Not pinned by any file in the reference (the p3 constant is
- * KoalaBear::GENERATOR / BabyBear::GENERATOR); the tests check primitivity. */
+/* Multiplicative generators 3 (KoalaBear) and 31 (BabyBear): the recursion
+ * builder's `generator()` (vm/src/compiler/recursion/ir/utils.rs:13-19), the
+ * g of the verifier's x = g w^rev(index) (circuit/fri.rs:151-171).  The tests
+ * also check that both are primitive roots. */
 uint32_t or_generator(int f) { return f == OR_KOALABEAR ? 3u : 31u; }
 int or_two_adicity(int f) { return f == OR_KOALABEAR ? 24 : 27; }
 /* two_adic_generator(k) = GEN^((p-1) >> k): order exactly 2^k, and
- * two_adic_generator(k)^2 = two_adic_generator(k-1). */
+ * two_adic_generator(k)^2 = two_adic_generator(k-1).  Which primitive 2^k-th
+ * root p3 uses (its TWO_ADIC_GENERATORS table) is not in any file of the
+ * reference: parity unpinned beyond these properties. */
 uint32_t or_two_adic_generator(int f, int log_n) {
     uint32_t p = P_OF(f);
     return fpow(p, or_generator(f), (uint64_t)(p - 1) >> log_n);
@@ -559,6 +562,106 @@ int or_mmcs_verify(int f, const size_t *heights, const size_t *widths, size_t n,
     free(buf);
     free(start);
     return memcmp(cur, root, 32) == 0;
+}
+
+/* ------------------------------------------------ seeded PCS commit root -- */
+/* TwoAdicFriPcs::commit of seeded matrices (matrix k = or_fill_random(seed k),
+ * h_k x w_k, on the domain s_k <w_h>) computed without holding any LDE whole,
+ * so that commitments of tens of GB of LDE fit in host memory:
+ *   - LDE shift GEN / s_k, rows bit-reversed (as or_coset_lde);
+ *   - the leaves of LDE height N hash the concatenated rows of every matrix of
+ *     that height in original order (the stable height-descending order of
+ *     MerkleTreeMmcs restricted to one height).  The overwrite-mode sponge is
+ *     run incrementally: a row's state and its position in the current chunk
+ *     are kept while the matrices' LDEs are produced `block_cols` columns at a
+ *     time; each input overwrites state[pos], a full chunk permutes, and a
+ *     partial last chunk permutes once at the end -- the same calls as
+ *     or_sponge on the whole concatenated row;
+ *   - layers as or_mmcs_commit: next[i] = C(prev[2i], prev[2i+1]), then
+ *     C(next[i], leaf digest of height len(next)) when that height exists.
+ * Pinned against or_mmcs_commit of or_coset_lde outputs (tests). */
+int or_pcs_commit_seeded(int f, size_t n, const uint64_t *seeds, const size_t *heights,
+                         const size_t *widths, const uint32_t *domain_shifts, int log_blowup,
+                         size_t block_cols, uint32_t root[8]) {
+    uint32_t p = P_OF(f);
+    if (n == 0 || block_cols == 0) return -1;
+    size_t H = 0;
+    for (size_t k = 0; k < n; k++) {
+        if (log2_exact(heights[k]) < 0) return -1;
+        size_t N = heights[k] << log_blowup;
+        if (N > H) H = N;
+    }
+    int lm = log2_exact(H);
+    /* leaves[l]: digests of the rows of LDE height 2^l (NULL when absent) */
+    uint32_t *leaves[64] = {0};
+    for (int l = lm; l >= 0; l--) {
+        size_t N = (size_t)1 << l;
+        int any = 0;
+        for (size_t k = 0; k < n; k++) any |= (heights[k] << log_blowup) == N;
+        if (!any) continue;
+        uint32_t *st = (uint32_t *)calloc(N * 16, sizeof(uint32_t));
+        size_t pos = 0; /* inputs absorbed so far, mod 8: the same for every row */
+        for (size_t k = 0; k < n; k++) {
+            size_t h = heights[k], w = widths[k];
+            if ((h << log_blowup) != N || w == 0) continue;
+            uint32_t *tr = (uint32_t *)malloc(h * w * sizeof(uint32_t));
+            or_fill_random(f, seeds[k], tr, h * w);
+            uint32_t shift = fmul(p, or_generator(f), fpow(p, domain_shifts[k], p - 2));
+            size_t cbmax = block_cols < w ? block_cols : w;
+            uint32_t *blk = (uint32_t *)malloc(h * cbmax * sizeof(uint32_t));
+            uint32_t *lde = (uint32_t *)malloc(N * cbmax * sizeof(uint32_t));
+            for (size_t c0 = 0; c0 < w; c0 += cbmax) {
+                size_t cb = w - c0 < cbmax ? w - c0 : cbmax;
+#pragma omp parallel for schedule(static)
+                for (size_t r = 0; r < h; r++) memcpy(blk + r * cb, tr + r * w + c0, cb * sizeof(uint32_t));
+                or_coset_lde(f, blk, h, cb, log_blowup, shift, 1, lde);
+#pragma omp parallel for schedule(static)
+                for (size_t r = 0; r < N; r++) {
+                    uint32_t *s = st + 16 * r;
+                    size_t q = pos;
+                    for (size_t c = 0; c < cb; c++) {
+                        s[q++] = lde[r * cb + c];
+                        if (q == 8) {
+                            or_poseidon2(f, s);
+                            q = 0;
+                        }
+                    }
+                }
+                pos = (pos + cb) % 8;
+            }
+            free(lde);
+            free(blk);
+            free(tr);
+        }
+        if (pos != 0) {
+#pragma omp parallel for schedule(static)
+            for (size_t r = 0; r < N; r++) or_poseidon2(f, st + 16 * r);
+        }
+        leaves[l] = (uint32_t *)malloc(N * 32);
+        for (size_t r = 0; r < N; r++) memcpy(leaves[l] + 8 * r, st + 16 * r, 32);
+        free(st);
+    }
+    uint32_t *cur = leaves[lm];
+    leaves[lm] = NULL;
+    for (int l = lm - 1; l >= 0; l--) {
+        size_t len = (size_t)1 << l;
+        uint32_t *nx = (uint32_t *)malloc(len * 32);
+        const uint32_t *inj = leaves[l];
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < len; i++) {
+            uint32_t d[8];
+            or_compress(f, cur + 16 * i, cur + 16 * i + 8, d);
+            if (inj) or_compress(f, d, inj + 8 * i, d);
+            memcpy(nx + 8 * i, d, 32);
+        }
+        free(cur);
+        free(leaves[l]);
+        leaves[l] = NULL;
+        cur = nx;
+    }
+    memcpy(root, cur, 32);
+    free(cur);
+    return 0;
 }
 
 /* ------------------------------------------------------------------ FRI -- */

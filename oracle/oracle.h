@@ -29,6 +29,8 @@
  *   or_pcs_open_values    p3 TwoAdicFriPcs::open, evaluation at the opening points
  *   or_pcs_reduce         p3 TwoAdicFriPcs::open, reduced openings (the verifier's
  *                         loop, circuit/fri.rs:118-241, applied to every row)
+ *   or_pcs_commit_seeded  p3 TwoAdicFriPcs::commit (vm/src/machine/prover.rs:238-244)
+ *                         = or_coset_lde + or_mmcs_commit, without holding LDEs
  */
 #ifndef PICO_ORACLE_H
 #define PICO_ORACLE_H
@@ -105,6 +107,15 @@ int or_mmcs_open(const or_matrix *mats, size_t n, const uint32_t *digests, size_
 int or_mmcs_verify(int field, const size_t *heights, const size_t *widths, size_t n,
                    const uint32_t root[8], size_t index, const uint32_t *rows,
                    const uint32_t *path);
+
+/* TwoAdicFriPcs::commit root of seeded matrices: matrix k is or_fill_random(
+ * seeds[k]) of heights[k] x widths[k] on the domain domain_shifts[k] <w_h>
+ * (canonical), LDE shift GEN / domain_shifts[k], one mixed-height tree.  LDEs
+ * are produced block_cols columns at a time and absorbed into per-row sponge
+ * states, so no LDE is held whole.  Returns 0, or -1 on bad shapes. */
+int or_pcs_commit_seeded(int field, size_t n, const uint64_t *seeds, const size_t *heights,
+                         const size_t *widths, const uint32_t *domain_shifts, int log_blowup,
+                         size_t block_cols, uint32_t root[8]);
 
 /* ---- FRI ---- */
 /* v: n ext4 elements (bit-reversed evaluation order), n even; out: n/2 */

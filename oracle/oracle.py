@@ -57,6 +57,7 @@ def lib():
             "or_num_threads": (i, []),
             "or_pcs_open_values": (None, [i, P, sz, sz, i, P, P]),
             "or_pcs_reduce": (None, [i, sz, P, P, P, P, P, P, P, P]),
+            "or_pcs_commit_seeded": (i, [i, sz, P, P, P, P, i, sz, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -202,6 +203,23 @@ def mmcs_verify(field, heights, widths, root, index, rows, path):
     rt = _u32(root)
     return bool(lib().or_mmcs_verify(field, _ptr(hs), _ptr(ws), len(heights), _ptr(rt), index,
                                      _ptr(r), _ptr(pth)))
+
+
+def pcs_commit_seeded(field, seeds, heights, widths, log_blowup, domain_shifts=None, block_cols=64):
+    """Root of TwoAdicFriPcs::commit over seeded matrices (random_matrix(field,
+    seeds[k], heights[k], widths[k]) on domains domain_shifts[k] <w_h>,
+    canonical shifts, default 1), without holding any LDE whole."""
+    n = len(seeds)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    hs = np.ascontiguousarray(heights, dtype=np.uint64).astype(np.uintp)
+    ws = np.ascontiguousarray(widths, dtype=np.uint64).astype(np.uintp)
+    sh = _u32(np.ones(n, np.uint32) if domain_shifts is None else domain_shifts)
+    root = np.zeros(8, np.uint32)
+    rc = lib().or_pcs_commit_seeded(field, n, _ptr(sd), _ptr(hs), _ptr(ws), _ptr(sh), log_blowup,
+                                    block_cols, _ptr(root))
+    if rc != 0:
+        raise ValueError("heights must be powers of two")
+    return root
 
 
 # -------------------------------------------------------------------- FRI --

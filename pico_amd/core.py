@@ -16,7 +16,7 @@ import time
 
 import numpy as np
 
-from ._lib import BABYBEAR, KOALABEAR, Challenger as _ChStruct, PgError, check, lib
+from ._lib import BABYBEAR, KOALABEAR, PG_E_INVALID, Challenger as _ChStruct, PgError, check, lib
 
 PRIMES = {KOALABEAR: 2130706433, BABYBEAR: 2013265921}
 GENERATORS = {KOALABEAR: 3, BABYBEAR: 31}
@@ -453,6 +453,11 @@ class TwoAdicFriPcs:
         return from_monty(self.field, root), ProverData(self.ctx, self.field, h)
 
     def get_evaluations_on_domain(self, data, idx, domain):
+        """Rows of committed matrix `idx` on `domain` (natural order).  As in
+        p3 TwoAdicFriPcs, only domains with shift GENERATOR (a sub-coset of
+        the committed LDE coset of a trace domain) are available."""
+        if domain.shift % PRIMES[self.field] != GENERATORS[self.field]:
+            raise PgError(PG_E_INVALID, "get_evaluations_on_domain: domain shift must be the field generator")
         m = lib().pg_tree_matrix(data.handle, idx)
         cols = lib().pg_buf_cols(m)
         out = np.zeros((domain.size(), cols), np.uint32)

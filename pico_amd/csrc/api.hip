@@ -330,6 +330,18 @@ struct Lease {
     }
 };
 
+// Synchronises a stream when it goes out of scope unless disarmed: declared
+// after the owners of device memory that queued kernels use, it runs first
+// on an early error return, so nothing is freed while still in use.
+struct SyncGuard {
+    hipStream_t s;
+    bool armed = true;
+    explicit SyncGuard(hipStream_t st) : s(st) {}
+    ~SyncGuard() {
+        if (armed) (void)hipStreamSynchronize(s);
+    }
+};
+
 static int bind_device(pg_ctx* ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
     return PG_OK;
@@ -1107,6 +1119,7 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
     std::vector<size_t> order;
     MatRef* d_sorted = nullptr;
     PG_TRY(tree_prepare(ctx, s, field, mats, &t, &order, &d_sorted));
+    SyncGuard guard(s);  // on an error below: wait for queued kernels before `t` frees
     const size_t H = t->H;
     // layer 0: every matrix of height H
     size_t g0 = 0, g1 = 0;
@@ -1148,6 +1161,7 @@ static int mmcs_commit_on_stream(pg_ctx* ctx, hipStream_t s, int field, const st
     }
     HIP_TRY(hipMemcpyAsync(t->root, t->digests + 8 * off, 32, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    guard.armed = false;
     memcpy(root, t->root, 32);
     t->owned = owned;
     *out = t.release();
@@ -1446,7 +1460,10 @@ extern "C" int pg_pcs_commit(pg_ctx* ctx, int field, const pg_buf* const* evals,
     }
     scratch.reset();
     int rc = mmcs_commit_on_stream(ctx, l.s, field, ldes, std::vector<bool>(n, true), root, out);
-    if (rc != PG_OK) cleanup();
+    if (rc != PG_OK) {
+        (void)hipStreamSynchronize(l.s);
+        cleanup();
+    }
     return rc;
 }
 
@@ -2202,13 +2219,20 @@ extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* eval
         b->launches.push_back([=](hipStream_t s) { return launch_gather_roots(rs, dst, nj, s); });
     }
     if (use_graph) {
-        HIP_TRY(hipStreamBeginCapture(l.s, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipSuccess;
-        for (auto& fn : b->launches)
-            if (e == hipSuccess) e = fn(l.s);
-        hipError_t e2 = hipStreamEndCapture(l.s, &b->graph);
+        // Captured on a stream of its own: the context's streams are shared
+        // round-robin by concurrent calls, whose work must not be recorded
+        // into this graph.
+        hipStream_t cap;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (auto& fn : b->launches)
+                if (e == hipSuccess) e = fn(cap);
+            const hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
+            if (e == hipSuccess) e = e2;
+        }
+        (void)hipStreamDestroy(cap);
         HIP_TRY(e);
-        HIP_TRY(e2);
         HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     *out = b.release();
@@ -2316,7 +2340,10 @@ static void run_svc_job(pg_service* sv, SvcJob* j, hipStream_t s) {
     j->trace = nullptr;
     if (rc == PG_OK) {
         rc = mmcs_commit_on_stream(ctx, s, j->field, {lde}, {true}, j->root, &j->tree);
-        if (rc != PG_OK) pg_buf_free(lde);
+        if (rc != PG_OK) {
+            (void)hipStreamSynchronize(s);
+            pg_buf_free(lde);
+        }
     }
     j->rc = rc;
     if (rc != PG_OK) j->err = g_err;
@@ -2677,11 +2704,18 @@ extern "C" int pg_fri_commit_phase(pg_ctx* ctx, int field, pg_challenger* ch, co
         PG_TRY(get_twiddles(ctx, l.s, field, len_log, &tw));
         PG_TRY(buf_new(ctx, (size_t)1 << (len_log - 1), 4, &cur));
         auto kt = make_timer(ctx);
-        TIMED(kt, "fri_fold", l.s,
-              launch_fri_fold(field, leaves->ptr, cur->rows, beta, (const uint32_t*)tw->ptr, cur->ptr, l.s));
+        if (kt) kt->begin("fri_fold", l.s);
+        hipError_t e = launch_fri_fold(field, leaves->ptr, cur->rows, beta, (const uint32_t*)tw->ptr, cur->ptr, l.s);
+        if (kt) kt->end(l.s);
         len_log--;
-        if (inputs[len_log]) HIP_TRY(launch_field_add(field, cur->ptr, inputs[len_log]->ptr, cur->rows * 4, l.s));
-        HIP_TRY(hipStreamSynchronize(l.s));
+        if (e == hipSuccess && inputs[len_log])
+            e = launch_field_add(field, cur->ptr, inputs[len_log]->ptr, cur->rows * 4, l.s);
+        const hipError_t es = hipStreamSynchronize(l.s);
+        if (e == hipSuccess) e = es;
+        if (e != hipSuccess) {
+            pg_buf_free(cur);
+            return fail(PG_E_DEVICE, "FRI fold round %d failed: %s", r, hipGetErrorString(e));
+        }
         r++;
     }
     const size_t nf = (size_t)1 << len_log;

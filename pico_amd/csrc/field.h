@@ -34,6 +34,8 @@ struct MontyField {
     static constexpr int SBOX = SBOX_;          // Poseidon2 S-box degree
 };
 
+// GEN = 3 / 31: the recursion builder's generator()
+// (vm/src/compiler/recursion/ir/utils.rs:13-19).
 // KoalaBear: 8 full + 20 partial rounds, x^3 (consts.rs:191-203).
 using KoalaBear = MontyField<0x7f000001u, 24, 3u, 3u, 20, 3>;
 // BabyBear: 8 full + 13 partial rounds, x^7 (consts.rs:176-186).

@@ -3,7 +3,7 @@
 Compiles merkle.hip to device assembly and counts the VALU instructions in the
 loop body of the single-matrix leaf-hash kernel (one sponge chunk = one
 Poseidon2 permutation plus its two 16-byte loads), per field, by opcode and by
-issue class.  Issue classes, from probes/valu_rates.hip on MI355X:
+issue class.  Issue classes, from a per-opcode throughput microbenchmark on MI355X:
   double   v_add_u32, v_sub_u32, v_subrev_u32, v_mov_b32, v_and_b32: up to
            two per lane per clock (measured 64-70 T/s against the 39.3 T/s
            single-issue ceiling), priced at half a slot

@@ -76,7 +76,7 @@ def op_pcs_commit(ctx):
         doms = [pcs.natural_domain_for_degree(m.rows) for m in mats]
         doms[1] = pico_amd.TwoAdicMultiplicativeCoset(10, pico_amd.GENERATORS[f] * pico_amd.two_adic_generator(f, 12) % p, f)
         root, pd = pcs.commit(list(zip(doms, mats)))
-        return [root, pcs.get_evaluations_on_domain(pd, 1, doms[1])]
+        return [root, pcs.get_evaluations_on_domain(pd, 1, pico_amd.TwoAdicMultiplicativeCoset(10, pico_amd.GENERATORS[f]))]
     finally:
         _free(pd, *mats)
 

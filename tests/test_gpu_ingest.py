@@ -79,7 +79,12 @@ def test_quotient_split_commit_matches_oracle(ctx, oracle, f, log_n, d, lb):
     root2, pd2 = pcs.commit(list(zip(doms, dense)))
     assert np.array_equal(root, root2)
     for k in range(d):
-        assert np.array_equal(pcs.get_evaluations_on_domain(pd, k, doms[k]), pcs.get_evaluations_on_domain(pd2, k, doms[k]))
+        gdom = pico_amd.TwoAdicMultiplicativeCoset(doms[k].log_n, g)
+        assert np.array_equal(pcs.get_evaluations_on_domain(pd, k, gdom), pcs.get_evaluations_on_domain(pd2, k, gdom))
+    # as in p3 TwoAdicFriPcs, a domain whose shift is not the generator is refused
+    if d > 1:
+        with pytest.raises(PgError):
+            pcs.get_evaluations_on_domain(pd, 1, doms[1])
     pd.free()
     pd2.free()
     dev.free()
@@ -168,7 +173,9 @@ def test_commit_host_matches_upload_then_commit(ctx, oracle, f, lb):
     assert np.array_equal(root_h, oracle.MerkleTree(f, ldes).root)
     assert np.array_equal(root_h, root_d)
     for k, d in enumerate(doms):
-        assert np.array_equal(pcs.get_evaluations_on_domain(pd_h, k, d), pcs.get_evaluations_on_domain(pd_d, k, d)), k
+        gdom = pico_amd.TwoAdicMultiplicativeCoset(d.log_n, pico_amd.GENERATORS[f])
+        assert np.array_equal(pcs.get_evaluations_on_domain(pd_h, k, gdom),
+                              pcs.get_evaluations_on_domain(pd_d, k, gdom)), k
     # Montgomery words on the host, from page-locked memory
     pinned = []
     for a in hosts:

@@ -10,6 +10,7 @@ import pytest
 import pico_amd
 from pico_amd import BABYBEAR, KOALABEAR
 from tests import pyref
+from tests.verifier import verify_open
 
 pytestmark = pytest.mark.gpu
 FIELDS = [KOALABEAR, BABYBEAR]
@@ -99,7 +100,9 @@ def _points(f, ch, log_trace):
     return zeta, [np.stack([zeta, gz])] * 3, [np.asarray(zeta, np.uint32)[None, :]]
 
 
-def test_open_verifies(ctx, oracle):
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("log_final", [None, 3])
+def test_open_verifies(ctx, oracle, f, log_final):
     """pg_pcs_open on a two-round commitment (a mixed-height main batch at
     zeta and g zeta, a quotient-like batch on a shifted domain at zeta),
     checked by a verifier written from circuit/fri.rs:36-436: transcript
@@ -107,8 +110,7 @@ def test_open_verifies(ctx, oracle):
     value, the PoW witness, the query indices), every input opening against
     its round root, each height's reduced opening recomputed from the opened
     rows, and every query's fold chain through the commit-phase openings to
-    the final value."""
-    f = KOALABEAR
+    the final value (tests/verifier.py)."""
     p = pico_amd.PRIMES[f]
     pcs, (root_main, main), (root_q, quot) = _two_rounds(ctx, oracle, f)
     ch = pico_amd.DuplexChallenger(f)
@@ -117,69 +119,24 @@ def test_open_verifies(ctx, oracle):
     zeta, main_pts, quot_pts = _points(f, ch, 9)
     rounds = [(main, main_pts), (quot, quot_pts)]
     nq, pow_bits = 12, 6
-    opened, proof = pcs.open(rounds, ch, num_queries=nq, pow_bits=pow_bits)
-    # transcript replay
-    vch = pico_amd.DuplexChallenger(f)
+    opened, proof = pcs.open(rounds, ch, num_queries=nq, pow_bits=pow_bits, log_final=log_final)
+    vch = oracle.Challenger(f)
     vch.observe(root_main)
     vch.observe(root_q)
     assert [int(x) for x in vch.sample_ext()] == [int(x) for x in zeta]
-    for per_round in opened:
-        for v in per_round:
-            vch.observe(np.asarray(v).ravel())
-    alpha = [int(a) for a in vch.sample_ext()]
-    assert alpha == [int(a) for a in proof["alpha"]]
-    R = len(proof["commits"])
-    for r in range(R):
-        vch.observe(proof["commits"][r])
-        assert [int(b) for b in vch.sample_ext()] == [int(b) for b in proof["betas"][r]]
-    final = proof["final_poly"]
-    assert np.all(final == final[0])  # the reference stops at 2^log_blowup with a constant vector
-    vch.observe(final[0])
-    assert vch.check_witness(pow_bits, proof["pow_witness"])
-    log_max = 10
-    assert R == log_max - 1
-    idx_all = [vch.sample_bits(log_max) for _ in range(nq)]
-    assert idx_all == proof["query_indices"]
     mm = pico_amd.MerkleTreeMmcs(ctx, f)
-    for qi, idx in enumerate(idx_all):
-        # input openings and the reduced opening of every height at this query
-        ro, pow_ctr = {}, {}
-        for r, (d, mat_points) in enumerate(rounds):
-            rows, path = proof["input_openings"][qi][r]
-            dims = [(m.cols, m.rows) for m in mm.get_matrices(d)]
-            assert mm.verify_batch(d.root(), dims, idx >> (log_max - d.log_height), rows, path)
-            for k, (w, n) in enumerate(dims):
-                lh = n.bit_length() - 1
-                i = idx >> (log_max - lh)
-                x = pyref.GEN[f] * pow(pyref.root_of_unity(f, lh), pyref.rev(i, lh), p) % p
-                for q, z in enumerate(np.asarray(mat_points[k]).reshape(-1, 4)):
-                    num = [0, 0, 0, 0]
-                    for c in range(w):
-                        ap = [1, 0, 0, 0]
-                        for _ in range(pow_ctr.get(lh, 0)):
-                            ap = pyref.ext_mul(f, ap, alpha)
-                        v = [int(t) for t in opened[r][k][q, c]]
-                        diff = [(v[0] - int(rows[k][c])) % p] + v[1:]
-                        num = [(a + b) % p for a, b in zip(num, pyref.ext_mul(f, ap, diff))]
-                        pow_ctr[lh] = pow_ctr.get(lh, 0) + 1
-                    den = [(int(z[0]) - x) % p, int(z[1]), int(z[2]), int(z[3])]
-                    term = pyref.ext_mul(f, num, _ext_inv(f, den))
-                    ro[lh] = [(a + b) % p for a, b in zip(ro.get(lh, [0, 0, 0, 0]), term)]
-        # fold chain through the commit-phase openings
-        cur = ro[log_max]
-        for r in range(R):
-            L = log_max - r
-            ir = idx >> r
-            s = [int(t) for t in proof["fri_siblings"][qi][r]]
-            pair = [cur, s] if ir % 2 == 0 else [s, cur]
-            leaf = np.array(pair[0] + pair[1], np.uint32)
-            assert mm.verify_batch(proof["commits"][r], [(8, 1 << (L - 1))], ir >> 1, [leaf],
-                                   proof["fri_paths"][qi][r]), (qi, r)
-            x = pow(pyref.root_of_unity(f, L), pyref.rev(ir >> 1, L - 1), p)
-            cur = pyref.fold_pair(f, pair[0], pair[1], [int(b) for b in proof["betas"][r]], x)
-            if L - 1 in ro:
-                cur = [(a + b) % p for a, b in zip(cur, ro[L - 1])]
-        assert cur == [int(t) for t in final[0]], qi
+    vr = [(root_main, [(m.cols, m.rows) for m in mm.get_matrices(main)], main_pts),
+          (root_q, [(m.cols, m.rows) for m in mm.get_matrices(quot)], quot_pts)]
+    verify_open(oracle, f, vch, vr, opened, proof, 1, nq, pow_bits, log_final=log_final)
+    # a planted mistake in one opened value is rejected
+    bad = [[np.array(v, copy=True) for v in per] for per in opened]
+    bad[0][1][0, 0, 0] = (int(bad[0][1][0, 0, 0]) + 1) % p
+    vch = oracle.Challenger(f)
+    vch.observe(root_main)
+    vch.observe(root_q)
+    vch.sample_ext()
+    with pytest.raises(AssertionError):
+        verify_open(oracle, f, vch, vr, bad, proof, 1, nq, pow_bits, log_final=log_final)
     main.free()
     quot.free()
 

@@ -189,6 +189,45 @@ def test_mmcs_vs_definition(oracle, f, shape):
             assert not oracle.mmcs_verify(f, heights, widths, tree.root, idx, opened, bp)
 
 
+SEEDED_SHAPES = [
+    # (log_h, width, domain shift exponent of GEN*w^e or None for 1)
+    [(5, 13, None), (5, 3, None), (4, 8, None), (2, 9, None), (4, 1, None)],
+    [(6, 300, None)],
+    [(3, 17, 1), (3, 5, 2), (1, 4, None)],
+    [(0, 7, None), (0, 2, None)],
+]
+
+
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("shape", SEEDED_SHAPES)
+@pytest.mark.parametrize("b", [1, 2, 3])
+@pytest.mark.parametrize("block_cols", [1, 3, 8, 64])
+def test_pcs_commit_seeded_vs_whole_ldes(oracle, f, shape, b, block_cols):
+    """The memory-lean commit (LDE in column blocks, sponge states carried
+    across blocks and matrices) equals the commitment of whole LDEs built
+    from the same seeded traces."""
+    p = oracle.PRIMES[f]
+    g = oracle.generator(f)
+    seeds = [(9 << 32) | (50 + i) for i in range(len(shape))]
+    shifts = []
+    for L, _, e in shape:
+        shifts.append(1 if e is None else g * pow(oracle.two_adic_generator(f, L + 2), e, p) % p)
+    ldes = []
+    for s, (L, w, _), sh in zip(seeds, shape, shifts):
+        a = oracle.random_matrix(f, s, 1 << L, w)
+        ldes.append(oracle.coset_lde(f, a, b, g * pow(sh, p - 2, p) % p))
+    want = oracle.MerkleTree(f, ldes).root
+    got = oracle.pcs_commit_seeded(f, seeds, [1 << L for L, _, _ in shape], [w for _, w, _ in shape], b,
+                                   domain_shifts=shifts, block_cols=block_cols)
+    assert np.array_equal(got, want)
+    # a planted mistake (a different seed for one matrix) changes the root
+    bad = list(seeds)
+    bad[-1] ^= 1
+    assert not np.array_equal(
+        oracle.pcs_commit_seeded(f, bad, [1 << L for L, _, _ in shape], [w for _, w, _ in shape], b,
+                                 domain_shifts=shifts, block_cols=block_cols), want)
+
+
 # --------------------------------------------------------------------- FRI --
 @pytest.mark.parametrize("f", FIELDS)
 def test_fri_fold_vs_line(oracle, f):
