@@ -246,6 +246,23 @@ int pg_pcs_evaluations(pg_ctx *ctx, const pg_tree *tree, size_t k, size_t rows, 
  * 4 distinct points per height. */
 int pg_pcs_open_values(pg_ctx *ctx, const pg_tree *t, size_t k, uint32_t log_blowup,
                        const uint32_t *points_mont, size_t npts, uint32_t *out_mont);
+/* pg_pcs_open_values_noalloc: the same values as pg_pcs_open_values, bit for
+ * bit, computed without allocating device memory.  It has no counterpart in
+ * the reference; it is for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It calls
+ * no hipMalloc, hipFree or pool allocation.  Instead it works in a static
+ * device array of PG_NOALLOC_SCRATCH_BYTES, one per GPU, which each GPU keeps
+ * until the process ends.  The array holds the twiddle table (4h bytes,
+ * h = trace height), the barycentric weights of two points at a time
+ * (32h bytes), the partial column sums and the output staging.  So trace
+ * heights reach 2^20 at two points.  A request that does not fit returns
+ * PG_E_INVALID with the byte count it needs.  Calls on one GPU share the
+ * array and are serialised by a per-device lock held for the whole call.
+ * Each call rewrites every part of the array it reads, so nothing is left
+ * over from the previous call. */
+#define PG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int pg_pcs_open_values_noalloc(pg_ctx *ctx, const pg_tree *t, size_t k, uint32_t log_blowup,
+                               const uint32_t *points_mont, size_t npts, uint32_t *out_mont);
 int pg_pcs_reduce_openings(pg_ctx *ctx, const pg_tree *const *rounds, size_t nrounds,
                            const uint32_t *npoints, const uint32_t *points_mont,
                            const uint32_t *values_mont, const uint32_t alpha_mont[4],

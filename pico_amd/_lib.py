@@ -91,6 +91,7 @@ SIGNATURES = {
     "pg_pcs_commit_host": (_i, [_P, _i, _P, _P, _P, _P, _sz, _u32, _i, _P, _pp]),
     "pg_pcs_evaluations": (_i, [_P, _P, _sz, _sz, _P]),
     "pg_pcs_open_values": (_i, [_P, _P, _sz, _u32, _P, _sz, _P]),
+    "pg_pcs_open_values_noalloc": (_i, [_P, _P, _sz, _u32, _P, _sz, _P]),
     "pg_pcs_reduce_openings": (_i, [_P, _P, _sz, _P, _P, _P, _P, _P]),
     "pg_pcs_open": (_i, [_P, _P, _sz, _P, _P, _u32, _i, _sz, _i, _P, _pp]),
     "pg_open_proof_sizes": (_i, [_P, _P, _P, _P, _P, _P, _P]),

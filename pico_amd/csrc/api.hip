@@ -347,6 +347,44 @@ static int bind_device(pg_ctx* ctx) {
     return PG_OK;
 }
 
+// ------------------------------------------------ allocation-free scratch --
+// pg_pcs_open_values_noalloc works in this array instead of device memory it
+// allocates.  Every GPU the library runs on holds its own copy (placed when
+// the code object is loaded on that device) until the process ends.  In host
+// code the name is only a stand-in: the copy's device address on each GPU is
+// looked up once per device by pg_create, with that device current, and only
+// that address is passed to kernels and copies.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = PG_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoallocSlot {
+    std::mutex mu;  // held for a whole call, so calls on one device run one at a time
+    uint8_t* ptr = nullptr;
+};
+static NoallocSlot g_noalloc[NOALLOC_MAX_DEVICES];
+
+// `device` must be current
+static int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return fail(PG_E_UNSUPPORTED, "device %d: the allocation-free scratch covers devices 0..%d", device,
+                    NOALLOC_MAX_DEVICES - 1);
+    NoallocSlot& sl = g_noalloc[device];
+    std::lock_guard<std::mutex> g(sl.mu);
+    if (sl.ptr) return PG_OK;
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    if (!p) return fail(PG_E_DEVICE, "hipGetSymbolAddress returned a null address for the scratch array");
+    sl.ptr = (uint8_t*)p;
+    return PG_OK;
+}
+
+// A caller-independent device region that a call works in instead of
+// allocating (null base: allocate as usual).
+struct FixedScratch {
+    uint8_t* base = nullptr;
+    size_t bytes = 0;
+};
+
 // --------------------------------------------------------------- staging --
 // Copies between device memory and pageable host memory, and gathers of
 // column blocks.  The runtime stages pageable copies itself at ~26-30 GB/s
@@ -597,6 +635,7 @@ extern "C" int pg_create(int device, size_t budget, pg_ctx** out) {
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(PG_E_INVALID, "device %d of %d", device, ndev);
     HIP_TRY(hipSetDevice(device));
+    PG_TRY(noalloc_scratch_init(device));
     std::unique_ptr<pg_ctx> c(new pg_ctx);
     c->device = device;
     if (budget == 0) {
@@ -1744,7 +1783,8 @@ struct WeightCache {
 
 template <class F>
 static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k, uint32_t log_blowup,
-                         const uint32_t* points, size_t npts, uint32_t* out, WeightCache* wc = nullptr) {
+                         const uint32_t* points, size_t npts, uint32_t* out, WeightCache* wc = nullptr,
+                         const FixedScratch* fixed = nullptr) {
     const pg_buf* m = t->mats[k];
     if (!is_pow2(m->rows) || (m->rows >> log_blowup) == 0 || log2_of(m->rows) < (int)log_blowup)
         return fail(PG_E_INVALID, "matrix %zu: height %zu has no trace coset at log_blowup %u", k, m->rows,
@@ -1752,26 +1792,64 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
     const int log_h = log2_of(m->rows) - (int)log_blowup;
     const size_t h = (size_t)1 << log_h, w = m->cols;
     if (npts == 0 || w == 0) return PG_OK;
-    std::shared_ptr<DevMem> tw;
-    PG_TRY(get_twiddles(ctx, s, t->field, log_h, &tw));
+    // In a fixed region the twiddle table is built in the region itself, two
+    // points go per launch (weights of two points at a time), and chunks are
+    // made taller until the partial sums fit.  Any chunking gives the same
+    // values: every sum is exact mod p.
+    const size_t per = fixed ? 2 : 4;  // points per launch
+    const size_t q_max = std::min(npts, per);
     // about 2^20 (chunk, column) threads, at least 16 rows per chunk
-    const uint64_t col_threads = open_values_vec4(m->ptr, m->cols, (uint32_t)w, (int)std::min<size_t>(npts, 4))
-                                     ? w / 4 : w;
+    const uint64_t col_threads = open_values_vec4(m->ptr, m->cols, (uint32_t)w, (int)q_max) ? w / 4 : w;
     uint32_t chunk_rows = 16;
     while ((uint64_t)chunk_rows * (1u << 20) < (uint64_t)h * col_threads) chunk_rows <<= 1;
-    const uint32_t nchunks = (uint32_t)((h + chunk_rows - 1) / chunk_rows);
-    const size_t per = 4;  // points per launch
-    const size_t q_max = std::min(npts, per);
-    // column sums in slices of about 64 chunks (at most 64 slices)
-    const uint32_t nslices = std::max<uint32_t>(1, std::min<uint32_t>(64, nchunks / 64));
-    const size_t b_w = wc ? 0 : q_max * h * 16, b_p = q_max * nchunks * w * 16, b_s = q_max * nslices * w * 16,
-                 b_o = npts * w * 16;
-    int rc;
-    auto scratch = make_devmem(ctx, b_w + b_p + b_s + b_o, &rc);
-    if (!scratch) return rc;
-    uint8_t* base = (uint8_t*)scratch->ptr;
+    struct Layout {
+        uint32_t nchunks, nslices;
+        size_t b_t, b_w, b_p, b_s, b_o;  // twiddles, weights, partials, slice sums, output
+        size_t total() const { return b_t + b_w + b_p + b_s + b_o; }
+    };
+    auto layout = [&](uint32_t cr) {
+        Layout y;
+        y.nchunks = (uint32_t)((h + cr - 1) / cr);
+        // column sums in slices of about 64 chunks (at most 64 slices)
+        y.nslices = std::max<uint32_t>(1, std::min<uint32_t>(64, y.nchunks / 64));
+        y.b_t = fixed ? round_up(4 * h, 256) : 0;
+        y.b_w = wc ? 0 : round_up(q_max * h * 16, 256);
+        y.b_p = round_up(q_max * y.nchunks * w * 16, 256);
+        y.b_s = round_up(q_max * y.nslices * w * 16, 256);
+        y.b_o = npts * w * 16;
+        return y;
+    };
+    Layout y = layout(chunk_rows);
+    if (fixed) {
+        while (y.total() > fixed->bytes && chunk_rows < h) y = layout(chunk_rows <<= 1);
+        if (y.total() > fixed->bytes)
+            return fail(PG_E_INVALID,
+                        "matrix %zu (trace height 2^%d, width %zu, %zu points) needs %zu bytes of scratch; the "
+                        "allocation-free form has %zu",
+                        k, log_h, w, npts, y.total(), fixed->bytes);
+    }
+    std::shared_ptr<DevMem> tw, scratch;
+    uint8_t* base;
+    const uint32_t* twp;
+    if (fixed) {
+        base = fixed->base;
+        twp = (const uint32_t*)base;
+    } else {
+        PG_TRY(get_twiddles(ctx, s, t->field, log_h, &tw));
+        twp = (const uint32_t*)tw->ptr;
+        int rc;
+        scratch = make_devmem(ctx, y.total(), &rc);
+        if (!scratch) return rc;
+        base = (uint8_t*)scratch->ptr;
+    }
+    SyncGuard guard(s);  // an early return waits for queued kernels before the scratch is released
+    if (fixed) HIP_TRY(launch_twiddles(t->field, log_h, (uint32_t*)base, s));
+    uint8_t* const wts = base + y.b_t;
+    uint8_t* const partial = wts + y.b_w;
+    uint8_t* const sums = partial + y.b_p;
+    uint8_t* const outs = sums + y.b_s;
     const uint32_t s_mont = to_monty<F>(F::GEN);
-    auto kt = make_timer(ctx);
+    std::unique_ptr<EventTimer> kt = fixed ? nullptr : make_timer(ctx);
     for (size_t q0 = 0; q0 < npts; q0 += per) {
         const int nq = (int)std::min(per, npts - q0);
         Ext4 z[4], fac[4];
@@ -1779,7 +1857,7 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
             z[q] = ext_from<F>(points + 4 * (q0 + q));
             fac[q] = bary_factor<F>(z[q], s_mont, log_h);
         }
-        Ext4* weights = (Ext4*)base;
+        Ext4* weights = (Ext4*)wts;
         int ready = 0;
         if (wc) {
             std::vector<uint32_t> key{(uint32_t)log_h, (uint32_t)nq};
@@ -1789,6 +1867,7 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
                 weights = (Ext4*)it->second->ptr;
                 ready = 1;
             } else {
+                int rc;
                 auto wb = make_devmem(ctx, (size_t)nq * h * 16, &rc);
                 if (!wb) return rc;
                 weights = (Ext4*)wb->ptr;
@@ -1796,13 +1875,13 @@ static int open_values_t(pg_ctx* ctx, hipStream_t s, const pg_tree* t, size_t k,
             }
         }
         TIMED(kt, "open_values", s,
-              launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, (const uint32_t*)tw->ptr, z, fac, nq,
-                                 weights, ready, (Ext4*)(base + b_w), nchunks, chunk_rows,
-                                 (Ext4*)(base + b_w + b_p), nslices,
-                                 (Ext4*)(base + b_w + b_p + b_s + q0 * w * 16), s));
+              launch_open_values(t->field, m->ptr, m->cols, (uint32_t)w, log_h, twp, z, fac, nq, weights, ready,
+                                 (Ext4*)partial, y.nchunks, chunk_rows, (Ext4*)sums, y.nslices,
+                                 (Ext4*)(outs + q0 * w * 16), s));
     }
-    HIP_TRY(hipMemcpyAsync(out, base + b_w + b_p + b_s, b_o, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out, outs, y.b_o, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    guard.armed = false;
     return PG_OK;
 }
 
@@ -1820,6 +1899,26 @@ static int open_values_cached(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t 
 extern "C" int pg_pcs_open_values(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t log_blowup,
                                   const uint32_t* points_mont, size_t npts, uint32_t* out_mont) {
     return open_values_cached(ctx, t, k, log_blowup, points_mont, npts, out_mont, nullptr);
+}
+
+// The same values with the device's static scratch array instead of
+// allocated memory (twiddles, weights, partial and slice sums, output
+// staging): no hipMalloc, hipFree or pool allocation.
+extern "C" int pg_pcs_open_values_noalloc(pg_ctx* ctx, const pg_tree* t, size_t k, uint32_t log_blowup,
+                                          const uint32_t* points_mont, size_t npts, uint32_t* out_mont) {
+    if (!ctx || !t || (npts && (!points_mont || !out_mont)) || k >= t->mats.size())
+        return fail(PG_E_INVALID, "bad argument");
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+        return fail(PG_E_UNSUPPORTED, "device %d has no allocation-free scratch", ctx->device);
+    PG_TRY(bind_device(ctx));
+    NoallocSlot& sl = g_noalloc[ctx->device];
+    std::lock_guard<std::mutex> g(sl.mu);
+    if (!sl.ptr) return fail(PG_E_DEVICE, "device %d: the scratch address was not looked up", ctx->device);
+    const FixedScratch fx{sl.ptr, NOALLOC_SCRATCH_BYTES};
+    Lease l(ctx);
+    return t->field == PG_KOALABEAR
+               ? open_values_t<KoalaBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont, nullptr, &fx)
+               : open_values_t<BabyBear>(ctx, l.s, t, k, log_blowup, points_mont, npts, out_mont, nullptr, &fx);
 }
 
 template <class F>
