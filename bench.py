@@ -62,6 +62,52 @@ def pmc_traffic(workload, kernel):
 
 HBM_PEAK_GBS = 8000.0
 
+LDE_KIND_IDS = {"lde_single": 0, "lde_pass_a": 1, "lde_pass_bc": 2, "lde_pass_d": 3}
+
+
+def lde_tile_log_e(log_s, kind, variant):
+    """log2 of the values per thread of the LDE pass kernel that runs a size
+    2^log_s transform: the choice of pico_amd/csrc/ntt.hip (shape_log_e, and
+    shape_alt_log_e for tuning variant 1)."""
+    coset_loop = kind in (0, 2)
+    if variant:
+        alt = {7: 3, 8: 5, 9: 4, 10: 5, 11: 6 if coset_loop else 4}.get(log_s, -1)
+        if alt >= 0:
+            return alt
+    if log_s <= 5:
+        return log_s
+    return {6: 3, 7: 4, 8: 4, 9: 5, 10: 4}.get(log_s, 4 if coset_loop else 6)
+
+
+def lde_slot_floors(st, field_id, log_h, w, lb):
+    """Per LDE pass of an h x w LDE at blowup 2^lb: the issue slots its
+    threads execute (static gfx950 instruction counts of the kernel that runs,
+    pico_amd/_build/isa_stats.json) and the time they take at the
+    single-issue ceiling."""
+    import pico_amd
+    if not st or "lde_kernels" not in st:
+        return {}
+    L = log_h
+    k1 = 0 if L <= 11 else (L + 1) // 2
+    k2 = L - k1
+    passes = {"lde_single": L} if L <= 11 else {"lde_pass_a": k1, "lde_pass_bc": k2, "lde_pass_d": k1}
+    elems = {"lde_single": 1 << L, "lde_pass_a": 1 << L, "lde_pass_bc": 1 << L, "lde_pass_d": 1 << (L + lb)}
+    out = {}
+    for fam, log_s in passes.items():
+        kind = LDE_KIND_IDS[fam]
+        var = pico_amd.tune_get(field_id, kind, log_s, w)
+        le = lde_tile_log_e(log_s, kind, var)
+        rec = next((r for r in st["lde_kernels"]
+                    if r["kind"] == fam and r["log_e"] == le and r["log_p"] == log_s - le), None)
+        if rec is None:
+            continue
+        ncos = (1 << lb) if kind in (0, 2) else 0
+        per_thread = rec["slots_outside_loop"] + ncos * rec["slots_per_coset"]
+        threads = elems[fam] * w >> le
+        out[fam] = {"tile": f"{1 << le}x{1 << (log_s - le)}", "slots_per_element": round(per_thread / (1 << le), 2),
+                    "slot_floor_ms": threads * per_thread / (VALU_PEAK_TSLOTS * 1e12) * 1e3}
+    return out
+
 
 def percentile(xs, q):
     s = sorted(xs)
@@ -233,11 +279,28 @@ def main():
     pass_bytes = {"lde_pass_a": 2 * h * by, "lde_pass_bc": h * by + n_rows * by,
                   "lde_pass_d": 2 * n_rows * by, "lde_single": h * by + n_rows * by}
     ntt = {}
+    floors = lde_slot_floors(st, f, args.log_h, w, lb)
     for fam, b in pass_bytes.items():
         if fam in kern:
             gbs = b / (kern[fam]["avg_ms"] / 1e3) / 1e9
             ntt[fam] = {"bytes": b, "avg_ms": round(kern[fam]["avg_ms"], 4), "GB/s": round(gbs, 1),
                         "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+            if fam in floors:
+                fl = floors[fam]
+                ntt[fam].update({"tile": fl["tile"], "valu_slots_per_element": fl["slots_per_element"],
+                                 "valu_slot_floor_ms": round(fl["slot_floor_ms"], 4),
+                                 "valu_issue_frac": round(fl["slot_floor_ms"] / kern[fam]["avg_ms"], 4)})
+    lde_ms = sum(kern[fam]["avg_ms"] * kern[fam]["launches_per_step"] for fam in pass_bytes if fam in kern)
+    if lde_ms:
+        # the LDE's algorithmic bytes (read the trace once, write the LDE once)
+        # over the time of all its passes
+        lde_bytes = h * w * 4 + n_rows * w * 4
+        floor = sum(fl["slot_floor_ms"] for fl in floors.values())
+        ntt["whole_lde"] = {"algorithmic_bytes": lde_bytes, "ms": round(lde_ms, 4),
+                            "GB/s": round(lde_bytes / (lde_ms / 1e3) / 1e9, 1),
+                            "frac_hbm": round(lde_bytes / (lde_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "valu_slot_floor_ms": round(floor, 4) if floor else None,
+                            "valu_issue_frac": round(floor / lde_ms, 4) if floor else None}
 
     out = {
         "metric": METRIC,

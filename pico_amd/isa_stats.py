@@ -11,6 +11,13 @@ issue class.  Issue classes, from a per-opcode throughput microbenchmark on MI35
            64-bit adds measure at 0.90 of the ceiling, the rest at 0.97)
 bench.py turns the slot count into the leaf hash's achieved issue rate.
 
+It also compiles ntt.hip and counts, for every instantiation of the LDE pass
+kernel lde_kernel<field, log E, log P, kind>, the VALU instructions and issue
+slots one thread executes outside the coset loop and per coset iteration
+(passes BC and SINGLE loop over the 2^b cosets; A and D have no such loop).
+A thread owns E elements, so slots per element = (outside + cosets x loop)/E;
+bench.py prices each LDE pass at that count against the single-issue ceiling.
+
 Usage: python3 isa_stats.py OUT.json   (run from pico_amd/, by the Makefile)
 """
 import json
@@ -70,15 +77,75 @@ def loop_body(lines):
     return best
 
 
-def main(out_path):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    asm_path = os.path.join(HERE, "_build", "merkle_gfx950.s")
-    os.makedirs(os.path.dirname(asm_path), exist_ok=True)
+def slots(hist):
+    dbl = sum(n for op, n in hist.items() if op in DOUBLE_ISSUE)
+    return sum(hist.values()) - 0.5 * dbl
+
+
+def histogram(lines):
+    hist = {}
+    for x in lines:
+        op = opcode(x)
+        if op:
+            hist[op] = hist.get(op, 0) + 1
+    return hist
+
+
+def coset_loop(lines):
+    """Lines of the loop whose header the compiler marks as a loop of depth 1
+    and that holds LDS accesses (the coset loop of passes BC and SINGLE),
+    from its first block to its last backward branch (s_branch included)."""
+    labels = {}
+    for i, line in enumerate(lines):
+        m = re.match(r"^(\.LBB\w+):", line)
+        if m:
+            labels[m.group(1)] = i
+    best = []
+    for i, line in enumerate(lines):
+        m = re.match(r"^\s+s_(?:cbranch_\w+|branch)\s+(\.LBB\w+)", line)
+        if m and m.group(1) in labels and labels[m.group(1)] < i:
+            body = lines[labels[m.group(1)]:i + 1]
+            if any(re.match(r"^\s+ds_", x) for x in body) and len(body) > len(best):
+                best = body
+    return best
+
+
+LDE_FIELDS = {"koalabear": "2130706433", "babybear": "2013265921"}
+LDE_KINDS = {0: "lde_single", 1: "lde_pass_a", 2: "lde_pass_bc", 3: "lde_pass_d"}
+
+
+def lde_stats(asm):
+    """Per field: one record per lde_kernel<field, log E, log P, kind>."""
+    out = {}
+    pat = re.compile(r"^(_ZN2pg10lde_kernelINS_10MontyFieldILj(\d+)E\w*?EEELi(\d+)ELi(\d+)ELi(\d)EEE\w*):",
+                     re.M)
+    for m in pat.finditer(asm):
+        name, prime, le, lp, kind = m.group(1), m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5))
+        field = next(f for f, pr in LDE_FIELDS.items() if pr == prime)
+        lines = kernel_lines(asm, name)
+        total = histogram(lines)
+        loop = histogram(coset_loop(lines)) if kind in (0, 2) else {}
+        out.setdefault(field, []).append({
+            "kind": LDE_KINDS[kind], "log_e": le, "log_p": lp,
+            "valu_outside_loop": sum(total.values()) - sum(loop.values()),
+            "slots_outside_loop": slots(total) - slots(loop),
+            "valu_per_coset": sum(loop.values()), "slots_per_coset": slots(loop)})
+    return out
+
+
+def compile_asm(hipcc, src, asm_path):
     subprocess.run([hipcc, "-O3", "-std=c++17", "--offload-arch=gfx950", "-S", "--cuda-device-only",
                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                    os.path.join(HERE, "csrc", "merkle.hip"), "-o", asm_path],
+                    os.path.join(HERE, "csrc", src), "-o", asm_path],
                    check=True, stderr=subprocess.DEVNULL)
-    asm = open(asm_path).read()
+    return open(asm_path).read()
+
+
+def main(out_path):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    asm = compile_asm(hipcc, "merkle.hip", os.path.join(HERE, "_build", "merkle_gfx950.s"))
+    ntt = lde_stats(compile_asm(hipcc, "ntt.hip", os.path.join(HERE, "_build", "ntt_gfx950.s")))
     stats = {}
     for field, prefix in KERNELS.items():
         lines = kernel_lines(asm, prefix)
@@ -95,10 +162,11 @@ def main(out_path):
             "leaf_hash_w8_double_issue_per_chunk": dbl,
             "leaf_hash_w8_slots_per_chunk": (total - dbl) + 0.5 * dbl,
             "leaf_hash_w8_opcodes": dict(sorted(hist.items(), key=lambda kv: -kv[1])),
+            "lde_kernels": ntt.get(field, []),
         }
     with open(out_path, "w") as f:
         json.dump(stats, f, indent=1)
-    print(json.dumps(stats))
+    print(json.dumps({f: {k: v for k, v in d.items() if k != "lde_kernels"} for f, d in stats.items()}))
 
 
 if __name__ == "__main__":

@@ -363,3 +363,26 @@ def test_commit_host_concurrent_threads_share_the_staging_pool(ctx):
     assert np.array_equal(got[0], solo[0]) and np.array_equal(got[1], solo[1])
     assert np.array_equal(got[2], big_solo)
     big.free()
+
+
+def test_commit_host_default_blocks_split_tail_matches_oracle(ctx, oracle):
+    """Default block sizing on a matrix whose last column block is split: at
+    2^19 rows a 256 MiB block is 128 columns, so 200 columns go as 128 + 72,
+    and the 72-column tail is uploaded and absorbed in narrower pieces.  The
+    root and the whole LDE of the split matrix equal the oracle's; a shorter
+    matrix is injected beside it."""
+    f = KOALABEAR
+    shapes = [(19, 200), (17, 12)]
+    hosts = [oracle.random_matrix(f, 6400 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, 1)
+    doms = [pcs.natural_domain_for_degree(1 << L) for L, _ in shapes]
+    print("default-block commit_host 2^19 x 200 + 2^17 x 12", flush=True)
+    root, pd = pcs.commit_host([(d, pico_amd.to_monty(f, a)) for d, a in zip(doms, hosts)], canonical=False)
+    try:
+        ldes = [oracle.coset_lde(f, a, 1, pico_amd.GENERATORS[f]) for a in hosts]
+        assert np.array_equal(root, oracle.MerkleTree(f, ldes).root)
+        got = pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd)
+        assert np.array_equal(got[0].to_host(f), ldes[0])
+        assert np.array_equal(got[1].to_host(f), ldes[1])
+    finally:
+        pd.free()

@@ -39,6 +39,8 @@ def test_upload_download_roundtrip(ctx, oracle, f):
 LDE_CASES = [(L, w, b) for L in range(0, 12) for (w, b) in [(1, 1), (5, 2), (16, 1), (33, 3)]]
 LDE_CASES += [(L, w, b) for L in range(12, 17) for (w, b) in [(3, 1), (17, 2), (40, 1)]]
 LDE_CASES += [(18, 2, 3), (20, 3, 1), (21, 5, 2), (22, 1, 1), (22, 2, 2)]
+# log_blowup 4: the compress config (vm/src/configs/stark_config/kb_poseidon2.rs:111-140)
+LDE_CASES += [(0, 3, 4), (5, 9, 4), (9, 16, 4), (11, 8, 4), (12, 5, 4), (15, 24, 4), (18, 3, 4)]
 
 
 @pytest.mark.parametrize("L,w,b", LDE_CASES)
@@ -145,7 +147,7 @@ def test_pcs_commit_mixed_heights_and_shifts(ctx, oracle, f):
     hosts = [oracle.random_matrix(f, 900 + i, h, w) for i, (h, w) in enumerate(shape)]
     # trace domains (shift 1) and quotient-chunk-like domains (shift g * w^i)
     shifts = [1, pico_amd.GENERATORS[f] * pico_amd.two_adic_generator(f, 12) % p, 1, 5, 1]
-    for lb in (1, 3):
+    for lb in (1, 3, 4):  # 4: the compress config (kb_poseidon2.rs:111-140)
         pcs = pico_amd.TwoAdicFriPcs(ctx, f, lb)
         doms = [pico_amd.TwoAdicMultiplicativeCoset(h.bit_length() - 1, s) for (h, _), s in zip(shape, shifts)]
         root, pd = pcs.commit([(d, up(ctx, f, a)) for d, a in zip(doms, hosts)])
