@@ -295,6 +295,28 @@ int pg_open_proof_sizes(const pg_open_proof *p, size_t *n_values, int *fri_round
 int pg_open_proof_read(const pg_open_proof *p, uint32_t *values, uint32_t alpha[4], uint32_t *commits,
                        uint32_t *betas, uint32_t *final_poly, uint32_t *pow_witness, uint64_t *query_indices,
                        uint32_t *input_openings, uint32_t *fri_siblings, uint32_t *fri_paths);
+/* The PcsProof part of a pg_open_proof as bytes: bincode 1 (little-endian,
+ * fixed-width integers, u64 sequence lengths) of p3-fri's
+ * FriProof<Challenge, ChallengeMmcs, Val, Vec<BatchOpening<Val, ValMmcs>>>,
+ * which the reference stores in every BaseProof::opening_proof and writes
+ * with bincode in MetaProof::save_to_file (vm/src/machine/proof.rs:86-101,
+ * :159-171).  Field order as the recursion verifier reads it
+ * (vm/src/compiler/recursion/circuit/types.rs:29-54): commit_phase_commits
+ * (u64 count, 8 words each); query_proofs (u64 count, then per query:
+ * input_proof = u64 rounds, per round u64 matrices, per matrix u64 width +
+ * row words, then u64 levels + 8 words per level; commit_phase_openings =
+ * u64 rounds, per round 4 sibling words + u64 levels + 8 words per level);
+ * final_poly (4 words); pow_witness (1 word).  A word is a u32 field element:
+ * the Montgomery value (p3's in-memory form), or the canonical value when
+ * canonical != 0.  encode: out may be null to get *len; the final vector must
+ * be constant (pg_pcs_open with log_final < 0).  decode: the bytes of one
+ * proof, every query of one shape; the result holds the proof parts only (no
+ * values, alpha, betas or query indices: the verifier's transcript derives
+ * those), readable with pg_open_proof_sizes / pg_open_proof_read and
+ * encodable again.  No counterpart beyond the reference's bincode of the same
+ * struct. */
+int pg_open_proof_encode(const pg_open_proof *p, int canonical, uint8_t *out, size_t cap, size_t *len);
+int pg_open_proof_decode(int field, const uint8_t *bytes, size_t len, int canonical, pg_open_proof **out);
 void pg_open_proof_free(pg_open_proof *p);
 
 /* ---- batched commits ------------------------------------------------------

@@ -96,6 +96,8 @@ SIGNATURES = {
     "pg_pcs_open": (_i, [_P, _P, _sz, _P, _P, _u32, _i, _sz, _i, _P, _pp]),
     "pg_open_proof_sizes": (_i, [_P, _P, _P, _P, _P, _P, _P]),
     "pg_open_proof_read": (_i, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "pg_open_proof_encode": (_i, [_P, _i, _P, _sz, _P]),
+    "pg_open_proof_decode": (_i, [_i, _P, _sz, _i, _pp]),
     "pg_open_proof_free": (None, [_P]),
     "pg_batch_create": (_i, [_P, _i, _P, _P, _P, _sz, _i, _pp]),
     "pg_batch_run": (_i, [_P, _P, _P]),

@@ -546,6 +546,9 @@ class TwoAdicFriPcs:
             fpaths = np.zeros(max(nq * fpw, 1), np.uint32)
             check(lib().pg_open_proof_read(h, _ptr(values), _ptr(alpha), _ptr(commits), _ptr(betas), _ptr(final),
                                            ctypes.byref(wit), _ptr(qidx), _ptr(inputs), _ptr(sib), _ptr(fpaths)))
+            # the FriProof as the reference's bincode holds it (constant final
+            # vector only: p3's FriProof keeps one final value)
+            bincode = _encode_open_proof(h, canonical=False) if log_final is None else None
         finally:
             lib().pg_open_proof_free(h)
         values = from_monty(f, values[:nvals])
@@ -585,8 +588,52 @@ class TwoAdicFriPcs:
         proof = {"alpha": from_monty(f, alpha), "commits": from_monty(f, commits[:R]),
                  "betas": from_monty(f, betas[:R]), "final_poly": from_monty(f, final), "pow_witness": wit.value,
                  "query_indices": [int(x) for x in qidx[:nq]], "input_openings": input_openings,
-                 "fri_siblings": from_monty(f, sib[:nq, :R]), "fri_paths": fri_paths}
+                 "fri_siblings": from_monty(f, sib[:nq, :R]), "fri_paths": fri_paths, "bincode": bincode}
         return opened, proof
+
+
+def _encode_open_proof(handle, canonical):
+    n = ctypes.c_size_t()
+    check(lib().pg_open_proof_encode(handle, int(bool(canonical)), None, 0, ctypes.byref(n)))
+    buf = (ctypes.c_uint8 * n.value)()
+    check(lib().pg_open_proof_encode(handle, int(bool(canonical)), buf, n.value, ctypes.byref(n)))
+    return bytes(buf)
+
+
+def decode_fri_proof(field, data, canonical=False):
+    """pg_open_proof_decode: bincode bytes of a FriProof (see
+    TwoAdicFriPcs.open's proof["bincode"]) -> its parts, canonical values:
+    commits (R x 8), input_openings[query][round] = (rows per matrix, path
+    levels x 8), fri_siblings (queries x R x 4), fri_paths[query][round]
+    (levels x 8), final_poly (4), pow_witness; plus "reencoded", the bytes
+    pg_open_proof_encode writes for the decoded proof (Montgomery words)."""
+    raw = (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(bytes(data) or b"\0")
+    h = ctypes.c_void_p()
+    check(lib().pg_open_proof_decode(field, raw, len(data), int(bool(canonical)), ctypes.byref(h)))
+    try:
+        sz = [ctypes.c_size_t() for _ in range(5)]
+        nr = ctypes.c_int()
+        check(lib().pg_open_proof_sizes(h, ctypes.byref(sz[0]), ctypes.byref(nr), ctypes.byref(sz[1]),
+                                        ctypes.byref(sz[2]), ctypes.byref(sz[3]), ctypes.byref(sz[4])))
+        _, nfinal, nq, inw, fpw = (x.value for x in sz)
+        R = nr.value
+        commits = np.zeros((max(R, 1), 8), np.uint32)
+        final = np.zeros(max(nfinal, 4), np.uint32)
+        wit = ctypes.c_uint32()
+        inputs = np.zeros(max(nq * inw, 1), np.uint32)
+        sib = np.zeros((max(nq, 1), max(R, 1), 4), np.uint32)
+        fpaths = np.zeros(max(nq * fpw, 1), np.uint32)
+        check(lib().pg_open_proof_read(h, None, None, _ptr(commits), None, _ptr(final), ctypes.byref(wit), None,
+                                       _ptr(inputs), _ptr(sib), _ptr(fpaths)))
+        reencoded = _encode_open_proof(h, canonical=False)
+    finally:
+        lib().pg_open_proof_free(h)
+    return {"commits": from_monty(field, commits[:R]), "final_poly": from_monty(field, final[:4]),
+            "pow_witness": wit.value,
+            "inputs_flat": from_monty(field, inputs[:nq * inw]).reshape(nq, inw),
+            "fri_siblings": from_monty(field, sib[:nq, :R]),
+            "fri_paths_flat": from_monty(field, fpaths[:nq * fpw]).reshape(nq, fpw),
+            "num_queries": nq, "reencoded": reencoded}
 
 
 class CommitBatch:
@@ -816,5 +863,5 @@ __all__ = [
     "fri_grind", "fri_answer_queries", "to_monty", "from_monty", "two_adic_generator", "PgError",
     "device_synchronize", "CommitBatch", "CommitService", "PinnedMatrix",
     "autotune", "tune_export", "tune_import", "tune_set", "tune_get", "LDE_PASSES", "load_tuning",
-    "TUNING_FILE",
+    "TUNING_FILE", "decode_fri_proof",
 ]

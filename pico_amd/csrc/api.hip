@@ -2920,6 +2920,12 @@ struct pg_open_proof {
     std::vector<uint32_t> inputs;
     size_t fri_path_words = 0;  // per query
     std::vector<uint32_t> siblings, fri_paths;
+    // shapes of the input openings: per round, its matrices' widths (the
+    // tree's original order) and its log height (path length)
+    std::vector<std::vector<size_t>> round_widths;
+    std::vector<int> round_log_h;
+    std::vector<int> fri_path_levels;  // per commit-phase round
+    size_t num_queries = 0;
 };
 
 extern "C" void pg_open_proof_free(pg_open_proof* p) { delete p; }
@@ -3000,9 +3006,16 @@ extern "C" int pg_pcs_open(pg_ctx* ctx, const pg_tree* const* rounds, size_t nro
     // 4. proof of work, query indices
     PG_TRY(pg_fri_grind(ctx, field, ch, pow_bits, &pf->pow_witness));
     pf->queries.resize(num_queries);
+    pf->num_queries = num_queries;
     for (auto& q : pf->queries) q = pg_challenger_sample_bits(ch, log_max);
     // 5. input openings, round by round, interleaved per query
-    for (size_t r = 0; r < nrounds; r++) pf->input_words += rounds[r]->total_width + 8 * (size_t)rounds[r]->log_h;
+    for (size_t r = 0; r < nrounds; r++) {
+        pf->input_words += rounds[r]->total_width + 8 * (size_t)rounds[r]->log_h;
+        std::vector<size_t> ws;
+        for (const pg_buf* m : rounds[r]->mats) ws.push_back(m->cols);
+        pf->round_widths.push_back(std::move(ws));
+        pf->round_log_h.push_back(rounds[r]->log_h);
+    }
     pf->inputs.resize(num_queries * pf->input_words);
     size_t roff = 0;
     std::vector<uint64_t> idx(num_queries);
@@ -3020,7 +3033,10 @@ extern "C" int pg_pcs_open(pg_ctx* ctx, const pg_tree* const* rounds, size_t nro
         roff += tw + pw;
     }
     // 6. commit-phase openings
-    for (auto* t : fri->trees) pf->fri_path_words += 8 * (size_t)t->log_h;
+    for (auto* t : fri->trees) {
+        pf->fri_path_words += 8 * (size_t)t->log_h;
+        pf->fri_path_levels.push_back(t->log_h);
+    }
     pf->siblings.resize(num_queries * (size_t)pf->fri_rounds * 4);
     pf->fri_paths.resize(num_queries * pf->fri_path_words);
     PG_TRY(pg_fri_answer_queries(ctx, fri, pf->queries.data(), num_queries, pf->siblings.data(),
@@ -3036,7 +3052,7 @@ extern "C" int pg_open_proof_sizes(const pg_open_proof* p, size_t* n_values, int
     if (n_values) *n_values = p->values.size();
     if (fri_rounds) *fri_rounds = p->fri_rounds;
     if (final_words) *final_words = p->final_poly.size();
-    if (num_queries) *num_queries = p->queries.size();
+    if (num_queries) *num_queries = p->num_queries;
     if (input_words_per_query) *input_words_per_query = p->input_words;
     if (fri_path_words_per_query) *fri_path_words_per_query = p->fri_path_words;
     return PG_OK;
@@ -3060,6 +3076,186 @@ extern "C" int pg_open_proof_read(const pg_open_proof* p, uint32_t* values, uint
     put(input_openings, p->inputs.data(), p->inputs.size() * 4);
     put(fri_siblings, p->siblings.data(), p->siblings.size() * 4);
     put(fri_paths, p->fri_paths.data(), p->fri_paths.size() * 4);
+    return PG_OK;
+}
+
+// ------------------------------------------------------- proof encoding --
+// Bincode 1 (little-endian, fixed-width integers, u64 sequence lengths) of
+// p3-fri's FriProof<Challenge, ChallengeMmcs, Val, Vec<BatchOpening>>, the
+// PcsProof TwoAdicFriPcs::open returns.  Field order as the reference's
+// recursion circuit reads it (FriProofVariable, QueryProofVariable,
+// BatchOpeningVariable, FriCommitPhaseProofStepVariable at
+// vm/src/compiler/recursion/circuit/types.rs:29-54, filled in that order by
+// circuit/witness/embed.rs:55-165):
+//   commit_phase_commits  Vec<Hash<Val, Val, 8>>  u64 n, n x 8 words
+//   query_proofs          Vec<QueryProof>         u64 n, then per query:
+//     input_proof         Vec<BatchOpening>       u64 rounds, then per round:
+//       opened_values     Vec<Vec<Val>>           u64 matrices, each u64 w + w words
+//       opening_proof     Vec<[Val; 8]>           u64 levels, levels x 8 words
+//     commit_phase_openings Vec<CommitPhaseProofStep>  u64 rounds, each:
+//       sibling_value     Challenge               4 words
+//       opening_proof     Vec<[Val; 8]>           u64 levels, levels x 8 words
+//   final_poly            Challenge               4 words
+//   pow_witness           Val                     1 word
+// Arrays and extension elements are fixed-size tuples: no length.  A word is
+// a field element as a u32: the Montgomery value p3 keeps in memory, or with
+// canonical != 0 the canonical value.
+struct ByteSink {
+    uint8_t* out;
+    size_t cap, n = 0;
+    void put(const void* v, size_t k) {
+        if (out && n + k <= cap) memcpy(out + n, v, k);
+        n += k;
+    }
+    void u64(uint64_t v) { put(&v, 8); }
+    void u32(uint32_t v) { put(&v, 4); }
+};
+
+static uint32_t word_out(int field, uint32_t v, int canonical) {
+    if (!canonical) return v;
+    return field == PG_KOALABEAR ? from_monty<KoalaBear>(v) : from_monty<BabyBear>(v);
+}
+
+extern "C" int pg_open_proof_encode(const pg_open_proof* p, int canonical, uint8_t* out, size_t cap, size_t* len) {
+    if (!p || !len) return fail(PG_E_INVALID, "bad argument");
+    if (p->final_poly.size() < 4 || p->final_poly.size() % 4) return fail(PG_E_INVALID, "proof has no final value");
+    for (size_t i = 4; i < p->final_poly.size(); i++)
+        if (p->final_poly[i] != p->final_poly[i % 4])
+            return fail(PG_E_UNSUPPORTED,
+                        "the final vector is not constant; p3's FriProof holds one final value (open with "
+                        "log_final < 0)");
+    if ((int)p->fri_path_levels.size() != p->fri_rounds || p->round_widths.size() != p->round_log_h.size())
+        return fail(PG_E_INVALID, "proof shapes are incomplete");
+    const int f = p->field;
+    ByteSink b{out, cap};
+    auto word = [&](uint32_t v) { b.u32(word_out(f, v, canonical)); };
+    b.u64((uint64_t)p->fri_rounds);
+    for (size_t i = 0; i < 8 * (size_t)p->fri_rounds; i++) word(p->commits[i]);
+    b.u64(p->num_queries);
+    for (size_t q = 0; q < p->num_queries; q++) {
+        const uint32_t* in = p->inputs.data() + q * p->input_words;
+        b.u64(p->round_widths.size());
+        for (size_t r = 0; r < p->round_widths.size(); r++) {
+            b.u64(p->round_widths[r].size());
+            for (size_t w : p->round_widths[r]) {
+                b.u64(w);
+                for (size_t j = 0; j < w; j++) word(*in++);
+            }
+            b.u64((uint64_t)p->round_log_h[r]);
+            for (size_t j = 0; j < 8 * (size_t)p->round_log_h[r]; j++) word(*in++);
+        }
+        b.u64((uint64_t)p->fri_rounds);
+        const uint32_t* fp = p->fri_paths.data() + q * p->fri_path_words;
+        for (int r = 0; r < p->fri_rounds; r++) {
+            for (int e = 0; e < 4; e++) word(p->siblings[(q * (size_t)p->fri_rounds + r) * 4 + e]);
+            b.u64((uint64_t)p->fri_path_levels[r]);
+            for (size_t j = 0; j < 8 * (size_t)p->fri_path_levels[r]; j++) word(*fp++);
+        }
+    }
+    for (int e = 0; e < 4; e++) word(p->final_poly[e]);
+    // the witness is held canonical
+    b.u32(canonical ? p->pow_witness
+                    : (f == PG_KOALABEAR ? to_monty<KoalaBear>(p->pow_witness) : to_monty<BabyBear>(p->pow_witness)));
+    *len = b.n;
+    if (out && b.n > cap) return fail(PG_E_INVALID, "buffer of %zu bytes; the encoding needs %zu", cap, b.n);
+    return PG_OK;
+}
+
+struct ByteSource {
+    const uint8_t* p;
+    size_t len, n = 0;
+    bool ok = true;
+    bool take(void* v, size_t k) {
+        if (!ok || n + k > len) return ok = false;
+        memcpy(v, p + n, k);
+        n += k;
+        return true;
+    }
+    uint64_t u64() {
+        uint64_t v = 0;
+        take(&v, 8);
+        return v;
+    }
+    // a sequence length whose elements take at least `unit` bytes each
+    uint64_t count(size_t unit) {
+        const uint64_t v = u64();
+        if (ok && unit && v > (len - n) / unit) ok = false;
+        return ok ? v : 0;
+    }
+};
+
+extern "C" int pg_open_proof_decode(int field, const uint8_t* bytes, size_t len, int canonical,
+                                    pg_open_proof** out) {
+    if (!out || (!bytes && len) || !field_ok(field)) return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    const uint32_t P = field == PG_KOALABEAR ? KoalaBear::P : BabyBear::P;
+    ByteSource src{bytes, len};
+    bool bad_word = false;
+    auto word = [&]() -> uint32_t {
+        uint32_t v = 0;
+        src.take(&v, 4);
+        if (v >= P) bad_word = true;
+        if (!canonical) return v;
+        return field == PG_KOALABEAR ? to_monty<KoalaBear>(v % P) : to_monty<BabyBear>(v % P);
+    };
+    std::unique_ptr<pg_open_proof> pf(new pg_open_proof);
+    pf->field = field;
+    const uint64_t R = src.count(32);
+    pf->fri_rounds = (int)R;
+    for (uint64_t i = 0; i < 8 * R && src.ok; i++) pf->commits.push_back(word());
+    const uint64_t nq = src.count(16);
+    pf->num_queries = nq;
+    bool shape_bad = false;
+    for (uint64_t q = 0; q < nq && src.ok && !shape_bad; q++) {
+        std::vector<std::vector<size_t>> widths;
+        std::vector<int> log_h;
+        const uint64_t nr = src.count(16);
+        for (uint64_t r = 0; r < nr && src.ok; r++) {
+            std::vector<size_t> ws;
+            const uint64_t nm = src.count(8);
+            for (uint64_t k = 0; k < nm && src.ok; k++) {
+                const uint64_t w = src.count(4);
+                ws.push_back(w);
+                for (uint64_t j = 0; j < w && src.ok; j++) pf->inputs.push_back(word());
+            }
+            const uint64_t levels = src.count(32);
+            for (uint64_t j = 0; j < 8 * levels && src.ok; j++) pf->inputs.push_back(word());
+            widths.push_back(std::move(ws));
+            log_h.push_back((int)levels);
+        }
+        const uint64_t nfr = src.count(24);
+        std::vector<int> fl;
+        for (uint64_t r = 0; r < nfr && src.ok; r++) {
+            for (int e = 0; e < 4; e++) pf->siblings.push_back(word());
+            const uint64_t levels = src.count(32);
+            for (uint64_t j = 0; j < 8 * levels && src.ok; j++) pf->fri_paths.push_back(word());
+            fl.push_back((int)levels);
+        }
+        if (q == 0) {
+            pf->round_widths = widths;
+            pf->round_log_h = log_h;
+            pf->fri_path_levels = fl;
+        } else if (widths != pf->round_widths || log_h != pf->round_log_h || fl != pf->fri_path_levels) {
+            shape_bad = true;
+        }
+        if (nfr != R) shape_bad = true;
+    }
+    pf->final_poly.resize(4);
+    for (int e = 0; e < 4; e++) pf->final_poly[e] = word();
+    const uint32_t wit = word();  // held canonical
+    pf->pow_witness = field == PG_KOALABEAR ? from_monty<KoalaBear>(wit) : from_monty<BabyBear>(wit);
+    if (!src.ok) return fail(PG_E_INVALID, "encoding ends early or declares more data than it holds");
+    if (src.n != len) return fail(PG_E_INVALID, "%zu bytes after the end of the proof", len - src.n);
+    if (bad_word) return fail(PG_E_INVALID, "a field element is not below p");
+    if (shape_bad)
+        return fail(PG_E_INVALID, "queries differ in shape (every query opens the same rounds, matrices and rounds "
+                                  "of the commit phase)");
+    for (size_t r = 0; r < pf->round_widths.size(); r++) {
+        for (size_t w : pf->round_widths[r]) pf->input_words += w;
+        pf->input_words += 8 * (size_t)pf->round_log_h[r];
+    }
+    for (int l : pf->fri_path_levels) pf->fri_path_words += 8 * (size_t)l;
+    *out = pf.release();
     return PG_OK;
 }
 

@@ -166,3 +166,42 @@ def splitmix_element(f, seed, i):
         if x < p:
             return x
     return x % p
+
+
+def encode_fri_proof(f, commits, queries, final_poly, pow_witness, canonical):
+    """Bincode 1 bytes of p3-fri's FriProof, written from the struct layout
+    the reference's recursion verifier reads (FriProofVariable and friends,
+    vm/src/compiler/recursion/circuit/types.rs:29-54): u64 little-endian
+    sequence lengths, u32 words, fixed-size arrays and extension elements
+    without a length.  All inputs canonical; `queries` is a list of
+    (input_proof, commit_phase_openings) with input_proof a list of
+    (rows per matrix, path digests) per round and commit_phase_openings a
+    list of (sibling ext element, path digests).  Words are written
+    canonical, or in Montgomery form (x 2^32 mod p) when canonical is False."""
+    import struct
+    p = P[f]
+
+    def word(x):
+        x = int(x) % p
+        return struct.pack("<I", x if canonical else (x << 32) % p)
+
+    def length(n):
+        return struct.pack("<Q", n)
+
+    def digests(ds):
+        return length(len(ds)) + b"".join(word(x) for d in ds for x in d)
+
+    out = [length(len(commits))] + [word(x) for c in commits for x in c]
+    out.append(length(len(queries)))
+    for input_proof, cp in queries:
+        out.append(length(len(input_proof)))
+        for rows, path in input_proof:
+            out.append(length(len(rows)))
+            for row in rows:
+                out.append(length(len(row)) + b"".join(word(x) for x in row))
+            out.append(digests(path))
+        out.append(length(len(cp)))
+        for sibling, path in cp:
+            out.append(b"".join(word(x) for x in sibling) + digests(path))
+    out.append(b"".join(word(x) for x in final_poly) + word(pow_witness))
+    return b"".join(out)

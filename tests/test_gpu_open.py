@@ -128,6 +128,21 @@ def test_open_verifies(ctx, oracle, f, log_final):
     vr = [(root_main, [(m.cols, m.rows) for m in mm.get_matrices(main)], main_pts),
           (root_q, [(m.cols, m.rows) for m in mm.get_matrices(quot)], quot_pts)]
     verify_open(oracle, f, vch, vr, opened, proof, 1, nq, pow_bits, log_final=log_final)
+    if log_final is None:
+        # the FriProof's bincode (vm/src/machine/proof.rs:86-101 stores it
+        # in every BaseProof) equals an independent encoding of the verified
+        # parts, and decodes to them
+        queries = [([(rows, path) for rows, path in proof["input_openings"][q]],
+                    [(proof["fri_siblings"][q][r], proof["fri_paths"][q][r]) for r in range(len(proof["commits"]))])
+                   for q in range(nq)]
+        want = pyref.encode_fri_proof(f, proof["commits"], queries, proof["final_poly"][0], proof["pow_witness"],
+                                      canonical=False)
+        assert proof["bincode"] == want
+        d = pico_amd.decode_fri_proof(f, proof["bincode"])
+        assert d["reencoded"] == want and d["pow_witness"] == proof["pow_witness"]
+        assert np.array_equal(d["commits"], proof["commits"])
+    else:
+        assert proof["bincode"] is None  # a non-constant final vector has no FriProof form
     # a planted mistake in one opened value is rejected
     bad = [[np.array(v, copy=True) for v in per] for per in opened]
     bad[0][1][0, 0, 0] = (int(bad[0][1][0, 0, 0]) + 1) % p
