@@ -441,6 +441,8 @@ def main():
         progress("shard commit + FRI done")
         out["shard_ingest"] = workloads.shard_ingest(ctx)
         progress("shard ingest done")
+        out["chip_tables"] = workloads.chip_tables(ctx)
+        progress("chip tables done")
         out["batched_jobs"] = workloads.batched_jobs(ctx)
         progress("batched jobs done")
         jobs = workloads.served_jobs(n=args.served_jobs)

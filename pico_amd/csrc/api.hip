@@ -970,6 +970,17 @@ static int lde_check(int field, size_t rows, size_t cols, uint32_t b, int* L_out
     return PG_OK;
 }
 
+// Device descriptor tables built on the host, uploaded with one copy.
+struct Staging {
+    std::vector<uint8_t> bytes;
+    size_t put(const void* p, size_t n) {  // p == nullptr reserves n zero bytes
+        const size_t off = round_up(bytes.size(), 256);
+        bytes.resize(off + n, 0);
+        if (n && p) memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
 // LDE of `in` into a new N x w buffer on stream s; `scratch` (may be null)
 // is reused when large enough.
 static int lde_on_stream(pg_ctx* ctx, hipStream_t s, int field, const pg_buf* in, uint32_t b,
@@ -1464,41 +1475,161 @@ extern "C" int pg_mmcs_verify_batch(int field, const size_t* heights, const size
 }
 
 // ------------------------------------------------------------------- PCS --
+// Pcs::commit of device matrices.  The LDEs of all matrices are queued on
+// one stream without a host synchronisation between them: matrices of one
+// height share each pass launch (a device table of jobs, as in batched
+// commits), a height held by a single matrix runs the plain per-matrix
+// launches, and the tree follows on the same stream.  Heights run one after
+// another and reuse one scratch buffer (the largest group's).  The
+// reference commits all chips of a shard in one call, sorted by height
+// (vm/src/machine/prover.rs:186-244).
 extern "C" int pg_pcs_commit(pg_ctx* ctx, int field, const pg_buf* const* evals, const uint32_t* dshift,
                              size_t n, uint32_t log_blowup, uint32_t root[8], pg_tree** out) {
     if (!ctx || !evals || !root || !out || !field_ok(field) || n == 0) return fail(PG_E_INVALID, "bad argument");
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
+    const uint32_t g = gen_monty(field);
+    std::vector<int> L(n);
+    std::vector<uint32_t> shift(n, g);
+    for (size_t k = 0; k < n; k++) {
+        if (!evals[k]) return fail(PG_E_INVALID, "null matrix %zu", k);
+        PG_TRY(lde_check(field, evals[k]->rows, evals[k]->cols, log_blowup, &L[k]));
+        if (dshift) {
+            if (dshift[k] == 0) return fail(PG_E_INVALID, "domain shift is zero");
+            shift[k] = field == PG_KOALABEAR ? fmul<KoalaBear>(g, finv<KoalaBear>(dshift[k]))
+                                             : fmul<BabyBear>(g, finv<BabyBear>(dshift[k]));
+        }
+    }
     std::vector<pg_buf*> ldes;
-    std::shared_ptr<DevMem> scratch;
     auto cleanup = [&]() {
         for (auto* b : ldes) pg_buf_free(b);
     };
-    const uint32_t g = gen_monty(field);
     for (size_t k = 0; k < n; k++) {
-        if (!evals[k]) {
-            cleanup();
-            return fail(PG_E_INVALID, "null matrix %zu", k);
-        }
-        uint32_t shift = g;
-        if (dshift) {
-            if (dshift[k] == 0) {
-                cleanup();
-                return fail(PG_E_INVALID, "domain shift is zero");
-            }
-            shift = field == PG_KOALABEAR ? fmul<KoalaBear>(g, finv<KoalaBear>(dshift[k]))
-                                          : fmul<BabyBear>(g, finv<BabyBear>(dshift[k]));
-        }
         pg_buf* o;
-        int rc = lde_on_stream(ctx, l.s, field, evals[k], log_blowup, shift, 1, &o, &scratch);
+        const int rc = buf_new(ctx, evals[k]->rows << log_blowup, evals[k]->cols, &o);
         if (rc != PG_OK) {
             cleanup();
             return rc;
         }
         ldes.push_back(o);
     }
-    scratch.reset();
-    int rc = mmcs_commit_on_stream(ctx, l.s, field, ldes, std::vector<bool>(n, true), root, out);
+    // tables (kept referenced until the stream is synchronised), scratch and
+    // the job tables of the shared launches
+    std::map<int, std::vector<size_t>, std::greater<int>> by_L;
+    for (size_t k = 0; k < n; k++) by_L[L[k]].push_back(k);
+    std::vector<std::shared_ptr<DevMem>> keep;
+    std::map<int, const uint32_t*> tw_of;
+    std::vector<const uint32_t*> sc_of(n);
+    uint64_t tmp_words = 0;
+    int rc = PG_OK;
+    for (auto& kv : by_L) {
+        std::shared_ptr<DevMem> tw;
+        if (rc == PG_OK) rc = get_twiddles(ctx, l.s, field, kv.first, &tw);
+        if (rc != PG_OK) break;
+        keep.push_back(tw);
+        tw_of[kv.first] = (const uint32_t*)tw->ptr;
+        uint64_t words = 0;
+        for (size_t k : kv.second) {
+            std::shared_ptr<DevMem> sc;
+            rc = get_scale(ctx, l.s, field, kv.first, (int)log_blowup, shift[k], &sc);
+            if (rc != PG_OK) break;
+            keep.push_back(sc);
+            sc_of[k] = (const uint32_t*)sc->ptr;
+            if (kv.first > 11) words += (uint64_t)ldes[k]->rows * ldes[k]->cols;
+        }
+        tmp_words = std::max(tmp_words, words);
+    }
+    std::shared_ptr<DevMem> scratch;
+    if (rc == PG_OK && tmp_words) scratch = make_devmem(ctx, tmp_words * 4, &rc);
+    if (rc != PG_OK) {
+        cleanup();
+        return rc;
+    }
+    uint32_t* const tmp = scratch ? (uint32_t*)scratch->ptr : nullptr;
+    struct PassRec {
+        int kind, log_s, variant, njobs;
+        size_t off_args, off_prefix;
+        uint64_t total;
+    };
+    std::map<int, std::vector<PassRec>> shared;  // log height -> its pass launches
+    Staging st;
+    for (auto& kv : by_L) {
+        if (kv.second.size() < 2) continue;
+        const int Lg = kv.first;
+        int k1, k2;
+        lde_split(Lg, &k1, &k2);
+        std::vector<LdeArgs> args;
+        uint64_t off = 0;  // each job's scratch rows within the group's share
+        uint32_t wmax = 0;
+        for (size_t k : kv.second) {
+            LdeArgs a{};
+            a.in = evals[k]->ptr;
+            a.ld_in = evals[k]->ld;
+            a.out = ldes[k]->ptr;
+            a.ld_out = ldes[k]->cols;
+            a.tmp = Lg > 11 ? tmp + off : nullptr;
+            a.tw = tw_of[Lg];
+            a.scale = sc_of[k];
+            a.w = (uint32_t)ldes[k]->cols;
+            a.L = Lg;
+            a.k1 = k1;
+            a.k2 = k2;
+            a.b = (int)log_blowup;
+            a.bitrev = 1;
+            args.push_back(a);
+            if (Lg > 11) off += (uint64_t)ldes[k]->rows * ldes[k]->cols;
+            wmax = std::max(wmax, a.w);
+        }
+        const int kinds3[3] = {KIND_A, KIND_BC, KIND_D};
+        for (int i = 0; i < (Lg <= 11 ? 1 : 3); i++) {
+            const int kind = Lg <= 11 ? KIND_SINGLE : kinds3[i];
+            const int log_s = (kind == KIND_BC || kind == KIND_SINGLE) ? k2 : k1;
+            const int variant = lde_variant(field, kind, log_s, wmax);  // one tile shape per launch
+            const int C = lde_pass_cols(kind, log_s, variant);
+            std::vector<uint64_t> prefix(1, 0);
+            for (auto& a : args)
+                prefix.push_back(prefix.back() + lde_pass_row_tiles(kind, Lg, a.b) * ((a.w + C - 1) / C));
+            shared[Lg].push_back(PassRec{kind, log_s, variant, (int)args.size(),
+                                         st.put(args.data(), args.size() * sizeof(LdeArgs)),
+                                         st.put(prefix.data(), prefix.size() * 8), prefix.back()});
+        }
+    }
+    std::shared_ptr<DevMem> desc;
+    if (!st.bytes.empty()) {
+        desc = make_devmem(ctx, st.bytes.size(), &rc);
+        if (!desc) {
+            cleanup();
+            return rc;
+        }
+    }
+    auto kt = make_timer(ctx);
+    hipError_t e = hipSuccess;
+    if (desc) e = hipMemcpyAsync(desc->ptr, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, l.s);
+    for (auto& kv : by_L) {
+        if (e != hipSuccess) break;
+        const int Lg = kv.first;
+        if (kv.second.size() == 1) {
+            const size_t k = kv.second[0];
+            e = launch_coset_lde(field, evals[k]->ptr, evals[k]->ld, Lg, (uint32_t)evals[k]->cols, (int)log_blowup,
+                                 tw_of[Lg], sc_of[k], Lg > 11 ? tmp : nullptr, ldes[k]->ptr, ldes[k]->cols, 1, l.s,
+                                 kt.get());
+            continue;
+        }
+        const uint8_t* dd = (const uint8_t*)desc->ptr;
+        for (const PassRec& pr : shared[Lg]) {
+            e = launch_lde_pass_batch(field, pr.kind, pr.log_s, pr.variant, (const LdeArgs*)(dd + pr.off_args),
+                                      (const uint64_t*)(dd + pr.off_prefix), pr.njobs, pr.total, (int)log_blowup,
+                                      l.s, kt.get());
+            if (e != hipSuccess) break;
+        }
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(l.s);  // queued work still reads the tables and writes the LDEs
+        cleanup();
+        return fail(PG_E_DEVICE, "coset LDE launch failed: %s", hipGetErrorString(e));
+    }
+    // the tree consumes the LDEs (it owns them from here) and synchronises
+    rc = mmcs_commit_on_stream(ctx, l.s, field, ldes, std::vector<bool>(n, true), root, out);
     if (rc != PG_OK) {
         (void)hipStreamSynchronize(l.s);
         cleanup();
@@ -2117,16 +2248,6 @@ extern "C" void pg_batch_free(pg_batch* b) {
 }
 
 // host staging for one device descriptor block
-struct Staging {
-    std::vector<uint8_t> bytes;
-    size_t put(const void* p, size_t n) {  // p == nullptr reserves n zero bytes
-        const size_t off = align_up(bytes.size(), 256);
-        bytes.resize(off + n, 0);
-        if (n && p) memcpy(bytes.data() + off, p, n);
-        return off;
-    }
-};
-
 extern "C" int pg_batch_create(pg_ctx* ctx, int field, const pg_buf* const* evals, const uint32_t* dshift,
                                const uint32_t* log_blowups, size_t njobs, int use_graph, pg_batch** out) {
     if (!ctx || !evals || !log_blowups || !out || !field_ok(field) || njobs == 0)

@@ -14,6 +14,9 @@ device through the C ABI and reports wall time from the host clock.
                      round, 16-bit proof of work and 100 query openings
   shard_ingest       the same 14 traces in host memory (page-locked and
                      pageable): upload then commit vs pg_pcs_commit_host
+  chip_tables        one shard's 40 chip tables (log-heights 10..22, mostly
+                     small; widths 8..300) in one Pcs::commit vs an LDE
+                     call per table then one MMCS commit
   batched_jobs       64 single-matrix commits, log-heights 10..18, widths
                      8..512, blowup 2/4/8: one launch sequence (hipGraph) vs
                      the same jobs committed one by one
@@ -198,6 +201,71 @@ def shard_ingest(ctx, shapes=None, reps=3):
     out["roots_identical"] = bool(all(np.array_equal(r, roots[0]) for r in roots))
     for pm in pinned:
         pm.free()
+    return out
+
+
+# ------------------------------------------------------------ chip tables --
+def chip_table_shapes(seed_base=6 << 32, n=40, min_log=10, max_log=22):
+    """One shard's chip tables: mostly small, a few tall (log-height
+    min_log + floor((max_log - min_log + 1) u^3)), widths 8..300."""
+    r = SplitMix64(seed_base)
+    span = max_log - min_log + 1
+    out = []
+    for _ in range(n):
+        u = r.uniform01()
+        out.append((min(max_log, min_log + int(span * u ** 3)), r.uniform_int(8, 300)))
+    return out
+
+
+def lde_launches(heights):
+    """LDE pass launches for matrices of these log-heights: one per matrix
+    (1 pass up to 2^11 rows, 3 above) when committed one by one; one per
+    distinct height in a single Pcs::commit."""
+    per = [1 if L <= 11 else 3 for L in heights]
+    return sum(per), sum(1 if L <= 11 else 3 for L in set(heights))
+
+
+def chip_tables(ctx, shapes=None, reps=3, log_blowup=1):
+    """One Pcs::commit of a shard's chip tables (their LDEs queued without
+    host synchronisation, equal heights sharing launches) against the same
+    commit composed one matrix at a time (an LDE call per matrix, each
+    waiting for its kernels, then one MMCS commit)."""
+    f = core.KOALABEAR
+    shapes = shapes or chip_table_shapes()
+    mats = [ctx.random_matrix(f, (6 << 32) | (100 + i), 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = core.TwoAdicFriPcs(ctx, f, log_blowup)
+    dft = core.Radix2DitParallel(ctx, f)
+    mmcs = core.MerkleTreeMmcs(ctx, f)
+    gen = core.GENERATORS[f]
+    cells = sum((1 << L) * w for L, w in shapes)
+
+    def together():
+        r, pd = pcs.commit([(pcs.natural_domain_for_degree(m.rows), m) for m in mats])
+        pd.free()
+        return r
+
+    def one_by_one():
+        ldes = [dft.coset_lde_batch(m, log_blowup, gen) for m in mats]
+        r, pd = mmcs.commit(ldes, take_ownership=True)
+        pd.free()
+        return r
+
+    out = {"tables": len(shapes), "cells": cells, "log_heights": sorted({L for L, _ in shapes})}
+    for name, fn in (("one_by_one", one_by_one), ("together", together)):
+        root = fn()  # warm
+        ts = []
+        for _ in range(reps):
+            core.device_synchronize()
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        out[name + "_ms"] = min(ts) * 1e3
+        out[name + "_root"] = [int(x) for x in root]
+    out["roots_identical"] = out["one_by_one_root"] == out["together_root"]
+    out["lde_launches_one_by_one"], out["lde_launches_together"] = lde_launches([L for L, _ in shapes])
+    out["speedup"] = out["one_by_one_ms"] / out["together_ms"]
+    for m in mats:
+        m.free()
     return out
 
 

@@ -158,6 +158,34 @@ def test_pcs_commit_mixed_heights_and_shifts(ctx, oracle, f):
             assert np.array_equal(g_.to_host(f), e)
 
 
+@pytest.mark.parametrize("f", FIELDS)
+@pytest.mark.parametrize("lb", [1, 2])
+def test_pcs_commit_chip_tables_match_oracle(ctx, oracle, f, lb):
+    """A shard's chip tables in one Pcs::commit (prover.rs:186-244): 40
+    tables of 2^6..2^15 rows (several per height, so the shared launches of
+    the one- and three-pass LDEs both run, beside single-table heights) and
+    widths 8..300; every LDE and the root equal the oracle's, and the same
+    commit composed one table at a time."""
+    from pico_amd import workloads
+    shapes = workloads.chip_table_shapes((6 << 32) | (7 + f), n=40, min_log=6, max_log=15)
+    hosts = [oracle.random_matrix(f, 7000 + i, 1 << L, w) for i, (L, w) in enumerate(shapes)]
+    pcs = pico_amd.TwoAdicFriPcs(ctx, f, lb)
+    devs = [up(ctx, f, a) for a in hosts]
+    root, pd = pcs.commit([(pcs.natural_domain_for_degree(a.shape[0]), d) for a, d in zip(hosts, devs)])
+    ldes, ot = oracle_pcs_commit(oracle, f, hosts, [1] * len(hosts), lb)
+    assert np.array_equal(root, ot.root)
+    for g_, e in zip(pico_amd.MerkleTreeMmcs(ctx, f).get_matrices(pd), ldes):
+        assert np.array_equal(g_.to_host(f), e)
+    pd.free()
+    dft = pico_amd.Radix2DitParallel(ctx, f)
+    one = [dft.coset_lde_batch(d, lb, pico_amd.GENERATORS[f]) for d in devs]
+    root1, pd1 = pico_amd.MerkleTreeMmcs(ctx, f).commit(one, take_ownership=True)
+    assert np.array_equal(root1, root)
+    pd1.free()
+    for d in devs:
+        d.free()
+
+
 def test_same_job_alone_and_concurrent(ctx, oracle):
     """Thread safety: identical results when jobs run concurrently."""
     f = KOALABEAR
