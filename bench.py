@@ -116,40 +116,55 @@ def percentile(xs, q):
     return s[lo] + (s[hi] - s[lo]) * (k - lo)
 
 
-def cpu_baseline(log_h, w, lb, budget_cols, leaf_rows):
-    """Oracle (C restatement, OpenMP) timed on a bounded sample of the same
-    job and scaled to the full job: LDE of `budget_cols` columns, leaf
-    hashing of `leaf_rows` rows, compress layers priced at the measured
-    permutation rate."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(log_h, w, lb, runs, gpu_root):
+    """The whole job on the host: oracle/cpu_baseline.c (the reference's CPU
+    commit path restated in Montgomery form, AVX2, OpenMP over every host
+    thread) commits the same seeded 2^log_h x w trace; one warm-up run, then
+    the median of `runs` timed runs.  Its root is checked against the GPU's."""
     import numpy as np
 
     from oracle import oracle
 
     f = oracle.KOALABEAR
     h = 1 << log_h
-    n = h << lb
-    a = oracle.random_matrix(f, 1, h, budget_cols)
-    t0 = time.perf_counter()
-    oracle.coset_lde(f, a, lb, oracle.generator(f))
-    t_lde = (time.perf_counter() - t0) * (w / budget_cols)
-    rows = oracle.random_matrix(f, 2, leaf_rows, w)
-    t0 = time.perf_counter()
-    oracle.MerkleTree(f, [rows])
-    t_leaf = time.perf_counter() - t0
-    perms_sample = leaf_rows * ((w + 7) // 8) + (leaf_rows - 1)
-    perm_rate = perms_sample / t_leaf
-    perms_full = n * ((w + 7) // 8) + (n - 1)
-    t_total = t_lde + perms_full / perm_rate
-    del np
+    mont = oracle.cpu_to_monty(f, oracle.random_matrix(f, (2 << 32) | 0, h, w))  # input form, untimed
+    root, _ = oracle.cpu_pcs_commit_mont(f, mont, lb)  # warm-up
+    times, phases = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.cpu_pcs_commit_mont(f, mont, lb)
+        times.append(time.perf_counter() - t0)
+        phases.append(oracle.cpu_last_phase_seconds())
+    del mont
+    med = float(np.median(times))
     return {
-        "value": (h * w) / t_total,
+        "value": (h * w) / med,
         "unit": "cells/s",
-        "cores": oracle.num_threads(),
+        "cores": oracle.cpu_num_threads(),
         "kind": "port",
-        "sample": (f"oracle LDE of {budget_cols} of {w} columns (2^{log_h} rows, blowup {1 << lb}) "
-                   f"scaled x{w // budget_cols}; Merkle: leaf hashing of {leaf_rows} rows x {w}, "
-                   f"all {perms_full} permutations priced at the measured {perm_rate:.3g} perm/s"),
-        "est_job_seconds": t_total,
+        "sample": (f"full job: Pcs::commit of the same seeded 2^{log_h} x {w} KoalaBear trace at blowup "
+                   f"{1 << lb} (C restatement of the reference algorithm in Montgomery form, AVX2, "
+                   f"not the reference binary); median of {runs} runs after 1 warm-up"),
+        "runs": runs,
+        "median_s": med,
+        "min_s": min(times),
+        "max_s": max(times),
+        "lde_s_median": float(np.median([a for a, _ in phases])),
+        "merkle_s_median": float(np.median([b for _, b in phases])),
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
+        "root_matches_gpu": bool(np.array_equal(oracle.from_monty(f, root), np.asarray(gpu_root))),
     }
 
 
@@ -180,8 +195,7 @@ def main():
     ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--log-blowup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cols", type=int, default=32)
-    ap.add_argument("--cpu-leaf-rows", type=int, default=1 << 15)
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after 1 warm-up)")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the shard+FRI, batched-jobs and served-stream sections")
     ap.add_argument("--commit-only", action="store_true",
@@ -451,7 +465,8 @@ def main():
         out["served_stream"] = served
         progress("served stream done")
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.log_h, w, lb, args.cpu_cols, args.cpu_leaf_rows)
+        out["cpu_baseline"] = cpu_baseline(args.log_h, w, lb, args.cpu_runs, out["root"])
+        progress("CPU baseline done")
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -31,6 +31,8 @@
  *                         loop, circuit/fri.rs:118-241, applied to every row)
  *   or_pcs_commit_seeded  p3 TwoAdicFriPcs::commit (vm/src/machine/prover.rs:238-244)
  *                         = or_coset_lde + or_mmcs_commit, without holding LDEs
+ *   cb_pcs_commit         the same commit for one trace, written for speed
+ *                         (cpu_baseline.c): the CPU baseline, not a checker
  */
 #ifndef PICO_ORACLE_H
 #define PICO_ORACLE_H
@@ -150,6 +152,22 @@ int or_fri_commit_phase(or_challenger *c, const uint32_t *v, int log_n, int log_
 
 /* OpenMP thread count actually used by the parallel loops */
 int or_num_threads(void);
+
+/* ---- cpu_baseline.c: the CPU baseline bench.py times ---------------------
+ * TwoAdicFriPcs::commit of one h x w trace (vm/src/machine/prover.rs:238-244)
+ * as the reference's CPU path computes it: Radix2DitParallel coset LDE at
+ * blowup 2^log_blowup with shift GENERATOR, rows bit-reversed, MerkleTreeMmcs
+ * root.  Unlike the functions above, it takes and returns MONTGOMERY words
+ * (p3's in-memory form) and is written for speed (AVX2, OpenMP).  lde_out
+ * (N x w, committed order) may be NULL.  Returns 0, or -1 on a bad shape or
+ * a failed allocation.  Pinned to or_coset_lde + or_mmcs_commit by
+ * tests/test_oracle.py. */
+int cb_pcs_commit(int field, const uint32_t *in_mont, size_t h, size_t w, int log_blowup,
+                  uint32_t root_mont[8], uint32_t *lde_out);
+void cb_to_monty(int field, const uint32_t *in, uint32_t *out, size_t n);
+int cb_num_threads(void);
+/* LDE and Merkle seconds of the last cb_pcs_commit */
+void cb_last_phase_seconds(double *lde_s, double *merkle_s);
 
 #ifdef __cplusplus
 }

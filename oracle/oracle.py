@@ -58,6 +58,10 @@ def lib():
             "or_pcs_open_values": (None, [i, P, sz, sz, i, P, P]),
             "or_pcs_reduce": (None, [i, sz, P, P, P, P, P, P, P, P]),
             "or_pcs_commit_seeded": (i, [i, sz, P, P, P, P, i, sz, P]),
+            "cb_pcs_commit": (i, [i, P, sz, sz, i, P, P]),
+            "cb_to_monty": (None, [i, P, P, sz]),
+            "cb_num_threads": (i, []),
+            "cb_last_phase_seconds": (None, [P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -334,3 +338,46 @@ def fri_commit_phase(ch, v, log_stop, inputs=None):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+# --------------------------------------------------- CPU baseline (timing) --
+# cpu_baseline.c: the reference's CPU commit path (Radix2DitParallel LDE +
+# MerkleTreeMmcs, Montgomery form, AVX2, OpenMP), timed by bench.py and
+# pinned to the functions above by tests/test_oracle.py.
+def cpu_to_monty(field, mat):
+    """Canonical matrix -> Montgomery words (the input form the timed commit takes)."""
+    a = _u32(mat)
+    out = np.empty_like(a)
+    lib().cb_to_monty(field, _ptr(a), _ptr(out), a.size)
+    return out
+
+
+def cpu_pcs_commit_mont(field, mont, log_blowup, want_lde=False):
+    """TwoAdicFriPcs::commit of one trace held in Montgomery words:
+    (root, LDE or None), both Montgomery, the LDE in committed row order."""
+    mont = _u32(mont)
+    h, w = mont.shape
+    root = np.zeros(8, np.uint32)
+    lde = np.empty((h << log_blowup, w), np.uint32) if want_lde else None
+    rc = lib().cb_pcs_commit(field, _ptr(mont), h, w, log_blowup, _ptr(root),
+                             _ptr(lde) if want_lde else None)
+    if rc != 0:
+        raise MemoryError("cb_pcs_commit failed (shape or allocation)")
+    return root, lde
+
+
+def cpu_pcs_commit(field, mat, log_blowup, want_lde=False):
+    """The same from canonical values; canonical outputs."""
+    root, lde = cpu_pcs_commit_mont(field, cpu_to_monty(field, mat), log_blowup, want_lde)
+    return from_monty(field, root), (from_monty(field, lde) if want_lde else None)
+
+
+def cpu_num_threads():
+    return lib().cb_num_threads()
+
+
+def cpu_last_phase_seconds():
+    """(LDE seconds, Merkle seconds) of the last cpu_pcs_commit* call."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().cb_last_phase_seconds(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value

@@ -459,3 +459,19 @@ def test_oracle_reduce_matches_verifier_loop(oracle, f):
                         term = pyref.ext_mul(f, term, dens[z2])
                 rhs = [(a + b) % p for a, b in zip(rhs, term)]
             assert lhs == rhs, (lh, i)
+
+
+@pytest.mark.parametrize("f", [0, 1])
+def test_cpu_baseline_matches_oracle(oracle, f):
+    """The CPU baseline bench.py times (oracle/cpu_baseline.c: Montgomery form,
+    AVX2, radix-4 passes, blocked layers) commits exactly what the checker
+    commits: the whole LDE and the root, over heights 1..2^13 (whole-matrix,
+    paired and blocked DIF layers), widths that leave a partial sponge chunk
+    and partial 8-row groups, blowups 1..8."""
+    for (L, w, lb) in [(0, 1, 0), (0, 3, 1), (1, 5, 1), (2, 9, 3), (3, 8, 2), (9, 13, 1), (10, 16, 3),
+                       (11, 24, 1), (12, 7, 2), (13, 9, 1)]:
+        a = oracle.random_matrix(f, 100 + L, 1 << L, w)
+        root, lde = oracle.cpu_pcs_commit(f, a, lb, want_lde=True)
+        ref = oracle.coset_lde(f, a, lb, oracle.generator(f))
+        assert np.array_equal(lde, ref), (L, w, lb)
+        assert np.array_equal(root, oracle.MerkleTree(f, [ref]).root), (L, w, lb)
