@@ -319,6 +319,61 @@ int pg_open_proof_encode(const pg_open_proof *p, int canonical, uint8_t *out, si
 int pg_open_proof_decode(int field, const uint8_t *bytes, size_t len, int canonical, pg_open_proof **out);
 void pg_open_proof_free(pg_open_proof *p);
 
+/* ---- quotient evaluation (compute_quotient_values) ------------------------
+ * The reference evaluates every chip's constraints on its quotient domain
+ * Q = GEN <w_|Q|>, |Q| = n 2^log_quotient_degree, from the main (and
+ * preprocessed) trace read back with get_evaluations_on_domain, folding them
+ * as acc = acc * alpha + c in assert order and multiplying by 1 / Z_H(x)
+ * (vm/src/machine/utils.rs:201-356, called per chip at
+ * vm/src/machine/prover.rs:441-508; folder vm/src/machine/folder.rs:222-226;
+ * selectors vm/src/compiler/recursion/circuit/domain.rs:46-60:
+ * is_first_row = Z_H(x) / (x - 1), is_last_row = Z_H(x) / (x - w_n^-1),
+ * is_transition = x - w_n^-1, Z_H(x) = x^n - 1).  Here the constraints cross
+ * the boundary as the expression DAG Pico's SymbolicConstraintFolder
+ * collects (p3 SymbolicExpression, vm/src/machine/folder.rs:30-120): one
+ * pg_air_node per expression node, operands before their users, and the
+ * asserted nodes in assert order.  pg_air_compile turns them into a
+ * register-allocated program; pg_quotient_values runs it on the device for
+ * every row of Q, reading the local row and the row 2^log_quotient_degree
+ * steps ahead (the reference's next_step) straight from the committed LDEs
+ * (no readback), and writes the |Q| x 4 quotient values (ext4, natural
+ * order on Q: exactly compute_quotient_values' output, whose
+ * flatten_to_base is the same memory; split it with pg_buf_view for the
+ * quotient commit).  This slice covers base-field constraints over main and
+ * preprocessed columns, public values, constants and the three selectors
+ * (the permutation trace's extension-field constraints are not covered). */
+#define PG_AIR_MAIN 0       /* a = row offset (0 local, 1 next), b = column */
+#define PG_AIR_PREPROCESSED 1 /* a = row offset, b = column */
+#define PG_AIR_PUBLIC 2     /* b = public value index */
+#define PG_AIR_IS_FIRST_ROW 3
+#define PG_AIR_IS_LAST_ROW 4
+#define PG_AIR_IS_TRANSITION 5
+#define PG_AIR_CONSTANT 6   /* a = canonical value */
+#define PG_AIR_ADD 7        /* a, b = operand node indices (earlier nodes) */
+#define PG_AIR_SUB 8
+#define PG_AIR_NEG 9        /* a = operand node index */
+#define PG_AIR_MUL 10
+typedef struct {
+    uint32_t kind, a, b;
+} pg_air_node;
+typedef struct pg_air_program pg_air_program;
+/* Checks the DAG (kinds, operand order, asserted nodes in range) and
+ * allocates registers over the emission order: for each assert in turn, the
+ * nodes it needs that are not yet computed, then the fold.  A register is
+ * reused once its value has no later reader.  nregs / ninstr may be null. */
+int pg_air_compile(int field, const pg_air_node *nodes, size_t nnodes, const uint32_t *asserts, size_t nasserts,
+                   pg_air_program **out, uint32_t *nregs, uint32_t *ninstr);
+void pg_air_program_free(pg_air_program *prog);
+/* main: matrix main_k of a committed tree at blowup 2^log_blowup (trace
+ * height n = its height >> log_blowup); pre: the preprocessed matrix (may be
+ * null when the program reads none), same n.  Requires log_quotient_degree <=
+ * log_blowup (Q is then a sub-coset of the committed LDE coset).  publics:
+ * Montgomery words.  out: a new |Q| x 4 buffer. */
+int pg_quotient_values(pg_ctx *ctx, const pg_air_program *prog, const pg_tree *main, size_t main_k,
+                       const pg_tree *pre, size_t pre_k, uint32_t log_blowup, uint32_t log_quotient_degree,
+                       const uint32_t *publics_mont, size_t npublics, const uint32_t alpha_mont[4],
+                       pg_buf **out);
+
 /* ---- batched commits ------------------------------------------------------
  * Many independent single-matrix Pcs::commit jobs (the reference proves the
  * shards of a batch in parallel rayon workers, vm/src/instances/machine/

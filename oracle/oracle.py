@@ -381,3 +381,154 @@ def cpu_last_phase_seconds():
     a, b = ctypes.c_double(), ctypes.c_double()
     lib().cb_last_phase_seconds(ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+# ---------------------------------------------------------- quotient values --
+# compute_quotient_values (vm/src/machine/utils.rs:201-356) restated over the
+# expression DAG directly, vectorised over the rows of the quotient coset
+# Q = GEN <w_Q> in natural order; the fold acc = acc * alpha + c follows
+# vm/src/machine/folder.rs:222-226 and the selectors
+# vm/src/compiler/recursion/circuit/domain.rs:46-60.  Node kinds as in the
+# C ABI (pg_air_node): 0 main, 1 preprocessed (a = row offset, b = column),
+# 2 public (b), 3 is_first_row, 4 is_last_row, 5 is_transition, 6 constant
+# (a), 7 add, 8 sub, 9 neg (a), 10 mul (a, b = earlier nodes).
+_W = {KOALABEAR: 3, BABYBEAR: 11}
+
+
+def _ext_mul_rows(f, a, b):
+    """a (n x 4) times b (n x 4 or 4), mod p, in F[x]/(x^4 - W)."""
+    p = np.uint64(PRIMES[f])
+    w = np.uint64(_W[f])
+    a = a.astype(np.uint64)
+    b = np.broadcast_to(np.asarray(b, np.uint64), a.shape)
+    c = [np.zeros(a.shape[0], np.uint64) for _ in range(7)]
+    for i in range(4):
+        for j in range(4):
+            c[i + j] = (c[i + j] + a[:, i] * b[:, j] % p) % p
+    out = np.empty_like(a)
+    for k in range(4):
+        out[:, k] = (c[k] + (w * c[k + 4] % p if k + 4 < 7 else 0)) % p
+    return out
+
+
+def quotient_values(field, nodes, asserts, main_q, log_n, log_qd, alpha, publics=(), pre_q=None):
+    """nodes (k x 3), asserts: the DAG; main_q / pre_q: the traces'
+    evaluations on Q, natural order (|Q| x width, canonical).  Returns the
+    |Q| x 4 quotient values (canonical)."""
+    p = PRIMES[field]
+    P = np.uint64(p)
+    q = main_q.shape[0]
+    assert q == 1 << (log_n + log_qd)
+    step = 1 << log_qd
+    g = generator(field)
+    wq = two_adic_generator(field, log_n + log_qd)
+    xs = np.empty(q, np.uint64)
+    cur = g
+    for i in range(q):
+        xs[i] = cur
+        cur = cur * wq % p
+    n = 1 << log_n
+    zh = np.array([(pow(int(x), n, p) - 1) % p for x in xs], np.uint64)
+    inv = np.vectorize(lambda v: pow(int(v), p - 2, p), otypes=[np.uint64])
+    w_last = pow(two_adic_generator(field, log_n), p - 2, p)
+    trans = (xs + P - np.uint64(w_last)) % P
+    first = zh * inv((xs + P - np.uint64(1)) % P) % P
+    last = zh * inv(trans) % P
+    nxt = (np.arange(q) + step) % q
+    mains = [main_q.astype(np.uint64), main_q[nxt].astype(np.uint64)]
+    pres = [pre_q.astype(np.uint64), pre_q[nxt].astype(np.uint64)] if pre_q is not None else None
+    vals = []
+    for kind, a, b in np.asarray(nodes, np.uint64).reshape(-1, 3).tolist():
+        if kind == 0:
+            v = mains[a][:, b]
+        elif kind == 1:
+            v = pres[a][:, b]
+        elif kind == 2:
+            v = np.full(q, publics[b] % p, np.uint64)
+        elif kind == 3:
+            v = first
+        elif kind == 4:
+            v = last
+        elif kind == 5:
+            v = trans
+        elif kind == 6:
+            v = np.full(q, a % p, np.uint64)
+        elif kind == 7:
+            v = (vals[a] + vals[b]) % P
+        elif kind == 8:
+            v = (vals[a] + P - vals[b]) % P
+        elif kind == 9:
+            v = (P - vals[a]) % P
+        elif kind == 10:
+            v = vals[a] * vals[b] % P
+        else:
+            raise ValueError(kind)
+        vals.append(v)
+    acc = np.zeros((q, 4), np.uint64)
+    for k in asserts:
+        acc = _ext_mul_rows(field, acc, alpha)
+        acc[:, 0] = (acc[:, 0] + vals[k]) % P
+    return (acc * inv(zh)[:, None] % P).astype(np.uint32)
+
+
+def constraints_at_point(field, nodes, asserts, main_local, main_next, zeta, log_n, alpha, publics=(),
+                         pre_local=None, pre_next=None):
+    """The verifier's side at one ext4 point (vm/src/machine/verifier.rs:
+    307-356): the folded constraints and 1 / Z_H(zeta), from the trace
+    polynomials' values at zeta and zeta w_n (ext4 per column)."""
+    p = PRIMES[field]
+
+    def mul(a, b):
+        return [int(x) for x in ext_mul(field, a, b)]
+
+    def add(a, b):
+        return [(x + y) % p for x, y in zip(a, b)]
+
+    def sub(a, b):
+        return [(x - y) % p for x, y in zip(a, b)]
+
+    def base(v):
+        return [v % p, 0, 0, 0]
+
+    z = [int(x) for x in zeta]
+    zn = z
+    for _ in range(log_n):
+        zn = mul(zn, zn)
+    zh = sub(zn, base(1))
+    w_last = pow(two_adic_generator(field, log_n), p - 2, p)
+    trans = sub(z, base(w_last))
+    first = mul(zh, [int(x) for x in ext_inv(field, sub(z, base(1)))])
+    last = mul(zh, [int(x) for x in ext_inv(field, trans)])
+    mains = [main_local, main_next]
+    pres = [pre_local, pre_next]
+    vals = []
+    for kind, a, b in np.asarray(nodes, np.uint64).reshape(-1, 3).tolist():
+        if kind == 0:
+            v = [int(x) for x in mains[a][b]]
+        elif kind == 1:
+            v = [int(x) for x in pres[a][b]]
+        elif kind == 2:
+            v = base(publics[b])
+        elif kind == 3:
+            v = first
+        elif kind == 4:
+            v = last
+        elif kind == 5:
+            v = trans
+        elif kind == 6:
+            v = base(a)
+        elif kind == 7:
+            v = add(vals[a], vals[b])
+        elif kind == 8:
+            v = sub(vals[a], vals[b])
+        elif kind == 9:
+            v = sub(base(0), vals[a])
+        elif kind == 10:
+            v = mul(vals[a], vals[b])
+        else:
+            raise ValueError(kind)
+        vals.append(v)
+    acc = base(0)
+    for k in asserts:
+        acc = add(mul(acc, [int(x) for x in alpha]), vals[k])
+    return acc, [int(x) for x in ext_inv(field, zh)]

@@ -99,6 +99,9 @@ SIGNATURES = {
     "pg_open_proof_encode": (_i, [_P, _i, _P, _sz, _P]),
     "pg_open_proof_decode": (_i, [_i, _P, _sz, _i, _pp]),
     "pg_open_proof_free": (None, [_P]),
+    "pg_air_compile": (_i, [_i, _P, _sz, _P, _sz, _pp, _P, _P]),
+    "pg_air_program_free": (None, [_P]),
+    "pg_quotient_values": (_i, [_P, _P, _P, _sz, _P, _sz, _u32, _u32, _P, _sz, _P, _pp]),
     "pg_batch_create": (_i, [_P, _i, _P, _P, _P, _sz, _i, _pp]),
     "pg_batch_run": (_i, [_P, _P, _P]),
     "pg_batch_take_tree": (_i, [_P, _sz, _pp]),

@@ -3562,3 +3562,279 @@ extern "C" int pg_tune_import(const char* text) {
     for (auto& v : rows) lde_set_variant(v[0], v[1], v[2], v[3], v[4]);
     return PG_OK;
 }
+
+// ----------------------------------------------------- quotient values --
+struct pg_air_program {
+    int field = 0;
+    std::vector<QInstr> code;
+    uint32_t nregs = 0;
+    bool need_selectors = false, reads_pre = false;
+    uint32_t max_main_col = 0, max_pre_col = 0, max_public = 0;  // one past the largest
+};
+
+extern "C" void pg_air_program_free(pg_air_program* prog) { delete prog; }
+
+extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes, const uint32_t* asserts,
+                              size_t nasserts, pg_air_program** out, uint32_t* nregs_out, uint32_t* ninstr_out) {
+    if (!out || !field_ok(field) || (nnodes && !nodes) || (nasserts && !asserts))
+        return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    if (nnodes > (1u << 30)) return fail(PG_E_INVALID, "too many nodes");
+    for (size_t k = 0; k < nnodes; k++) {
+        const pg_air_node& nd = nodes[k];
+        switch (nd.kind) {
+            case PG_AIR_MAIN:
+            case PG_AIR_PREPROCESSED:
+                if (nd.a > 1) return fail(PG_E_INVALID, "node %zu: row offset %u (0 local, 1 next)", k, nd.a);
+                break;
+            case PG_AIR_ADD:
+            case PG_AIR_SUB:
+            case PG_AIR_MUL:
+                if (nd.a >= k || nd.b >= k) return fail(PG_E_INVALID, "node %zu: operands must be earlier nodes", k);
+                break;
+            case PG_AIR_NEG:
+                if (nd.a >= k) return fail(PG_E_INVALID, "node %zu: operand must be an earlier node", k);
+                break;
+            case PG_AIR_PUBLIC:
+            case PG_AIR_IS_FIRST_ROW:
+            case PG_AIR_IS_LAST_ROW:
+            case PG_AIR_IS_TRANSITION:
+            case PG_AIR_CONSTANT:
+                break;
+            default:
+                return fail(PG_E_INVALID, "node %zu: unknown kind %u", k, nd.kind);
+        }
+    }
+    for (size_t j = 0; j < nasserts; j++)
+        if (asserts[j] >= nnodes) return fail(PG_E_INVALID, "assert %zu: node %u out of range", j, asserts[j]);
+    // Leaves (column loads, constants, selectors, public values) are not
+    // kept in registers: each use loads its own copy just before the
+    // instruction that reads it, so the register count follows the depth of
+    // the expressions, not the number of columns an AIR reads.
+    auto is_leaf = [&](uint32_t k) { return nodes[k].kind < PG_AIR_ADD; };
+    auto is_binary = [&](uint32_t k) {
+        return nodes[k].kind == PG_AIR_ADD || nodes[k].kind == PG_AIR_SUB || nodes[k].kind == PG_AIR_MUL;
+    };
+    // emission order: per assert, its not yet computed inner nodes (operands
+    // first), then the fold
+    struct Item {
+        bool is_assert;
+        uint32_t node;
+    };
+    std::vector<Item> order;
+    std::vector<uint8_t> state(nnodes, 0);  // 0 new, 1 on stack, 2 emitted
+    std::vector<uint32_t> stack;
+    for (size_t j = 0; j < nasserts; j++) {
+        if (!is_leaf(asserts[j])) stack.push_back(asserts[j]);
+        while (!stack.empty()) {
+            const uint32_t k = stack.back();
+            if (state[k] == 2) {
+                stack.pop_back();
+                continue;
+            }
+            const pg_air_node& nd = nodes[k];
+            if (state[k] == 0) {
+                state[k] = 1;
+                if (is_binary(k) && !is_leaf(nd.b) && state[nd.b] != 2) stack.push_back(nd.b);
+                if (!is_leaf(nd.a) && state[nd.a] != 2) stack.push_back(nd.a);
+                continue;
+            }
+            state[k] = 2;
+            stack.pop_back();
+            order.push_back(Item{false, k});
+        }
+        order.push_back(Item{true, asserts[j]});
+    }
+    // last reader of every computed inner node
+    std::vector<size_t> last_use(nnodes, 0);
+    for (size_t t = 0; t < order.size(); t++) {
+        const uint32_t k = order[t].node;
+        if (order[t].is_assert) {
+            last_use[k] = t;
+        } else {
+            if (!is_leaf(nodes[k].a)) last_use[nodes[k].a] = t;
+            if (is_binary(k) && !is_leaf(nodes[k].b)) last_use[nodes[k].b] = t;
+        }
+    }
+    std::unique_ptr<pg_air_program> prog(new pg_air_program);
+    prog->field = field;
+    std::vector<uint32_t> reg(nnodes, 0), free_regs;
+    auto take_reg = [&]() {
+        if (free_regs.empty()) return prog->nregs++;
+        auto it = std::min_element(free_regs.begin(), free_regs.end());
+        const uint32_t r = *it;
+        free_regs.erase(it);
+        return r;
+    };
+    auto to_m = [&](uint32_t canon) {
+        return field == PG_KOALABEAR ? to_monty<KoalaBear>(canon % KoalaBear::P) : to_monty<BabyBear>(canon % BabyBear::P);
+    };
+    // a leaf's load into register r
+    auto leaf_instr = [&](uint32_t k, uint32_t r) {
+        const pg_air_node& nd = nodes[k];
+        switch (nd.kind) {
+            case PG_AIR_MAIN:
+                prog->max_main_col = std::max(prog->max_main_col, nd.b + 1);
+                return QInstr{QI_MAIN, r, nd.a, nd.b};
+            case PG_AIR_PREPROCESSED:
+                prog->reads_pre = true;
+                prog->max_pre_col = std::max(prog->max_pre_col, nd.b + 1);
+                return QInstr{QI_PRE, r, nd.a, nd.b};
+            case PG_AIR_PUBLIC:  // the value is patched in at launch
+                prog->max_public = std::max(prog->max_public, nd.b + 1);
+                return QInstr{QI_CONST, r, 0, nd.b};
+            case PG_AIR_IS_FIRST_ROW: prog->need_selectors = true; return QInstr{QI_FIRST, r, 0, 0};
+            case PG_AIR_IS_LAST_ROW: prog->need_selectors = true; return QInstr{QI_LAST, r, 0, 0};
+            case PG_AIR_IS_TRANSITION: prog->need_selectors = true; return QInstr{QI_TRANS, r, 0, 0};
+            default: return QInstr{QI_CONST, r, to_m(nd.a), 0xFFFFFFFFu};
+        }
+    };
+    // the register holding operand k at step t: an inner node's, or a fresh
+    // load of a leaf (released after the instruction)
+    std::vector<uint32_t> temps;
+    auto operand = [&](uint32_t k) {
+        if (!is_leaf(k)) return reg[k];
+        const uint32_t r = take_reg();
+        prog->code.push_back(leaf_instr(k, r));
+        temps.push_back(r);
+        return r;
+    };
+    for (size_t t = 0; t < order.size(); t++) {
+        const uint32_t k = order[t].node;
+        const pg_air_node& nd = nodes[k];
+        temps.clear();
+        if (order[t].is_assert) {
+            const uint32_t r = operand(k);
+            prog->code.push_back(QInstr{QI_ASSERT, 0, r, 0});
+            for (uint32_t x : temps) free_regs.push_back(x);
+            if (!is_leaf(k) && last_use[k] == t) free_regs.push_back(reg[k]);
+            continue;
+        }
+        const uint32_t ra = operand(nd.a);
+        const uint32_t rb = is_binary(k) ? (nd.b == nd.a ? ra : operand(nd.b)) : 0;
+        const uint32_t op = nd.kind == PG_AIR_ADD ? QI_ADD
+                            : nd.kind == PG_AIR_SUB ? QI_SUB
+                            : nd.kind == PG_AIR_MUL ? QI_MUL
+                                                    : QI_NEG;
+        // operands are read before the result is written: registers whose
+        // values die here may take the result
+        for (uint32_t x : temps) free_regs.push_back(x);
+        if (!is_leaf(nd.a) && last_use[nd.a] == t) free_regs.push_back(reg[nd.a]);
+        if (is_binary(k) && nd.b != nd.a && !is_leaf(nd.b) && last_use[nd.b] == t) free_regs.push_back(reg[nd.b]);
+        reg[k] = take_reg();
+        prog->code.push_back(QInstr{op, reg[k], ra, rb});
+    }
+    if (prog->nregs > QUOTIENT_MAX_REGS)
+        return fail(PG_E_UNSUPPORTED, "the program needs %u registers; at most %u are supported", prog->nregs,
+                    QUOTIENT_MAX_REGS);
+    if (nregs_out) *nregs_out = prog->nregs;
+    if (ninstr_out) *ninstr_out = (uint32_t)prog->code.size();
+    *out = prog.release();
+    return PG_OK;
+}
+
+template <class F>
+static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, const pg_buf* mm, const pg_buf* pm,
+                      int log_n, uint32_t qdb, const uint32_t* publics, const uint32_t* alpha_w, pg_buf** out) {
+    const int log_q = log_n + (int)qdb;
+    const uint32_t q = 1u << log_q;
+    // program with the public values patched in, then the Z_H table
+    std::vector<QInstr> code = prog->code;
+    for (auto& in : code)
+        if (in.op == QI_CONST && in.b != 0xFFFFFFFFu) in.a = publics[in.b];
+    const uint32_t nz = 1u << qdb;
+    std::vector<uint32_t> zh(2 * nz);
+    const uint32_t gen = to_monty<F>(F::GEN);
+    const uint32_t gn = fpow<F>(gen, 1ull << log_n);  // GEN^n
+    const uint32_t wq = to_monty<F>(two_adic_gen_canon<F>((int)qdb));
+    uint32_t x = gn;
+    for (uint32_t j = 0; j < nz; j++) {  // x^n at x_i, i = j (mod 2^qdb): GEN^n w_{2^qdb}^j
+        zh[j] = fsub<F>(x, F::R1);
+        zh[nz + j] = finv<F>(zh[j]);
+        x = fmul<F>(x, wq);
+    }
+    const size_t b_code = round_up(code.size() * sizeof(QInstr), 256), b_zh = zh.size() * 4;
+    int rc;
+    std::shared_ptr<DevMem> tw;
+    PG_TRY(get_twiddles(ctx, s, prog->field, log_q, &tw));
+    auto desc = make_devmem(ctx, b_code + b_zh, &rc);
+    if (!desc) return rc;
+    pg_buf* o;
+    PG_TRY(buf_new(ctx, q, 4, &o));
+    SyncGuard guard(s);
+    uint8_t* dd = (uint8_t*)desc->ptr;
+    hipError_t e = hipSuccess;
+    if (!code.empty()) e = hipMemcpyAsync(dd, code.data(), code.size() * sizeof(QInstr), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dd + b_code, zh.data(), b_zh, hipMemcpyHostToDevice, s);
+    QuotientArgs a{};
+    a.main = mm->ptr;
+    a.ld_main = mm->ld;
+    a.pre = pm ? pm->ptr : nullptr;
+    a.ld_pre = pm ? pm->ld : 0;
+    a.prog = (const QInstr*)dd;
+    a.nprog = (uint32_t)code.size();
+    a.nregs = prog->nregs;
+    a.tw = (const uint32_t*)tw->ptr;
+    a.zh = (const uint32_t*)(dd + b_code);
+    a.zh_mask = nz - 1;
+    a.q = q;
+    a.log_q = (uint32_t)log_q;
+    a.next_step = nz;
+    a.shift = gen;
+    a.omega_n_inv = finv<F>(to_monty<F>(two_adic_gen_canon<F>(log_n)));
+    a.need_selectors = prog->need_selectors;
+    a.alpha = ext_from<F>(alpha_w);
+    a.out = o->ptr;
+    auto kt = make_timer(ctx);
+    if (e == hipSuccess) {
+        if (kt) kt->begin("quotient", s);
+        e = launch_quotient(prog->field, a, s);
+        if (kt) kt->end(s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        pg_buf_free(o);
+        return fail(PG_E_DEVICE, "quotient evaluation failed: %s", hipGetErrorString(e));
+    }
+    guard.armed = false;
+    *out = o;
+    return PG_OK;
+}
+
+extern "C" int pg_quotient_values(pg_ctx* ctx, const pg_air_program* prog, const pg_tree* main, size_t main_k,
+                                  const pg_tree* pre, size_t pre_k, uint32_t log_blowup, uint32_t log_qd,
+                                  const uint32_t* publics, size_t npublics, const uint32_t alpha[4], pg_buf** out) {
+    if (!ctx || !prog || !main || !alpha || !out || main_k >= main->mats.size())
+        return fail(PG_E_INVALID, "bad argument");
+    *out = nullptr;
+    if (main->field != prog->field) return fail(PG_E_INVALID, "the program and the trace are over different fields");
+    const pg_buf* mm = main->mats[main_k];
+    if (!is_pow2(mm->rows) || log2_of(mm->rows) < (int)log_blowup)
+        return fail(PG_E_INVALID, "main matrix height %zu has no trace domain at log_blowup %u", mm->rows, log_blowup);
+    const int log_n = log2_of(mm->rows) - (int)log_blowup;
+    if (log_qd > log_blowup)
+        return fail(PG_E_UNSUPPORTED, "log_quotient_degree %u > log_blowup %u: the quotient domain is not inside "
+                                      "the committed LDE", log_qd, log_blowup);
+    if (log_n + (int)log_qd > 32 - 1) return fail(PG_E_UNSUPPORTED, "quotient domain too large");
+    if (prog->max_main_col > mm->cols)
+        return fail(PG_E_INVALID, "the program reads main column %u of %zu", prog->max_main_col - 1, mm->cols);
+    const pg_buf* pm = nullptr;
+    if (prog->reads_pre) {
+        if (!pre || pre_k >= pre->mats.size()) return fail(PG_E_INVALID, "the program reads a preprocessed trace");
+        if (pre->field != prog->field) return fail(PG_E_INVALID, "preprocessed trace over another field");
+        pm = pre->mats[pre_k];
+        if (pm->rows != mm->rows)
+            return fail(PG_E_INVALID, "preprocessed height %zu differs from the main trace's %zu", pm->rows, mm->rows);
+        if (prog->max_pre_col > pm->cols)
+            return fail(PG_E_INVALID, "the program reads preprocessed column %u of %zu", prog->max_pre_col - 1,
+                        pm->cols);
+    }
+    if (prog->max_public > npublics || (npublics && !publics))
+        return fail(PG_E_INVALID, "the program reads public value %u of %zu", prog->max_public - 1, npublics);
+    PG_TRY(bind_device(ctx));
+    Lease l(ctx);
+    return prog->field == PG_KOALABEAR
+               ? quotient_t<KoalaBear>(ctx, l.s, prog, mm, pm, log_n, log_qd, publics, alpha, out)
+               : quotient_t<BabyBear>(ctx, l.s, prog, mm, pm, log_n, log_qd, publics, alpha, out);
+}

@@ -190,4 +190,34 @@ hipError_t launch_rows_bitrev(const uint32_t* src, uint64_t ld, uint32_t w, int 
 hipError_t launch_fill_random(int field, uint64_t seed, uint32_t* out, size_t n, int monty,
                               hipStream_t stream);
 
+
+// ---- quotient (quotient.hip) ----------------------------------------------
+// One compiled constraint instruction.  MAIN / PRE: a = row offset (0 local,
+// 1 next), b = column; CONST: a = Montgomery value; ADD / SUB / MUL: a, b =
+// registers; NEG: a = register; ASSERT: acc = acc * alpha + register a.
+// Every instruction but ASSERT writes register dst.
+enum : uint32_t {
+    QI_MAIN = 0, QI_PRE, QI_CONST, QI_FIRST, QI_LAST, QI_TRANS, QI_ADD, QI_SUB, QI_NEG, QI_MUL, QI_ASSERT
+};
+struct QInstr {
+    uint32_t op, dst, a, b;
+};
+constexpr uint32_t QUOTIENT_MAX_REGS = 256;  // 64 KiB of LDS per 64-thread workgroup
+struct QuotientArgs {
+    const uint32_t* main;  // committed LDE, bit-reversed rows
+    uint64_t ld_main;
+    const uint32_t* pre;   // may be null when the program reads no preprocessed column
+    uint64_t ld_pre;
+    const QInstr* prog;
+    uint32_t nprog, nregs;
+    const uint32_t* tw;    // w_Q^e, e < |Q|
+    const uint32_t* zh;    // Z_H at the 2^qdb values of x^n, then their inverses
+    uint32_t zh_mask;      // 2^qdb - 1
+    uint32_t q, log_q, next_step;
+    uint32_t shift, omega_n_inv;  // GEN and w_n^-1 (Montgomery)
+    int need_selectors;
+    Ext4 alpha;
+    uint32_t* out;         // |Q| x 4
+};
+hipError_t launch_quotient(int field, const QuotientArgs& a, hipStream_t stream);
 }  // namespace pg

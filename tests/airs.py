@@ -1,0 +1,115 @@
+"""AIRs for the quotient tests, each with a generator of valid traces
+(canonical values, n rows).  Written with pico_amd.air.AirBuilder, whose
+vocabulary is p3's AirBuilder as Pico's chips use it.
+
+  toy        the reference's ToyChip (vm/src/chips/chips/toys/toy.rs:150-161):
+             assert_bool(is_add); a + is_add b - (1 - is_add) b - result = 0.
+             Degree 2, log quotient degree 0.
+  fibonacci  first row (left, right) = (pv0, pv1); transition next.left =
+             right, next.right = left + right; last row right = pv2.  Every
+             selector and the next row; degree 2.
+  cube       first row (x, y) = (pv0, pv1); z = x^2 y on every row;
+             transition next.x = y, next.y = z; last row z = pv2.  Degree 3:
+             log quotient degree 1, so the next row is 2 steps ahead on Q.
+  gated      a preprocessed 0/1 column s: s (a - b) = 0 on every row and
+             (1 - s) (next.a - a) = 0 on transitions.
+"""
+import numpy as np
+
+from pico_amd.air import AirBuilder
+
+P = {0: 2130706433, 1: 2013265921}
+
+
+def toy_air(f):
+    b = AirBuilder(f)
+    a, bb, res, is_add = (b.main(i) for i in range(4))
+    b.assert_bool(is_add)
+    b.assert_zero(a + is_add * bb - (1 - is_add) * bb - res)
+    return b, 0  # builder, log quotient degree
+
+
+def toy_trace(f, n, rng):
+    p = P[f]
+    a = rng.integers(0, 256, n)
+    bb = rng.integers(0, 256, n)
+    is_add = rng.integers(0, 2, n)
+    res = np.where(is_add == 1, a + bb, (a - bb) % p)
+    t = np.stack([a, bb, res, is_add], 1).astype(np.uint64) % p
+    t[n - n // 4:] = 0  # padding rows, as pad_to_power_of_two leaves them
+    return t.astype(np.uint32), []
+
+
+def fibonacci_air(f):
+    b = AirBuilder(f)
+    left, right = b.main(0), b.main(1)
+    nleft, nright = b.main(0, next=True), b.main(1, next=True)
+    b.when_first_row().assert_eq(left, b.public(0))
+    b.when_first_row().assert_eq(right, b.public(1))
+    b.when_transition().assert_eq(right, nleft)
+    b.when_transition().assert_eq(left + right, nright)
+    b.when_last_row().assert_eq(right, b.public(2))
+    return b, 0
+
+
+def fibonacci_trace(f, n, rng):
+    p = P[f]
+    x, y = int(rng.integers(0, p)), int(rng.integers(0, p))
+    pv = [x, y]
+    rows = []
+    for _ in range(n):
+        rows.append((x, y))
+        x, y = y, (x + y) % p
+    pv.append(rows[-1][1])
+    return np.array(rows, np.uint32), pv
+
+
+def cube_air(f):
+    b = AirBuilder(f)
+    x, y, z = b.main(0), b.main(1), b.main(2)
+    b.when_first_row().assert_eq(x, b.public(0))
+    b.when_first_row().assert_eq(y, b.public(1))
+    b.assert_eq(z, x * x * y)
+    b.when_transition().assert_eq(b.main(0, next=True), y)
+    b.when_transition().assert_eq(b.main(1, next=True), z)
+    b.when_last_row().assert_eq(z, b.public(2))
+    return b, 1
+
+
+def cube_trace(f, n, rng):
+    p = P[f]
+    x, y = int(rng.integers(0, p)), int(rng.integers(0, p))
+    pv = [x, y]
+    rows = []
+    for _ in range(n):
+        z = x * x % p * y % p
+        rows.append((x, y, z))
+        x, y = y, z
+    pv.append(rows[-1][2])
+    return np.array(rows, np.uint32), pv
+
+
+def gated_air(f):
+    b = AirBuilder(f)
+    s, a, bb = b.preprocessed(0), b.main(0), b.main(1)
+    b.assert_zero(s * (a - bb))
+    b.when_transition().assert_zero((1 - s) * (b.main(0, next=True) - a))
+    return b, 0
+
+
+def gated_trace(f, n, rng):
+    """(main, preprocessed, publics)"""
+    p = P[f]
+    s = rng.integers(0, 2, n)
+    a = np.zeros(n, np.uint64)
+    cur = int(rng.integers(0, p))
+    for i in range(n):
+        a[i] = cur
+        if s[i] == 1:
+            cur = int(rng.integers(0, p))  # a row with s = 1 may change a on the next row
+    bb = np.where(s == 1, a, rng.integers(0, p, n).astype(np.uint64))
+    return np.stack([a, bb], 1).astype(np.uint32), s.astype(np.uint32).reshape(n, 1)
+
+
+AIRS = {"toy": (toy_air, toy_trace), "fibonacci": (fibonacci_air, fibonacci_trace),
+        "cube": (cube_air, cube_trace), "gated": (gated_air, None)}

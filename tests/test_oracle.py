@@ -475,3 +475,107 @@ def test_cpu_baseline_matches_oracle(oracle, f):
         ref = oracle.coset_lde(f, a, lb, oracle.generator(f))
         assert np.array_equal(lde, ref), (L, w, lb)
         assert np.array_equal(root, oracle.MerkleTree(f, [ref]).root), (L, w, lb)
+
+
+def _ext_scale(f, z, c):
+    p = oracle_mod().PRIMES[f]
+    return [int(x) * c % p for x in z]
+
+
+def oracle_mod():
+    from oracle import oracle as o
+    return o
+
+
+def _ext_from_coords(oracle, f, vals):
+    """sum_e x^e v_e for the four coordinate values v_e (ext4 each)"""
+    p = oracle.PRIMES[f]
+    acc = [0, 0, 0, 0]
+    for e, v in enumerate(vals):
+        mono = [0, 0, 0, 0]
+        mono[e] = 1
+        acc = [(a + b) % p for a, b in zip(acc, oracle.ext_mul(f, mono, v))]
+    return acc
+
+
+@pytest.mark.parametrize("f", [0, 1])
+@pytest.mark.parametrize("name", ["toy", "fibonacci", "cube", "gated"])
+def test_quotient_oracle_meets_the_verifier_identity(oracle, f, name):
+    """The oracle's compute_quotient_values, pinned by the verifier's
+    out-of-domain check (vm/src/machine/verifier.rs:307-356): for a valid
+    trace, the quotient interpolated from its values on Q and evaluated at a
+    random ext4 zeta equals the constraints folded at zeta (from the trace
+    polynomials at zeta and zeta w_n) times 1 / Z_H(zeta).  A trace with one
+    wrong entry breaks the identity."""
+    from tests import airs
+    rng = np.random.default_rng(40 + f)
+    log_n = 6
+    n = 1 << log_n
+    builder, qdb = getattr(airs, name + "_air")(f)
+    nodes, asserts = builder.node_array(), builder.asserts
+    if name == "gated":
+        trace, pre = airs.gated_trace(f, n, rng)
+        pv = []
+    else:
+        trace, pv = getattr(airs, name + "_trace")(f, n, rng)
+        pre = None
+    g = oracle.generator(f)
+    p = oracle.PRIMES[f]
+    wn = oracle.two_adic_generator(f, log_n)
+    zeta = [int(x) for x in rng.integers(0, p, 4)]
+    alpha = [int(x) for x in rng.integers(0, p, 4)]
+
+    def check(trace):
+        main_q = oracle.coset_lde(f, trace, qdb, g, bit_reverse=False)
+        pre_q = oracle.coset_lde(f, pre, qdb, g, bit_reverse=False) if pre is not None else None
+        quot = oracle.quotient_values(f, nodes, asserts, main_q, log_n, qdb, alpha, pv, pre_q)
+        # q(zeta): the values on Q (natural order) as a committed LDE at blowup 1
+        qn = quot.shape[0]
+        rev = [int(format(i, f"0{log_n + qdb}b")[::-1], 2) if log_n + qdb else 0 for i in range(qn)]
+        coords = oracle.pcs_open_values(f, quot[rev], 0, zeta)
+        q_at = _ext_from_coords(oracle, f, coords)
+        lde = oracle.coset_lde(f, trace, 1, g)
+        ml, mn = oracle.pcs_open_values(f, lde, 1, zeta), oracle.pcs_open_values(f, lde, 1, _ext_scale(f, zeta, wn))
+        pl = pn = None
+        if pre is not None:
+            plde = oracle.coset_lde(f, pre, 1, g)
+            pl, pn = oracle.pcs_open_values(f, plde, 1, zeta), oracle.pcs_open_values(f, plde, 1,
+                                                                                     _ext_scale(f, zeta, wn))
+        folded, inv_zh = oracle.constraints_at_point(f, nodes, asserts, ml, mn, zeta, log_n, alpha, pv, pl, pn)
+        return [int(x) for x in oracle.ext_mul(f, folded, inv_zh)] == q_at
+
+    assert check(trace)
+    bad = trace.copy()
+    bad[n // 2, 0] = (int(bad[n // 2, 0]) + 1) % p
+    assert not check(bad)
+
+
+def test_air_compile_reuses_registers():
+    """pg_air_compile: a chain of 200 asserts over shared subexpressions
+    needs few registers (each freed after its last reader, column loads
+    repeated at each use), keeps every assert, and rejects malformed DAGs."""
+    import pico_amd
+    from pico_amd.air import AirBuilder
+    b = AirBuilder(0)
+    acc = b.main(0)
+    for k in range(200):
+        acc = acc * b.main(k % 7) + k
+        b.assert_zero(acc - b.main(k % 5, next=True))
+    prog = b.compile()
+    assert prog.nregs <= 5
+    # one instruction per inner node and per assert, plus one load per use
+    # of a leaf operand
+    kinds = [kind for kind, _, _ in b.nodes]
+    want = len(b.asserts) + sum(1 for k in b.asserts if kinds[k] < 7)
+    for kind, a, bb in b.nodes:
+        if kind >= 7:
+            ops = [a] if (kind == 9 or a == bb) else [a, bb]
+            want += 1 + sum(1 for o in ops if kinds[o] < 7)
+    assert prog.ninstr == want
+    bad = np.array([[7, 1, 0]], np.uint32)  # add of nodes not yet defined
+    with pytest.raises(pico_amd.PgError):
+        pico_amd.air.AirProgram(0, bad, np.zeros(1, np.uint32))
+    with pytest.raises(pico_amd.PgError):
+        pico_amd.air.AirProgram(0, np.array([[0, 2, 0]], np.uint32), np.zeros(1, np.uint32))  # offset 2
+    with pytest.raises(pico_amd.PgError):
+        pico_amd.air.AirProgram(0, np.array([[0, 0, 0]], np.uint32), np.array([3], np.uint32))  # assert range
