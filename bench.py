@@ -181,6 +181,34 @@ def golden_root_check(h, w, lb, root):
     return [int(x) for x in root] == g["root"]
 
 
+def copy_calibration(pico_amd, reps=5):
+    """Five 1 GiB device-to-device copies through hipMemcpy (the runtime's
+    copy kernel reads 1 GiB and writes 1 GiB per dispatch)."""
+    import ctypes
+    ctx = pico_amd.Context(0)
+    a, b = ctx.matrix(1 << 18, 1024), ctx.matrix(1 << 18, 1024)  # 1 GiB each
+    hip = None
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = ctypes.CDLL(name)
+            break
+        except OSError:
+            continue
+    if hip is None:
+        raise OSError("libamdhip64.so not found")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    for _ in range(reps):
+        rc = hip.hipMemcpy(b.device_ptr, a.device_ptr, 1 << 30, 3)  # hipMemcpyDeviceToDevice
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    pico_amd.device_synchronize()
+    a.free()
+    b.free()
+    ctx.close()
+    print(json.dumps({"copy_calibration_bytes_per_dispatch": 1 << 30, "dispatches": reps}), flush=True)
+
+
 def progress(msg):
     """A line on stderr per finished section (stdout carries only the JSON)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -200,6 +228,9 @@ def main():
                     help="skip the shard+FRI, batched-jobs and served-stream sections")
     ap.add_argument("--commit-only", action="store_true",
                     help="skip the end-to-end and Pcs::open legs too (profiled runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only five plain 1 GiB device-to-device copies (the runtime's copy kernel): the "
+                         "known-size kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--served-jobs", type=int, default=32)
     ap.add_argument("--rates", type=str, default="5,20,50,200")
     args = ap.parse_args()
@@ -213,6 +244,10 @@ def main():
     import numpy as np
 
     import pico_amd
+
+    if args.copy_calibration:
+        copy_calibration(pico_amd)
+        return
 
     # Every C-ABI call returns after its device work has completed, so the
     # timed region is bracketed by completed device work on both sides.  torch
@@ -457,6 +492,8 @@ def main():
         progress("shard ingest done")
         out["chip_tables"] = workloads.chip_tables(ctx)
         progress("chip tables done")
+        out["quotient_eval"] = workloads.quotient_eval(ctx)
+        progress("quotient evaluation done")
         out["batched_jobs"] = workloads.batched_jobs(ctx)
         progress("batched jobs done")
         jobs = workloads.served_jobs(n=args.served_jobs)

@@ -17,6 +17,9 @@ device through the C ABI and reports wall time from the host clock.
   chip_tables        one shard's 40 chip tables (log-heights 10..22, mostly
                      small; widths 8..300) in one Pcs::commit vs an LDE
                      call per table then one MMCS commit
+  quotient_eval      compute_quotient_values on the device for a 256-column
+                     chip (2^20 rows, degree 3) against the readback it
+                     replaces
   batched_jobs       64 single-matrix commits, log-heights 10..18, widths
                      8..512, blowup 2/4/8: one launch sequence (hipGraph) vs
                      the same jobs committed one by one
@@ -267,6 +270,56 @@ def chip_tables(ctx, shapes=None, reps=3, log_blowup=1):
     for m in mats:
         m.free()
     return out
+
+
+# --------------------------------------------------------------- quotient --
+def quotient_eval(ctx, log_n=20, width=256, log_blowup=1, reps=3):
+    """compute_quotient_values on the device for one wide chip: a 2^log_n x
+    width trace committed at blowup 2^log_blowup, one degree-3 transition
+    constraint per column (main[c] main[c+1] main[c+2] - next.main[c] under
+    is_transition: log quotient degree 1, |Q| = 2^(log_n+1)), against the
+    readback of the same rows that the host-side evaluation needs
+    (get_evaluations_on_domain, prover.rs:467-490)."""
+    from . import air
+    f = core.KOALABEAR
+    b = air.AirBuilder(f)
+    for c in range(width):
+        x, y, z = b.main(c), b.main((c + 1) % width), b.main((c + 2) % width)
+        b.when_transition().assert_zero(x * y * z - b.main(c, next=True))
+    prog = b.compile()
+    trace = ctx.random_matrix(f, (2 << 32) | 0, 1 << log_n, width)
+    pcs = core.TwoAdicFriPcs(ctx, f, log_blowup)
+    _, pd = pcs.commit([(pcs.natural_domain_for_degree(1 << log_n), trace)])
+    trace.free()
+    alpha = [31, 41, 59, 26]
+    q = air.quotient_values(ctx, prog, pd, 0, log_blowup, 1, alpha)  # warm
+    q.free()
+    ts = []
+    for _ in range(reps):
+        core.device_synchronize()
+        t0 = time.perf_counter()
+        q = air.quotient_values(ctx, prog, pd, 0, log_blowup, 1, alpha)
+        ts.append(time.perf_counter() - t0)
+        q.free()
+    ctx.profile(True)
+    ctx.profile_reset()
+    q = air.quotient_values(ctx, prog, pd, 0, log_blowup, 1, alpha)
+    ctx.profile(False)
+    kms = ctx.profile_get("quotient")[0]
+    q.free()
+    rows_q = 1 << (log_n + 1)
+    host = np.zeros((rows_q, width), np.uint32)
+    rb = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        core.check(core.lib().pg_pcs_evaluations(ctx.handle, pd.handle, 0, rows_q, core._ptr(host)))
+        rb.append(time.perf_counter() - t0)
+    del host
+    pd.free()
+    return {"trace": [1 << log_n, width], "quotient_rows": rows_q, "constraints": width,
+            "program_instructions": prog.ninstr, "program_registers": prog.nregs,
+            "ms": min(ts) * 1e3, "kernel_ms": kms, "rows_per_s": rows_q / min(ts),
+            "readback_it_replaces_ms": min(rb) * 1e3}
 
 
 # ---------------------------------------------------------------- batched --
