@@ -285,6 +285,41 @@ PG_HD void internal_layer(uint32_t (&s)[16]) {
     s[15] = fmadd_small<F, CBIG>(s[15], sum);
 }
 
+// Internal layer with one Montgomery reduction per element, on lazy inputs
+// (s_0 < p, the others < 2p).  With S = sum s_j and S_m = S R (mod p) below
+// 2p:
+//   s_i <- REDC(Dm_i s_i + S_m),   Dm_i = D_i R (the Montgomery diagonal),
+// REDC(v) = v 2^-32 mod p.  v <= (p - 1)(2p - 1) + 2p - 1 < p 2^32, so the
+// result lies in [0, 2p) and is D_i s_i + S (mod p) for every entry alike:
+// three issue slots per element (a 64-bit multiply-add, a low multiply and
+// a 64-bit multiply-add).  The outputs stay lazy (< 2p).
+template <class F>
+PG_HD void internal_layer_redc(uint32_t (&s)[16]) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) t += s[i];  // < 31 p < 2^36
+    uint32_t sm;
+    if constexpr (F::P == KoalaBear::P) {
+        // q = t >> 31 <= 30 and t - q p = (t - q 2^31) + q (2^31 - p) < 2^31 + 30 (2^24 - 1) < 2p;
+        // then S R = REDC(S_r R^2), S_r R^2 < 2p p < p 2^32
+        const uint32_t q = (uint32_t)(t >> 31);
+        sm = fmul_lazy<F>((uint32_t)t - q * F::P, F::R2);
+    } else {
+        // BabyBear's 2^31 - p is too large for that bound: u = REDC(t) =
+        // S R^-1 (t < p 2^32, u < 2p), then S R = REDC(u R^3)
+        constexpr uint32_t R3 = (uint32_t)((uint64_t)F::R2 * F::R1 % F::P);
+        const uint32_t m = (uint32_t)t * F::MU;
+        const uint32_t u = (uint32_t)((t + (uint64_t)m * F::P) >> 32);
+        sm = fmul_lazy<F>(u, R3);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint64_t v = (uint64_t)P2<F>::K.diag[i] * s[i] + sm;
+        const uint32_t m = (uint32_t)v * F::MU;
+        s[i] = (uint32_t)((v + (uint64_t)m * F::P) >> 32);
+    }
+}
+
 template <class F>
 PG_HD void permute(uint32_t (&s)[16]) {
     if constexpr (F::P == KoalaBear::P) {
@@ -303,15 +338,15 @@ PG_HD void permute(uint32_t (&s)[16]) {
             }
             external_layer_wide<F>(s);
         }
-        s[0] = fadd<F>(s[0], P2<F>::K.int_rc[0]);
+        // partial rounds: the internal layer leaves every element below 2p;
+        // s_0 is reduced before its round constant and S-box
 #pragma unroll
         for (int r = 0; r < F::ROUNDS_P; r++) {
-            s[0] = sbox<F>(s[0]);
-            internal_layer<F>(s);
-            if (r + 1 < F::ROUNDS_P) s[0] = fadd<F>(s[0], P2<F>::K.int_rc[r + 1]);
+            s[0] = sbox<F>(fadd<F>(freduce<F>(s[0]), P2<F>::K.int_rc[r]));
+            internal_layer_redc<F>(s);
         }
 #pragma unroll
-        for (int i = 0; i < 16; i++) s[i] = fadd<F>(s[i], P2<F>::K.ext_rc[4][i]);
+        for (int i = 0; i < 16; i++) s[i] = fadd<F>(freduce<F>(s[i]), P2<F>::K.ext_rc[4][i]);
 #pragma unroll
         for (int r = 4; r < 8; r++) {
 #pragma unroll
@@ -332,9 +367,11 @@ PG_HD void permute(uint32_t (&s)[16]) {
     }
 #pragma unroll
     for (int r = 0; r < F::ROUNDS_P; r++) {
-        s[0] = sbox<F>(fadd<F>(s[0], P2<F>::K.int_rc[r]));
-        internal_layer<F>(s);
+        s[0] = sbox<F>(fadd<F>(freduce<F>(s[0]), P2<F>::K.int_rc[r]));
+        internal_layer_redc<F>(s);
     }
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = freduce<F>(s[i]);
 #pragma unroll
     for (int r = 4; r < 8; r++) {
 #pragma unroll
