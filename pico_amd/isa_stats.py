@@ -1,9 +1,10 @@
 """Build-time statistics of this package's own gfx950 kernels.
 
-Compiles merkle.hip to device assembly and counts the VALU instructions in the
-loop body of the single-matrix leaf-hash kernel (one sponge chunk = one
-Poseidon2 permutation plus its two 16-byte loads), per field, by opcode and by
-issue class.  Issue classes, from a per-opcode throughput microbenchmark on MI355X:
+Compiles merkle.hip to device assembly and counts the VALU instructions one
+trip of the chunk loop of the single-matrix leaf-hash kernel executes (one
+sponge chunk = one Poseidon2 permutation plus its two 16-byte loads; a loop
+nested in it, KoalaBear's partial rounds, counted once per trip), per field,
+by opcode and by issue class.  Issue classes, from a per-opcode throughput microbenchmark on MI355X:
   double   v_add_u32, v_sub_u32, v_subrev_u32, v_mov_b32, v_and_b32: up to
            two per lane per clock (measured 64-70 T/s against the 39.3 T/s
            single-issue ceiling), priced at half a slot
@@ -141,6 +142,27 @@ def compile_asm(hipcc, src, asm_path):
     return open(asm_path).read()
 
 
+def inner_loop(lines, body):
+    """The loop nested in `body` (the compiler's marker "Inner Loop Header:
+    Depth=2" to its backward branch), or []."""
+    start = next((i for i, l in enumerate(body) if "Loop Header: Depth=2" in l), None)
+    if start is None:
+        return []
+    m = re.match(r"^(\.LBB\w+):", body[start - 1] if not body[start].startswith(".LBB") else body[start])
+    label = m.group(1) if m else None
+    end = max((i for i, l in enumerate(body) if label and re.match(r"^\s+s_(?:cbranch_\w+|branch)\s+" + re.escape(label) + r"\b", l)),
+              default=None)
+    return body[start:end + 1] if end is not None else []
+
+
+def partial_trips():
+    """ROUNDS_P / P2_PARTIAL_UNROLL for KoalaBear (20 partial rounds), read
+    from poseidon2.h: the trips of the partial-round loop."""
+    src = open(os.path.join(HERE, "csrc", "poseidon2.h")).read()
+    unroll = int(re.search(r"constexpr int P2_PARTIAL_UNROLL = (\d+);", src).group(1))
+    return 20 // unroll
+
+
 def main(out_path):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
@@ -149,11 +171,13 @@ def main(out_path):
     stats = {}
     for field, prefix in KERNELS.items():
         lines = kernel_lines(asm, prefix)
-        hist = {}
-        for x in loop_body(lines):
-            op = opcode(x)
-            if op:
-                hist[op] = hist.get(op, 0) + 1
+        body = loop_body(lines)
+        hist = histogram(body)
+        # a nested loop (KoalaBear's partial rounds) runs `trips` times per chunk
+        inner = inner_loop(lines, body)
+        trips = partial_trips() if inner and field == "koalabear" else 1
+        for op, n in histogram(inner).items():
+            hist[op] = hist.get(op, 0) + (trips - 1) * n
         dbl = sum(n for op, n in hist.items() if op in DOUBLE_ISSUE)
         total = sum(hist.values())
         stats[field] = {
