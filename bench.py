@@ -204,9 +204,12 @@ def copy_calibration(pico_amd, reps=5):
     print(json.dumps({"copy_calibration_bytes_per_dispatch": 1 << 30, "dispatches": reps}), flush=True)
 
 
+T_START = time.perf_counter()
+
+
 def progress(msg):
     """A line on stderr per finished section (stdout carries only the JSON)."""
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -226,8 +229,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only five plain 1 GiB device-to-device copies (the runtime's copy kernel): the "
                          "known-size kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes")
-    ap.add_argument("--served-jobs", type=int, default=32)
-    ap.add_argument("--rates", type=str, default="5,20,50,200")
+    ap.add_argument("--served-jobs", type=int, default=48)
+    ap.add_argument("--rates", type=str, default="5,10,20,50,100,200")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -491,10 +494,17 @@ def main():
         out["batched_jobs"] = workloads.batched_jobs(ctx)
         progress("batched jobs done")
         jobs = workloads.served_jobs(n=args.served_jobs)
+        rates = [float(r) for r in args.rates.split(",")]
         served = {"jobs": [list(j) for j in jobs], "sequential": workloads.served_sequential(ctx, jobs)}
-        served["service"] = [workloads.served_stream(ctx, float(r), jobs) for r in args.rates.split(",")]
+        served["service"] = [workloads.served_stream(ctx, r, jobs) for r in rates]
+        progress("served stream (device-resident traces) done")
+        # the same jobs arriving in host memory: each job's upload is part of it
+        host = workloads.served_host_traces(ctx, jobs)
+        served["from_host"] = {"sequential": workloads.served_sequential(ctx, jobs, host_traces=host),
+                               "service": [workloads.served_stream(ctx, r, jobs, host_traces=host) for r in rates]}
+        del host
         out["served_stream"] = served
-        progress("served stream done")
+        progress("served stream (host traces) done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.log_h, w, lb, args.cpu_runs, out["root"])
         progress("CPU baseline done")

@@ -411,6 +411,15 @@ int pg_service_create(pg_ctx *ctx, size_t budget_bytes, int workers, pg_service 
 int pg_service_submit(pg_service *svc, int field, pg_buf *trace, uint32_t log_blowup, uint64_t *job_id);
 int pg_service_submit_synthetic(pg_service *svc, int field, uint64_t seed, size_t rows, size_t cols,
                                 uint32_t log_blowup, uint64_t *job_id);
+/* A trace in host memory, rows x cols words (Montgomery form, or canonical
+ * values when canonical != 0), as the reference's emulator hands a shard's
+ * records to the prover over its channel (riscv.rs:93-129; the traces are
+ * generated on the host, vm/src/machine/prover.rs:169-223).  The job runs
+ * pg_pcs_commit_host: the upload is part of the job and overlaps its LDE and
+ * hashing.  The caller keeps `host` unchanged until pg_service_wait returns
+ * for the job (or it is cancelled). */
+int pg_service_submit_host(pg_service *svc, int field, const uint32_t *host, size_t rows, size_t cols,
+                           int canonical, uint32_t log_blowup, uint64_t *job_id);
 int pg_service_wait(pg_service *svc, uint64_t job_id, uint32_t root[8], double times[3]);
 int pg_service_open(pg_service *svc, uint64_t job_id, const uint64_t *indices, size_t n_idx,
                     uint32_t *rows_out, uint32_t *paths_out);

@@ -110,6 +110,7 @@ SIGNATURES = {
     "pg_service_create": (_i, [_P, _sz, _i, _pp]),
     "pg_service_submit": (_i, [_P, _i, _P, _u32, _P]),
     "pg_service_submit_synthetic": (_i, [_P, _i, _u64, _sz, _sz, _u32, _P]),
+    "pg_service_submit_host": (_i, [_P, _i, _P, _sz, _sz, _i, _u32, _P]),
     "pg_service_wait": (_i, [_P, _u64, _P, _P]),
     "pg_service_open": (_i, [_P, _u64, _P, _sz, _P, _P]),
     "pg_service_release": (_i, [_P, _u64]),

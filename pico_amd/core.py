@@ -687,6 +687,7 @@ class CommitService:
         h = ctypes.c_void_p()
         check(lib().pg_service_create(ctx.handle, budget, workers, ctypes.byref(h)))
         self.handle = h
+        self._held = {}  # host traces of submitted jobs, until waited for
 
     def submit(self, field, trace, log_blowup):
         """Hands `trace` (a DeviceMatrix) to the service; returns a job id."""
@@ -694,6 +695,18 @@ class CommitService:
         check(lib().pg_service_submit(self.handle, field, trace.handle, log_blowup, ctypes.byref(jid)))
         trace._owned = False
         trace.handle = None
+        return jid.value
+
+    def submit_host(self, field, trace, log_blowup, canonical=True):
+        """A 2-D uint32 host array (canonical values, or Montgomery words with
+        canonical=False); the upload is part of the job.  The service keeps a
+        reference to the array until the job is waited for."""
+        arr = np.ascontiguousarray(trace, dtype=np.uint32)
+        arr = arr.reshape(-1, 1) if arr.ndim == 1 else arr
+        jid = ctypes.c_uint64()
+        check(lib().pg_service_submit_host(self.handle, field, _ptr(arr), arr.shape[0], arr.shape[1],
+                                           int(bool(canonical)), log_blowup, ctypes.byref(jid)))
+        self._held[jid.value] = arr
         return jid.value
 
     def submit_synthetic(self, field, seed, rows, cols, log_blowup):
@@ -706,7 +719,10 @@ class CommitService:
         """(root canonical[8], (t_submit, t_start, t_done))"""
         root = np.zeros(8, np.uint32)
         times = np.zeros(3, np.float64)
-        check(lib().pg_service_wait(self.handle, job_id, _ptr(root), _ptr(times)))
+        try:
+            check(lib().pg_service_wait(self.handle, job_id, _ptr(root), _ptr(times)))
+        finally:
+            self._held.pop(job_id, None)  # the job no longer reads its host trace
         return from_monty(field, root), tuple(times)
 
     def open(self, job_id, field, indices, total_width, log_height):

@@ -2502,6 +2502,8 @@ struct SvcJob {
     int field = 0;
     uint64_t seed = 0;
     pg_buf* trace = nullptr;  // owned input, or null for a seeded synthetic trace
+    const uint32_t* host = nullptr;  // or a caller-held host trace
+    int canonical = 0;
     size_t rows = 0, cols = 0;
     int lb = 1;
     size_t footprint = 0;
@@ -2542,6 +2544,16 @@ static size_t job_footprint(size_t rows, size_t cols, int lb, bool owns_trace) {
 static void run_svc_job(pg_service* sv, SvcJob* j, hipStream_t s) {
     pg_ctx* ctx = sv->ctx;
     (void)hipSetDevice(ctx->device);
+    if (j->host) {
+        // a trace in host memory: the host-memory commit (column blocks staged
+        // and uploaded while earlier blocks are extended and hashed)
+        const size_t rows = j->rows, cols = j->cols;
+        j->rc = pg_pcs_commit_host(ctx, j->field, &j->host, &rows, &cols, nullptr, 1, (uint32_t)j->lb, j->canonical,
+                                   j->root, &j->tree);
+        j->host = nullptr;  // the caller may reuse its memory once the job is done
+        if (j->rc != PG_OK) j->err = g_err;
+        return;
+    }
     pg_buf* tr = j->trace;
     int rc = PG_OK;
     if (!tr) {
@@ -2672,6 +2684,23 @@ extern "C" int pg_service_submit_synthetic(pg_service* sv, int field, uint64_t s
     std::unique_ptr<SvcJob> j(new SvcJob);
     j->field = field;
     j->seed = seed;
+    j->rows = rows;
+    j->cols = cols;
+    j->lb = (int)log_blowup;
+    j->footprint = job_footprint(rows, cols, j->lb, true);
+    return svc_enqueue(sv, std::move(j), id_out);
+}
+
+extern "C" int pg_service_submit_host(pg_service* sv, int field, const uint32_t* host, size_t rows, size_t cols,
+                                      int canonical, uint32_t log_blowup, uint64_t* id_out) {
+    if (!sv || !host || !id_out || !field_ok(field) || !is_pow2(rows) || cols == 0)
+        return fail(PG_E_INVALID, "bad argument");
+    int L = 0;
+    PG_TRY(lde_check(field, rows, cols, log_blowup, &L));
+    std::unique_ptr<SvcJob> j(new SvcJob);
+    j->field = field;
+    j->host = host;
+    j->canonical = canonical != 0;
     j->rows = rows;
     j->cols = cols;
     j->lb = (int)log_blowup;

@@ -381,10 +381,23 @@ def served_jobs(seed_base=5 << 32, n=40):
     return jobs
 
 
-def served_stream(ctx, rate, jobs, budget=200 * 10 ** 9, workers=3, openings=50, seed=5):
+def served_host_traces(ctx, jobs):
+    """The jobs' traces in ordinary (pageable) host memory, Montgomery words:
+    the same seeded matrices the synthetic jobs fill on the device, as a
+    shard's records reach the prover from the emulator."""
+    out = []
+    for k, (f, L, w, _) in enumerate(jobs):
+        m = ctx.random_matrix(f, (5 << 32) | (10000 + k), 1 << L, w)
+        out.append(m.to_host())
+        m.free()
+    return out
+
+
+def served_stream(ctx, rate, jobs, budget=200 * 10 ** 9, workers=3, openings=50, seed=5, host_traces=None):
     """Poisson arrivals at `rate` jobs/s into the commit service; each job is
     finished when its 50 openings have been answered.  Returns throughput and
-    latency percentiles (arrival -> last opening)."""
+    latency percentiles (arrival -> last opening).  With host_traces the jobs
+    arrive as host arrays and their upload is part of each job."""
     svc = core.CommitService(ctx, budget=budget, workers=workers)
     rng = SplitMix64((5 << 32) | seed)
     arrivals, t = [], 0.0
@@ -401,7 +414,10 @@ def served_stream(ctx, rate, jobs, budget=200 * 10 ** 9, workers=3, openings=50,
             delay = start + arrivals[k] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            ids[k] = svc.submit_synthetic(f, (5 << 32) | (10000 + k), 1 << L, w, b)
+            if host_traces is not None:
+                ids[k] = svc.submit_host(f, host_traces[k], b, canonical=False)
+            else:
+                ids[k] = svc.submit_synthetic(f, (5 << 32) | (10000 + k), 1 << L, w, b)
             ready.release()
 
     def finisher(k):
@@ -432,16 +448,21 @@ def served_stream(ctx, rate, jobs, budget=200 * 10 ** 9, workers=3, openings=50,
             "p99_ms": percentile(lat, 99) * 1e3, "peak_running": stats["peak_running"]}
 
 
-def served_sequential(ctx, jobs, openings=50):
+def served_sequential(ctx, jobs, openings=50, host_traces=None):
     """The plain path: each job committed, opened 50 times and released before
-    the next starts (all jobs present at time 0)."""
+    the next starts (all jobs present at time 0); from host_traces through
+    the host-memory commit when given."""
     start = time.perf_counter()
     lat = []
     for k, (f, L, w, b) in enumerate(jobs):
-        m = ctx.random_matrix(f, (5 << 32) | (10000 + k), 1 << L, w)
         pcs = core.TwoAdicFriPcs(ctx, f, b)
-        _, pd = pcs.commit([(pcs.natural_domain_for_degree(1 << L), m)])
-        m.free()
+        dom = pcs.natural_domain_for_degree(1 << L)
+        if host_traces is not None:
+            _, pd = pcs.commit_host([(dom, host_traces[k])], canonical=False)
+        else:
+            m = ctx.random_matrix(f, (5 << 32) | (10000 + k), 1 << L, w)
+            _, pd = pcs.commit([(dom, m)])
+            m.free()
         q = np.random.default_rng(k).integers(0, 1 << (L + b), openings)
         core.MerkleTreeMmcs(ctx, f).open_many(q, pd)
         pd.free()

@@ -6,7 +6,8 @@ recompute, against a verifier written from the reference's FRI circuit:
     widths up to 512 at log_blowup 1 and 3, submitted together to the commit
     service (vm/src/instances/machine/riscv.rs:93-129 hands shards to the
     prover the same way); every root equals the oracle's and the job's root
-    alone, and 50 openings per job verify against the oracle's root;
+    alone, and 50 openings per job verify against the oracle's root; the
+    same jobs again as host arrays, their upload part of each job;
   - 64 batched single-matrix jobs (log_h 10..18, widths 8..512, blowup 2..8):
     graph-replayed batch and one-by-one commits both equal the oracle;
   - one shard: 14 traces of 2^14..2^22 rows in one mixed-height commit and
@@ -67,6 +68,33 @@ def test_served_mix_matches_golden(ctx, oracle):
     assert stats["reserved_bytes"] == 0
     for k, (f, L, w, b) in enumerate(jobs):  # the same result alone
         assert [int(x) for x in _alone(ctx, f, (5 << 32) | (20000 + k), L, w, b)] == g["roots"][k], k
+
+
+def test_served_mix_from_host_matches_golden(ctx, oracle):
+    """The same served jobs arriving as host arrays (the emulator's records
+    reach the prover in host memory): the upload is part of each job, and
+    every root still equals the oracle's."""
+    g = _golden("served_mix.json")
+    jobs = [tuple(j) for j in g["jobs_field_log_h_width_log_blowup"]]
+    host = []
+    for k, (f, L, w, _) in enumerate(jobs):
+        m = ctx.random_matrix(f, (5 << 32) | (20000 + k), 1 << L, w)
+        host.append(m.to_host())  # Montgomery words
+        m.free()
+    svc = pico_amd.CommitService(ctx, budget=0, workers=3)
+    try:
+        ids = [svc.submit_host(f, host[k], b, canonical=False) for k, (f, L, w, b) in enumerate(jobs)]
+        for k, (f, L, w, b) in enumerate(jobs):
+            root, _ = svc.wait(ids[k], f)
+            assert [int(x) for x in root] == g["roots"][k], k
+            q = np.random.default_rng(k).integers(0, 1 << (L + b), 8)
+            rows, paths = svc.open(ids[k], f, q, w, L + b)
+            for j in range(8):
+                assert oracle.mmcs_verify(f, [1 << (L + b)], [w], g["roots"][k], int(q[j]), [rows[j]], paths[j])
+            svc.release(ids[k])
+        assert svc.stats()["peak_running"] >= 2
+    finally:
+        svc.close()
 
 
 def test_batched_jobs_match_golden(ctx):
