@@ -3596,7 +3596,8 @@ extern "C" int pg_tune_import(const char* text) {
 struct pg_air_program {
     int field = 0;
     std::vector<QInstr> code;
-    uint32_t nregs = 0;
+    std::vector<uint32_t> consts;  // Montgomery; the public values follow them at launch
+    uint32_t nregs = 0, nslots = 0, nasserts = 0, nstages = 0;
     bool need_selectors = false, reads_pre = false;
     uint32_t max_main_col = 0, max_pre_col = 0, max_public = 0;  // one past the largest
 };
@@ -3608,13 +3609,14 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
     if (!out || !field_ok(field) || (nnodes && !nodes) || (nasserts && !asserts))
         return fail(PG_E_INVALID, "bad argument");
     *out = nullptr;
-    if (nnodes > (1u << 30)) return fail(PG_E_INVALID, "too many nodes");
+    if (nnodes > (1u << 28) || nasserts > (1u << 28)) return fail(PG_E_INVALID, "too many nodes");
     for (size_t k = 0; k < nnodes; k++) {
         const pg_air_node& nd = nodes[k];
         switch (nd.kind) {
             case PG_AIR_MAIN:
             case PG_AIR_PREPROCESSED:
                 if (nd.a > 1) return fail(PG_E_INVALID, "node %zu: row offset %u (0 local, 1 next)", k, nd.a);
+                if (nd.b >= (1u << 24)) return fail(PG_E_INVALID, "node %zu: column %u", k, nd.b);
                 break;
             case PG_AIR_ADD:
             case PG_AIR_SUB:
@@ -3625,6 +3627,8 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
                 if (nd.a >= k) return fail(PG_E_INVALID, "node %zu: operand must be an earlier node", k);
                 break;
             case PG_AIR_PUBLIC:
+                if (nd.b >= (1u << 24)) return fail(PG_E_INVALID, "node %zu: public value %u", k, nd.b);
+                break;
             case PG_AIR_IS_FIRST_ROW:
             case PG_AIR_IS_LAST_ROW:
             case PG_AIR_IS_TRANSITION:
@@ -3636,10 +3640,10 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
     }
     for (size_t j = 0; j < nasserts; j++)
         if (asserts[j] >= nnodes) return fail(PG_E_INVALID, "assert %zu: node %u out of range", j, asserts[j]);
-    // Leaves (column loads, constants, selectors, public values) are not
-    // kept in registers: each use loads its own copy just before the
-    // instruction that reads it, so the register count follows the depth of
-    // the expressions, not the number of columns an AIR reads.
+    // Leaves (columns, constants, selectors, public values) are operands read
+    // in place: columns from staged windows, the rest from the constant table
+    // or the selectors.  Only inner nodes take registers, so the register
+    // count follows the depth of the expressions, not the columns read.
     auto is_leaf = [&](uint32_t k) { return nodes[k].kind < PG_AIR_ADD; };
     auto is_binary = [&](uint32_t k) {
         return nodes[k].kind == PG_AIR_ADD || nodes[k].kind == PG_AIR_SUB || nodes[k].kind == PG_AIR_MUL;
@@ -3685,73 +3689,191 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             if (is_binary(k) && !is_leaf(nodes[k].b)) last_use[nodes[k].b] = t;
         }
     }
-    std::unique_ptr<pg_air_program> prog(new pg_air_program);
-    prog->field = field;
-    std::vector<uint32_t> reg(nnodes, 0), free_regs;
-    auto take_reg = [&]() {
-        if (free_regs.empty()) return prog->nregs++;
-        auto it = std::min_element(free_regs.begin(), free_regs.end());
-        const uint32_t r = *it;
-        free_regs.erase(it);
-        return r;
-    };
-    auto to_m = [&](uint32_t canon) {
-        return field == PG_KOALABEAR ? to_monty<KoalaBear>(canon % KoalaBear::P) : to_monty<BabyBear>(canon % BabyBear::P);
-    };
-    // a leaf's load into register r
-    auto leaf_instr = [&](uint32_t k, uint32_t r) {
-        const pg_air_node& nd = nodes[k];
-        switch (nd.kind) {
-            case PG_AIR_MAIN:
-                prog->max_main_col = std::max(prog->max_main_col, nd.b + 1);
-                return QInstr{QI_MAIN, r, nd.a, nd.b};
-            case PG_AIR_PREPROCESSED:
-                prog->reads_pre = true;
-                prog->max_pre_col = std::max(prog->max_pre_col, nd.b + 1);
-                return QInstr{QI_PRE, r, nd.a, nd.b};
-            case PG_AIR_PUBLIC:  // the value is patched in at launch
-                prog->max_public = std::max(prog->max_public, nd.b + 1);
-                return QInstr{QI_CONST, r, 0, nd.b};
-            case PG_AIR_IS_FIRST_ROW: prog->need_selectors = true; return QInstr{QI_FIRST, r, 0, 0};
-            case PG_AIR_IS_LAST_ROW: prog->need_selectors = true; return QInstr{QI_LAST, r, 0, 0};
-            case PG_AIR_IS_TRANSITION: prog->need_selectors = true; return QInstr{QI_TRANS, r, 0, 0};
-            default: return QInstr{QI_CONST, r, to_m(nd.a), 0xFFFFFFFFu};
+    // A value used only by the item right after the one computing it is
+    // forwarded in a VGPR and takes no register; any value is forwarded to
+    // that next item.
+    std::vector<size_t> item_of(nnodes, SIZE_MAX);
+    for (size_t t = 0; t < order.size(); t++)
+        if (!order[t].is_assert) item_of[order[t].node] = t;
+    auto forward_only = [&](uint32_t k) { return last_use[k] == item_of[k] + 1; };
+    // the program for S window slots
+    auto build = [&](uint32_t S) {
+        std::unique_ptr<pg_air_program> prog(new pg_air_program);
+        prog->field = field;
+        prog->nasserts = (uint32_t)nasserts;
+        std::vector<uint32_t> reg(nnodes, 0), free_regs;
+        auto take_reg = [&]() {
+            if (free_regs.empty()) return prog->nregs++;
+            auto it = std::min_element(free_regs.begin(), free_regs.end());
+            const uint32_t r = *it;
+            free_regs.erase(it);
+            return r;
+        };
+        // Symbolic operands, encoded once the register count (which places
+        // the windows after the registers) and the constant count are known.
+        enum : uint8_t { OP_REG, OP_WIN, OP_WIN_NEXT, OP_CONST, OP_PUBLIC, OP_SEL };
+        struct Op {
+            uint8_t kind = OP_REG;
+            uint32_t x = 0;  // register, window word offset, constant, public or selector index
+        };
+        struct Pend {
+            uint32_t op, dst;
+            Op a, b;
+        };
+        std::vector<Pend> code;
+        std::map<uint32_t, uint32_t> const_index;
+        auto to_m = [&](uint32_t canon) {
+            return field == PG_KOALABEAR ? to_monty<KoalaBear>(canon % KoalaBear::P)
+                                         : to_monty<BabyBear>(canon % BabyBear::P);
+        };
+        // Staged windows: least recently used replaced; `keep` is a slot the
+        // instruction being emitted already reads (never replaced under it).
+        struct Win {
+            bool used = false;
+            uint32_t src = 0, col0 = 0;
+            uint64_t stamp = 0;
+        };
+        Win win[QUOTIENT_SLOTS];
+        uint64_t clock = 0;
+        auto window = [&](uint32_t src, uint32_t col, int keep) {
+            const uint32_t col0 = col / QW * QW;
+            int sl = -1;
+            for (int k = 0; k < (int)S; k++)
+                if (win[k].used && win[k].src == src && win[k].col0 == col0) sl = k;
+            if (sl < 0) {
+                for (int k = 0; k < (int)S && sl < 0; k++)
+                    if (!win[k].used) sl = k;
+                if (sl < 0)
+                    for (int k = 0; k < (int)S; k++)
+                        if (k != keep && (sl < 0 || win[k].stamp < win[sl].stamp)) sl = k;
+                win[sl].used = true;
+                win[sl].src = src;
+                win[sl].col0 = col0;
+                Pend st{QI_STAGE, (uint32_t)sl, {}, {}};
+                st.a.x = col0;
+                st.b.x = src;
+                code.push_back(st);
+                prog->nstages++;
+                prog->nslots = std::max(prog->nslots, (uint32_t)sl + 1);
+            }
+            win[sl].stamp = ++clock;
+            return sl;
+        };
+        // operand for node k read at item t; `slot` receives the window slot
+        // it reads (or -1)
+        auto operand = [&](uint32_t k, size_t t, int keep, int* slot) {
+            const pg_air_node& nd = nodes[k];
+            Op o;
+            *slot = -1;
+            if (!is_leaf(k)) {
+                if (item_of[k] + 1 == t) {
+                    o.kind = OP_SEL;
+                    o.x = QO_SEL_PREV;
+                } else {
+                    o.kind = OP_REG;
+                    o.x = reg[k];
+                }
+                return o;
+            }
+            switch (nd.kind) {
+                case PG_AIR_MAIN:
+                case PG_AIR_PREPROCESSED: {
+                    const bool pre = nd.kind == PG_AIR_PREPROCESSED;
+                    if (pre) {
+                        prog->reads_pre = true;
+                        prog->max_pre_col = std::max(prog->max_pre_col, nd.b + 1);
+                    } else {
+                        prog->max_main_col = std::max(prog->max_main_col, nd.b + 1);
+                    }
+                    const int sl = window((pre ? 2u : 0u) + nd.a, nd.b, keep);
+                    *slot = sl;
+                    o.kind = nd.a ? OP_WIN_NEXT : OP_WIN;
+                    o.x = (uint32_t)sl * QSLOT + (nd.b % QW) * QRS;
+                    return o;
+                }
+                case PG_AIR_PUBLIC:
+                    prog->max_public = std::max(prog->max_public, nd.b + 1);
+                    o.kind = OP_PUBLIC;
+                    o.x = nd.b;
+                    return o;
+                case PG_AIR_IS_FIRST_ROW:
+                case PG_AIR_IS_LAST_ROW:
+                case PG_AIR_IS_TRANSITION:
+                    prog->need_selectors = true;
+                    o.kind = OP_SEL;
+                    o.x = nd.kind == PG_AIR_IS_FIRST_ROW ? QO_SEL_FIRST
+                          : nd.kind == PG_AIR_IS_LAST_ROW ? QO_SEL_LAST
+                                                          : QO_SEL_TRANS;
+                    return o;
+                default: {
+                    const uint32_t v = to_m(nd.a);
+                    auto it = const_index.find(v);
+                    if (it == const_index.end()) {
+                        it = const_index.emplace(v, (uint32_t)prog->consts.size()).first;
+                        prog->consts.push_back(v);
+                    }
+                    o.kind = OP_CONST;
+                    o.x = it->second;
+                    return o;
+                }
+            }
+        };
+        for (size_t t = 0; t < order.size(); t++) {
+            const uint32_t k = order[t].node;
+            const pg_air_node& nd = nodes[k];
+            int sa = -1, sb = -1;
+            if (order[t].is_assert) {
+                Pend in{QI_ASSERT, QO_NO_DST, operand(k, t, -1, &sa), {}};
+                code.push_back(in);
+                if (!is_leaf(k) && last_use[k] == t && !forward_only(k)) free_regs.push_back(reg[k]);
+                continue;
+            }
+            const Op oa = operand(nd.a, t, -1, &sa);
+            const Op ob = is_binary(k) ? operand(nd.b, t, sa, &sb) : Op{};
+            const uint32_t op = nd.kind == PG_AIR_ADD ? QI_ADD
+                                : nd.kind == PG_AIR_SUB ? QI_SUB
+                                : nd.kind == PG_AIR_MUL ? QI_MUL
+                                                        : QI_NEG;
+            // operands are read before the result is written: registers whose
+            // values die here may take the result
+            if (!is_leaf(nd.a) && last_use[nd.a] == t && !forward_only(nd.a)) free_regs.push_back(reg[nd.a]);
+            if (is_binary(k) && nd.b != nd.a && !is_leaf(nd.b) && last_use[nd.b] == t && !forward_only(nd.b))
+                free_regs.push_back(reg[nd.b]);
+            const uint32_t dst = forward_only(k) ? QO_NO_DST : (reg[k] = take_reg());
+            code.push_back(Pend{op, dst, oa, ob});
         }
-    };
-    // the register holding operand k at step t: an inner node's, or a fresh
-    // load of a leaf (released after the instruction)
-    std::vector<uint32_t> temps;
-    auto operand = [&](uint32_t k) {
-        if (!is_leaf(k)) return reg[k];
-        const uint32_t r = take_reg();
-        prog->code.push_back(leaf_instr(k, r));
-        temps.push_back(r);
-        return r;
-    };
-    for (size_t t = 0; t < order.size(); t++) {
-        const uint32_t k = order[t].node;
-        const pg_air_node& nd = nodes[k];
-        temps.clear();
-        if (order[t].is_assert) {
-            const uint32_t r = operand(k);
-            prog->code.push_back(QInstr{QI_ASSERT, 0, r, 0});
-            for (uint32_t x : temps) free_regs.push_back(x);
-            if (!is_leaf(k) && last_use[k] == t) free_regs.push_back(reg[k]);
-            continue;
+        const uint32_t nconst = (uint32_t)prog->consts.size();
+        const uint32_t wbase = prog->nregs * QT;  // windows follow the registers
+        auto encode = [&](const Op& o) -> uint32_t {
+            switch (o.kind) {
+                case OP_REG: return QO_LDS | (o.x * QT);
+                case OP_WIN: return QO_LDS | (wbase + o.x);
+                case OP_WIN_NEXT: return QO_NEXT | (wbase + o.x);
+                case OP_CONST: return QO_CONST | o.x;
+                case OP_PUBLIC: return QO_CONST | (nconst + o.x);
+                default: return QO_SEL | o.x;
+            }
+        };
+        prog->code.reserve(code.size());
+        for (const Pend& in : code) {
+            if (in.op == QI_STAGE)
+                prog->code.push_back(QInstr{QI_STAGE, in.dst, in.a.x, in.b.x});
+            else
+                prog->code.push_back(QInstr{in.op, in.dst == QO_NO_DST ? QO_NO_DST : in.dst * QT, encode(in.a),
+                                            in.op == QI_ADD || in.op == QI_SUB || in.op == QI_MUL ? encode(in.b)
+                                                                                                  : 0u});
         }
-        const uint32_t ra = operand(nd.a);
-        const uint32_t rb = is_binary(k) ? (nd.b == nd.a ? ra : operand(nd.b)) : 0;
-        const uint32_t op = nd.kind == PG_AIR_ADD ? QI_ADD
-                            : nd.kind == PG_AIR_SUB ? QI_SUB
-                            : nd.kind == PG_AIR_MUL ? QI_MUL
-                                                    : QI_NEG;
-        // operands are read before the result is written: registers whose
-        // values die here may take the result
-        for (uint32_t x : temps) free_regs.push_back(x);
-        if (!is_leaf(nd.a) && last_use[nd.a] == t) free_regs.push_back(reg[nd.a]);
-        if (is_binary(k) && nd.b != nd.a && !is_leaf(nd.b) && last_use[nd.b] == t) free_regs.push_back(reg[nd.b]);
-        reg[k] = take_reg();
-        prog->code.push_back(QInstr{op, reg[k], ra, rb});
+        return prog;
+    };
+    // Fewest window slots (the most workgroups per CU) that stage at most
+    // 1/8 more windows than the most slots do.
+    std::unique_ptr<pg_air_program> prog = build(QUOTIENT_SLOTS);
+    for (uint32_t S = 2; S < QUOTIENT_SLOTS; S++) {
+        std::unique_ptr<pg_air_program> p = build(S);
+        if (p->nstages * 8 <= prog->nstages * 9) {
+            prog = std::move(p);
+            break;
+        }
     }
     if (prog->nregs > QUOTIENT_MAX_REGS)
         return fail(PG_E_UNSUPPORTED, "the program needs %u registers; at most %u are supported", prog->nregs,
@@ -3767,10 +3889,10 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
                       int log_n, uint32_t qdb, const uint32_t* publics, const uint32_t* alpha_w, pg_buf** out) {
     const int log_q = log_n + (int)qdb;
     const uint32_t q = 1u << log_q;
-    // program with the public values patched in, then the Z_H table
-    std::vector<QInstr> code = prog->code;
-    for (auto& in : code)
-        if (in.op == QI_CONST && in.b != 0xFFFFFFFFu) in.a = publics[in.b];
+    // constant table (the program's constants, then the public values), the
+    // Z_H table and the powers of alpha
+    std::vector<uint32_t> consts = prog->consts;
+    consts.insert(consts.end(), publics, publics + prog->max_public);
     const uint32_t nz = 1u << qdb;
     std::vector<uint32_t> zh(2 * nz);
     const uint32_t gen = to_monty<F>(F::GEN);
@@ -3782,29 +3904,53 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
         zh[nz + j] = finv<F>(zh[j]);
         x = fmul<F>(x, wq);
     }
-    const size_t b_code = round_up(code.size() * sizeof(QInstr), 256), b_zh = zh.size() * 4;
+    const uint32_t m = prog->nasserts;
+    std::vector<uint32_t> apow((size_t)m * 4);
+    {
+        const Ext4 al = ext_from<F>(alpha_w);
+        Ext4 pw{{F::R1, 0u, 0u, 0u}};
+        for (uint32_t j = m; j-- > 0;) {  // assert j is multiplied by alpha^(m-1-j)
+            for (int e = 0; e < 4; e++) apow[(size_t)j * 4 + e] = pw.c[e];
+            pw = emul<F>(pw, al);
+        }
+    }
+    // the kernel reads one instruction past the end (zero padding)
+    const size_t b_code = round_up((prog->code.size() + 1) * sizeof(QInstr), 256), b_zh = round_up(zh.size() * 4, 256),
+                 b_const = round_up(consts.size() * 4, 256), b_pow = apow.size() * 4;
     int rc;
     std::shared_ptr<DevMem> tw;
     PG_TRY(get_twiddles(ctx, s, prog->field, log_q, &tw));
-    auto desc = make_devmem(ctx, b_code + b_zh, &rc);
+    auto desc = make_devmem(ctx, b_code + b_zh + b_const + b_pow + 256, &rc);
     if (!desc) return rc;
     pg_buf* o;
     PG_TRY(buf_new(ctx, q, 4, &o));
     SyncGuard guard(s);
     uint8_t* dd = (uint8_t*)desc->ptr;
     hipError_t e = hipSuccess;
-    if (!code.empty()) e = hipMemcpyAsync(dd, code.data(), code.size() * sizeof(QInstr), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(dd + b_code, zh.data(), b_zh, hipMemcpyHostToDevice, s);
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(dd + off, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    std::vector<QInstr> code = prog->code;
+    code.push_back(QInstr{QI_STAGE, 0, 0, 0});
+    put(0, code.data(), code.size() * sizeof(QInstr));
+    put(b_code, zh.data(), zh.size() * 4);
+    put(b_code + b_zh, consts.data(), consts.size() * 4);
+    put(b_code + b_zh + b_const, apow.data(), b_pow);
     QuotientArgs a{};
     a.main = mm->ptr;
     a.ld_main = mm->ld;
+    a.w_main = (uint32_t)mm->cols;
     a.pre = pm ? pm->ptr : nullptr;
     a.ld_pre = pm ? pm->ld : 0;
+    a.w_pre = pm ? (uint32_t)pm->cols : 0;
     a.prog = (const QInstr*)dd;
-    a.nprog = (uint32_t)code.size();
+    a.nprog = (uint32_t)prog->code.size();
     a.nregs = prog->nregs;
-    a.tw = (const uint32_t*)tw->ptr;
+    a.nslots = prog->nslots;
     a.zh = (const uint32_t*)(dd + b_code);
+    a.consts = (const uint32_t*)(dd + b_code + b_zh);
+    a.apow = (const uint32_t*)(dd + b_code + b_zh + b_const);
+    a.tw = (const uint32_t*)tw->ptr;
     a.zh_mask = nz - 1;
     a.q = q;
     a.log_q = (uint32_t)log_q;
@@ -3812,7 +3958,6 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
     a.shift = gen;
     a.omega_n_inv = finv<F>(to_monty<F>(two_adic_gen_canon<F>(log_n)));
     a.need_selectors = prog->need_selectors;
-    a.alpha = ext_from<F>(alpha_w);
     a.out = o->ptr;
     auto kt = make_timer(ctx);
     if (e == hipSuccess) {

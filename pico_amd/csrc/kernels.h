@@ -192,32 +192,56 @@ hipError_t launch_fill_random(int field, uint64_t seed, uint32_t* out, size_t n,
 
 
 // ---- quotient (quotient.hip) ----------------------------------------------
-// One compiled constraint instruction.  MAIN / PRE: a = row offset (0 local,
-// 1 next), b = column; CONST: a = Montgomery value; ADD / SUB / MUL: a, b =
-// registers; NEG: a = register; ASSERT: acc = acc * alpha + register a.
-// Every instruction but ASSERT writes register dst.
-enum : uint32_t {
-    QI_MAIN = 0, QI_PRE, QI_CONST, QI_FIRST, QI_LAST, QI_TRANS, QI_ADD, QI_SUB, QI_NEG, QI_MUL, QI_ASSERT
-};
+// A workgroup of QT threads evaluates QT consecutive rows of the committed
+// LDE (one row per thread).  Its LDS holds the program's registers (register
+// r of thread t at word r QT + t) followed by `nslots` staged windows: QW
+// columns of the block's QT rows (local rows) or of the block of rows that
+// are their next rows, column k of a window at word k QRS + row.
+//
+// Instructions.  STAGE: dst = slot, a = first column, b = source (0 main
+// local, 1 main next, 2 preprocessed local, 3 preprocessed next).  ADD / SUB /
+// MUL: dst = operand a op operand b; NEG: dst = -operand a; ASSERT: the
+// accumulator gains operand a times the next power of alpha.  Operands are
+// codes whose top two bits give the kind: QO_LDS (the LDS word index + the
+// thread's row), QO_NEXT (LDS word index + the slot row of the thread's next
+// row), QO_CONST (index into the constant table: the AIR's constants, then
+// the public values), QO_SEL (is_first_row, is_last_row, is_transition, or
+// the previous arithmetic instruction's result, held in a VGPR).  dst of an
+// arithmetic instruction is the LDS word of its register, or QO_NO_DST for a
+// value only the next instruction reads (forwarded).
+enum : uint32_t { QI_STAGE = 0, QI_ADD, QI_SUB, QI_MUL, QI_NEG, QI_ASSERT };
+constexpr uint32_t QO_LDS = 0u << 30, QO_NEXT = 1u << 30, QO_CONST = 2u << 30, QO_SEL = 3u << 30;
+constexpr uint32_t QO_KIND = 3u << 30, QO_INDEX = ~(3u << 30);
+constexpr uint32_t QO_SEL_FIRST = 0, QO_SEL_LAST = 1, QO_SEL_TRANS = 2, QO_SEL_PREV = 3;
+constexpr uint32_t QO_NO_DST = 0xFFFFFFFFu;
 struct QInstr {
     uint32_t op, dst, a, b;
 };
-constexpr uint32_t QUOTIENT_MAX_REGS = 256;  // 64 KiB of LDS per 64-thread workgroup
+constexpr int QT = 64;             // rows (threads) per workgroup
+constexpr int QW = 16;             // columns per staged window (64-byte row segments)
+constexpr int QRS = QT + 4;        // LDS words per window column (conflict-free staging writes)
+constexpr int QSLOT = QW * QRS;    // LDS words per window
+constexpr uint32_t QUOTIENT_SLOTS = 4;
+constexpr uint32_t QUOTIENT_MAX_REGS = 176;  // with 4 windows: < 64 KiB of LDS per workgroup
 struct QuotientArgs {
     const uint32_t* main;  // committed LDE, bit-reversed rows
     uint64_t ld_main;
+    uint32_t w_main;
     const uint32_t* pre;   // may be null when the program reads no preprocessed column
     uint64_t ld_pre;
+    uint32_t w_pre;
     const QInstr* prog;
-    uint32_t nprog, nregs;
+    uint32_t nprog, nregs, nslots;
+    const uint32_t* consts;  // constant table (Montgomery)
+    const uint32_t* apow;    // alpha^(m-1-j) for assert j (ext4, Montgomery)
     const uint32_t* tw;    // w_Q^e, e < |Q|
     const uint32_t* zh;    // Z_H at the 2^qdb values of x^n, then their inverses
     uint32_t zh_mask;      // 2^qdb - 1
     uint32_t q, log_q, next_step;
     uint32_t shift, omega_n_inv;  // GEN and w_n^-1 (Montgomery)
     int need_selectors;
-    Ext4 alpha;
     uint32_t* out;         // |Q| x 4
 };
+size_t quotient_lds_bytes(uint32_t nregs, uint32_t nslots);
 hipError_t launch_quotient(int field, const QuotientArgs& a, hipStream_t stream);
 }  // namespace pg

@@ -111,5 +111,43 @@ def gated_trace(f, n, rng):
     return np.stack([a, bb], 1).astype(np.uint32), s.astype(np.uint32).reshape(n, 1)
 
 
+def random_air(f, rng, width, pre_width, nconstraints, npublics=3):
+    """Random constraints over `width` main and `pre_width` preprocessed
+    columns: sums, differences and products of up to four leaves (columns at
+    the local or next row spread over every column window, constants,
+    public values, selectors), some under a when_* filter.  Degree <= 5 with
+    a filter, so log quotient degree 2 covers it."""
+    b = AirBuilder(f)
+    p = P[f]
+
+    def leaf():
+        r = rng.integers(0, 10)
+        if r < 6 or (r < 8 and not pre_width):
+            return b.main(int(rng.integers(0, width)), next=bool(rng.integers(0, 2)))
+        if r < 8:
+            return b.preprocessed(int(rng.integers(0, pre_width)), next=bool(rng.integers(0, 2)))
+        if r == 8:
+            return b.public(int(rng.integers(0, npublics)))
+        return b.constant(int(rng.integers(0, p)))
+
+    for _ in range(nconstraints):
+        e = leaf()
+        for _ in range(int(rng.integers(0, 4))):
+            op = rng.integers(0, 3)
+            e = e * leaf() if op == 0 else e + leaf() if op == 1 else e - leaf()
+        if rng.integers(0, 5) == 0:
+            e = -e
+        sel = rng.integers(0, 5)
+        if sel == 0:
+            b.when_first_row().assert_zero(e)
+        elif sel == 1:
+            b.when_last_row().assert_zero(e)
+        elif sel == 2:
+            b.when_transition().assert_zero(e)
+        else:
+            b.assert_zero(e)
+    return b, 2
+
+
 AIRS = {"toy": (toy_air, toy_trace), "fibonacci": (fibonacci_air, fibonacci_trace),
         "cube": (cube_air, cube_trace), "gated": (gated_air, None)}

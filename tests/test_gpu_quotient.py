@@ -68,6 +68,41 @@ def test_quotient_matches_oracle(ctx, oracle, f, name):
                 assert np.array_equal(got, want), (name, log_n, lb, valid)
 
 
+@pytest.mark.parametrize("f", FIELDS)
+def test_quotient_wide_random_air_matches_oracle(ctx, oracle, f):
+    """Random constraints over 70 main and 21 preprocessed columns: the
+    program stages many column windows (more than its slots hold, so
+    windows are replaced and staged again), windows that run past the trace
+    width, local and next-row windows of both traces, constants and public
+    values.  Heights 2^0..2^9 (row blocks smaller than a workgroup up to
+    several blocks), quotient degree 2^0..2^2."""
+    rng = np.random.default_rng(31 + f)
+    p = pico_amd.PRIMES[f]
+    g = pico_amd.GENERATORS[f]
+    builder, _ = airs.random_air(f, rng, 70, 21, 120)
+    prog = builder.compile()
+    nodes, asserts = builder.node_array(), builder.asserts
+    for log_n in (0, 2, 5, 6, 9):
+        for qdb, lb in ((0, 1), (1, 1), (2, 2)):
+            n = 1 << log_n
+            print(f"wide quotient field {f} 2^{log_n} qdb={qdb} b={lb}", flush=True)
+            main = rng.integers(0, p, (n, 70)).astype(np.uint32)
+            pre = rng.integers(0, p, (n, 21)).astype(np.uint32)
+            pv = [int(x) for x in rng.integers(0, p, 3)]
+            alpha = [int(x) for x in rng.integers(0, p, 4)]
+            _, md = _commit(ctx, f, lb, main)
+            _, pd = _commit(ctx, f, lb, pre)
+            q = air.quotient_values(ctx, prog, md, 0, lb, qdb, alpha, pv, pd, 0)
+            got = q.to_host(f)
+            q.free()
+            md.free()
+            pd.free()
+            want = oracle.quotient_values(f, nodes, asserts, oracle.coset_lde(f, main, qdb, g, bit_reverse=False),
+                                          log_n, qdb, alpha, pv,
+                                          oracle.coset_lde(f, pre, qdb, g, bit_reverse=False))
+            assert np.array_equal(got, want), (log_n, qdb, lb)
+
+
 def test_quotient_rejects_bad_requests(ctx):
     f = KOALABEAR
     builder, _ = airs.cube_air(f)
