@@ -3843,10 +3843,10 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             code.push_back(Pend{op, dst, oa, ob});
         }
         const uint32_t nconst = (uint32_t)prog->consts.size();
-        const uint32_t wbase = prog->nregs * QT;  // windows follow the registers
+        const uint32_t wbase = prog->nregs * QB;  // windows follow the registers
         auto encode = [&](const Op& o) -> uint32_t {
             switch (o.kind) {
-                case OP_REG: return QO_LDS | (o.x * QT);
+                case OP_REG: return QO_LDS | (o.x * QB);
                 case OP_WIN: return QO_LDS | (wbase + o.x);
                 case OP_WIN_NEXT: return QO_NEXT | (wbase + o.x);
                 case OP_CONST: return QO_CONST | o.x;
@@ -3859,7 +3859,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             if (in.op == QI_STAGE)
                 prog->code.push_back(QInstr{QI_STAGE, in.dst, in.a.x, in.b.x});
             else
-                prog->code.push_back(QInstr{in.op, in.dst == QO_NO_DST ? QO_NO_DST : in.dst * QT, encode(in.a),
+                prog->code.push_back(QInstr{in.op, in.dst == QO_NO_DST ? QO_NO_DST : in.dst * QB, encode(in.a),
                                             in.op == QI_ADD || in.op == QI_SUB || in.op == QI_MUL ? encode(in.b)
                                                                                                   : 0u});
         }

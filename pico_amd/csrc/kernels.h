@@ -192,21 +192,22 @@ hipError_t launch_fill_random(int field, uint64_t seed, uint32_t* out, size_t n,
 
 
 // ---- quotient (quotient.hip) ----------------------------------------------
-// A workgroup of QT threads evaluates QT consecutive rows of the committed
-// LDE (one row per thread).  Its LDS holds the program's registers (register
-// r of thread t at word r QT + t) followed by `nslots` staged windows: QW
-// columns of the block's QT rows (local rows) or of the block of rows that
-// are their next rows, column k of a window at word k QRS + row.
+// A workgroup (one wave of QT threads) evaluates QB = QT QR consecutive rows
+// of the committed LDE; thread t owns rows t + QT k, k < QR.  Its LDS holds
+// the program's registers (register r of row u at word r QB + u) followed by
+// `nslots` staged windows: QW columns of the block's QB rows (local rows) or
+// of the block of rows that are their next rows, column c of a window at
+// word c QRS + row.
 //
 // Instructions.  STAGE: dst = slot, a = first column, b = source (0 main
 // local, 1 main next, 2 preprocessed local, 3 preprocessed next).  ADD / SUB /
 // MUL: dst = operand a op operand b; NEG: dst = -operand a; ASSERT: the
 // accumulator gains operand a times the next power of alpha.  Operands are
 // codes whose top two bits give the kind: QO_LDS (the LDS word index + the
-// thread's row), QO_NEXT (LDS word index + the slot row of the thread's next
-// row), QO_CONST (index into the constant table: the AIR's constants, then
-// the public values), QO_SEL (is_first_row, is_last_row, is_transition, or
-// the previous arithmetic instruction's result, held in a VGPR).  dst of an
+// row), QO_NEXT (LDS word index + the slot row of the row's next row),
+// QO_CONST (index into the constant table: the AIR's constants, then the
+// public values), QO_SEL (is_first_row, is_last_row, is_transition, or the
+// previous arithmetic instruction's result, held in VGPRs).  dst of an
 // arithmetic instruction is the LDS word of its register, or QO_NO_DST for a
 // value only the next instruction reads (forwarded).
 enum : uint32_t { QI_STAGE = 0, QI_ADD, QI_SUB, QI_MUL, QI_NEG, QI_ASSERT };
@@ -217,12 +218,14 @@ constexpr uint32_t QO_NO_DST = 0xFFFFFFFFu;
 struct QInstr {
     uint32_t op, dst, a, b;
 };
-constexpr int QT = 64;             // rows (threads) per workgroup
-constexpr int QW = 16;             // columns per staged window (64-byte row segments)
-constexpr int QRS = QT + 4;        // LDS words per window column (conflict-free staging writes)
-constexpr int QSLOT = QW * QRS;    // LDS words per window
+constexpr int QT = 64;               // threads per workgroup (one wave)
+constexpr int QR = 4;                // rows per thread
+constexpr int QB = QT * QR;          // rows per workgroup
+constexpr int QW = 4;                // columns per staged window (16-byte row segments)
+constexpr int QRS = QB + QT / QW;    // LDS words per window column (conflict-free staging writes)
+constexpr int QSLOT = QW * QRS;      // LDS words per window
 constexpr uint32_t QUOTIENT_SLOTS = 4;
-constexpr uint32_t QUOTIENT_MAX_REGS = 176;  // with 4 windows: < 64 KiB of LDS per workgroup
+constexpr uint32_t QUOTIENT_MAX_REGS = (65536 / 4 - QUOTIENT_SLOTS * QSLOT) / QB;  // < 64 KiB of LDS
 struct QuotientArgs {
     const uint32_t* main;  // committed LDE, bit-reversed rows
     uint64_t ld_main;

@@ -560,13 +560,13 @@ def test_air_compile_reuses_registers():
     b = AirBuilder(0)
     acc = b.main(0)
     for k in range(200):
-        acc = acc * b.main(k % 7) + k
-        b.assert_zero(acc - b.main(k % 5, next=True))
+        acc = acc * b.main(k % 2) + k
+        b.assert_zero(acc - b.main(1 - k % 2, next=True))
     prog = b.compile()
     assert prog.nregs <= 3
     # one instruction per inner node and per assert, plus the staging of the
-    # two column windows read (main columns 0..6 at the local row, 0..4 at
-    # the next row), which four window slots hold for the whole program
+    # two column windows read (main columns 0 and 1 at the local row and at
+    # the next row), which the window slots hold for the whole program
     kinds = [kind for kind, _, _ in b.nodes]
     assert prog.ninstr == len(b.asserts) + sum(1 for k in kinds if k >= 7) + 2
     bad = np.array([[7, 1, 0]], np.uint32)  # add of nodes not yet defined
