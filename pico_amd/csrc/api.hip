@@ -3712,6 +3712,8 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
         // Symbolic operands, encoded once the register count (which places
         // the windows after the registers) and the constant count are known.
         enum : uint8_t { OP_REG, OP_WIN, OP_WIN_NEXT, OP_CONST, OP_PUBLIC, OP_SEL };
+        enum : uint32_t { P_STAGE, P_ADD, P_SUB, P_MUL, P_NEG, P_ASSERT };  // pending instructions
+        enum : uint32_t { S_FIRST, S_LAST, S_TRANS, S_PREV };               // OP_SEL operands
         struct Op {
             uint8_t kind = OP_REG;
             uint32_t x = 0;  // register, window word offset, constant, public or selector index
@@ -3749,7 +3751,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
                 win[sl].used = true;
                 win[sl].src = src;
                 win[sl].col0 = col0;
-                Pend st{QI_STAGE, (uint32_t)sl, {}, {}};
+                Pend st{P_STAGE, (uint32_t)sl, {}, {}};
                 st.a.x = col0;
                 st.b.x = src;
                 code.push_back(st);
@@ -3768,7 +3770,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             if (!is_leaf(k)) {
                 if (item_of[k] + 1 == t) {
                     o.kind = OP_SEL;
-                    o.x = QO_SEL_PREV;
+                    o.x = S_PREV;
                 } else {
                     o.kind = OP_REG;
                     o.x = reg[k];
@@ -3801,9 +3803,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
                 case PG_AIR_IS_TRANSITION:
                     prog->need_selectors = true;
                     o.kind = OP_SEL;
-                    o.x = nd.kind == PG_AIR_IS_FIRST_ROW ? QO_SEL_FIRST
-                          : nd.kind == PG_AIR_IS_LAST_ROW ? QO_SEL_LAST
-                                                          : QO_SEL_TRANS;
+                    o.x = nd.kind == PG_AIR_IS_FIRST_ROW ? S_FIRST : nd.kind == PG_AIR_IS_LAST_ROW ? S_LAST : S_TRANS;
                     return o;
                 default: {
                     const uint32_t v = to_m(nd.a);
@@ -3823,17 +3823,17 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             const pg_air_node& nd = nodes[k];
             int sa = -1, sb = -1;
             if (order[t].is_assert) {
-                Pend in{QI_ASSERT, QO_NO_DST, operand(k, t, -1, &sa), {}};
+                Pend in{P_ASSERT, QO_NO_DST, operand(k, t, -1, &sa), {}};
                 code.push_back(in);
                 if (!is_leaf(k) && last_use[k] == t && !forward_only(k)) free_regs.push_back(reg[k]);
                 continue;
             }
             const Op oa = operand(nd.a, t, -1, &sa);
             const Op ob = is_binary(k) ? operand(nd.b, t, sa, &sb) : Op{};
-            const uint32_t op = nd.kind == PG_AIR_ADD ? QI_ADD
-                                : nd.kind == PG_AIR_SUB ? QI_SUB
-                                : nd.kind == PG_AIR_MUL ? QI_MUL
-                                                        : QI_NEG;
+            const uint32_t op = nd.kind == PG_AIR_ADD ? P_ADD
+                                : nd.kind == PG_AIR_SUB ? P_SUB
+                                : nd.kind == PG_AIR_MUL ? P_MUL
+                                                        : P_NEG;
             // operands are read before the result is written: registers whose
             // values die here may take the result
             if (!is_leaf(nd.a) && last_use[nd.a] == t && !forward_only(nd.a)) free_regs.push_back(reg[nd.a]);
@@ -3842,26 +3842,44 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             const uint32_t dst = forward_only(k) ? QO_NO_DST : (reg[k] = take_reg());
             code.push_back(Pend{op, dst, oa, ob});
         }
+        // the LDS layout (registers, windows, selectors, constants) and the
+        // device encoding: operand kinds folded into the opcode
         const uint32_t nconst = (uint32_t)prog->consts.size();
-        const uint32_t wbase = prog->nregs * QB;  // windows follow the registers
-        auto encode = [&](const Op& o) -> uint32_t {
+        const uint32_t wbase = prog->nregs * QB;
+        const uint32_t sbase = wbase + prog->nslots * QSLOT;
+        const uint32_t cbase = sbase + (prog->need_selectors ? 3u * QB : 0u);
+        auto kind = [&](const Op& o) -> uint32_t {
             switch (o.kind) {
-                case OP_REG: return QO_LDS | (o.x * QB);
-                case OP_WIN: return QO_LDS | (wbase + o.x);
-                case OP_WIN_NEXT: return QO_NEXT | (wbase + o.x);
-                case OP_CONST: return QO_CONST | o.x;
-                case OP_PUBLIC: return QO_CONST | (nconst + o.x);
-                default: return QO_SEL | o.x;
+                case OP_WIN_NEXT: return QK_NEXT;
+                case OP_CONST:
+                case OP_PUBLIC: return QK_BCAST;
+                case OP_SEL: return o.x == S_PREV ? QK_PREV : QK_ROW;
+                default: return QK_ROW;
+            }
+        };
+        auto index = [&](const Op& o) -> uint32_t {
+            switch (o.kind) {
+                case OP_REG: return o.x * QB;
+                case OP_WIN:
+                case OP_WIN_NEXT: return wbase + o.x;
+                case OP_CONST: return cbase + o.x;
+                case OP_PUBLIC: return cbase + nconst + o.x;
+                default: return o.x == S_PREV ? 0u : sbase + o.x * QB;
             }
         };
         prog->code.reserve(code.size());
         for (const Pend& in : code) {
-            if (in.op == QI_STAGE)
-                prog->code.push_back(QInstr{QI_STAGE, in.dst, in.a.x, in.b.x});
-            else
-                prog->code.push_back(QInstr{in.op, in.dst == QO_NO_DST ? QO_NO_DST : in.dst * QB, encode(in.a),
-                                            in.op == QI_ADD || in.op == QI_SUB || in.op == QI_MUL ? encode(in.b)
-                                                                                                  : 0u});
+            const uint32_t dst = in.dst == QO_NO_DST ? QO_NO_DST : in.dst * QB;
+            switch (in.op) {
+                case P_STAGE: prog->code.push_back(QInstr{QOP_STAGE, in.dst, in.a.x, in.b.x}); break;
+                case P_NEG: prog->code.push_back(QInstr{QOP_NEG + kind(in.a), dst, index(in.a), 0}); break;
+                case P_ASSERT: prog->code.push_back(QInstr{QOP_ASSERT + kind(in.a), 0, index(in.a), 0}); break;
+                default: {
+                    const uint32_t op = in.op == P_ADD ? 0u : in.op == P_SUB ? 1u : 2u;
+                    prog->code.push_back(
+                        QInstr{16 * op + 4 * kind(in.a) + kind(in.b), dst, index(in.a), index(in.b)});
+                }
+            }
         }
         return prog;
     };
@@ -3904,6 +3922,9 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
         zh[nz + j] = finv<F>(zh[j]);
         x = fmul<F>(x, wq);
     }
+    if (quotient_lds_bytes(prog->nregs, prog->nslots, prog->need_selectors, (uint32_t)consts.size()) > 65536)
+        return fail(PG_E_UNSUPPORTED, "the program's registers, windows and %zu constants need more than 64 KiB of LDS",
+                    consts.size());
     const uint32_t m = prog->nasserts;
     std::vector<uint32_t> apow((size_t)m * 4);
     {
@@ -3931,7 +3952,7 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dd + off, src, bytes, hipMemcpyHostToDevice, s);
     };
     std::vector<QInstr> code = prog->code;
-    code.push_back(QInstr{QI_STAGE, 0, 0, 0});
+    code.push_back(QInstr{QOP_STAGE, 0, 0, 0});
     put(0, code.data(), code.size() * sizeof(QInstr));
     put(b_code, zh.data(), zh.size() * 4);
     put(b_code + b_zh, consts.data(), consts.size() * 4);
@@ -3949,6 +3970,8 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
     a.nslots = prog->nslots;
     a.zh = (const uint32_t*)(dd + b_code);
     a.consts = (const uint32_t*)(dd + b_code + b_zh);
+    a.nconsts = (uint32_t)consts.size();
+    a.nasserts = m;
     a.apow = (const uint32_t*)(dd + b_code + b_zh + b_const);
     a.tw = (const uint32_t*)tw->ptr;
     a.zh_mask = nz - 1;

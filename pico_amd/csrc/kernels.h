@@ -193,35 +193,38 @@ hipError_t launch_fill_random(int field, uint64_t seed, uint32_t* out, size_t n,
 
 // ---- quotient (quotient.hip) ----------------------------------------------
 // A workgroup (one wave of QT threads) evaluates QB = QT QR consecutive rows
-// of the committed LDE; thread t owns rows t + QT k, k < QR.  Its LDS holds
-// the program's registers (register r of row u at word r QB + u) followed by
-// `nslots` staged windows: QW columns of the block's QB rows (local rows) or
-// of the block of rows that are their next rows, column c of a window at
-// word c QRS + row.
+// of the committed LDE; thread t owns rows t + QT k, k < QR.  Its LDS holds,
+// in this order: the program's registers (register r of row u at word
+// r QB + u); `nslots` staged windows (QW columns of the block's QB rows, or
+// of the block of rows that are their next rows; column c of a window at
+// word c QRS + row); the selectors is_first_row, is_last_row, is_transition
+// (QB words each) when the program reads them; the constant table (the
+// AIR's constants, then the public values).
 //
-// Instructions.  STAGE: dst = slot, a = first column, b = source (0 main
-// local, 1 main next, 2 preprocessed local, 3 preprocessed next).  ADD / SUB /
-// MUL: dst = operand a op operand b; NEG: dst = -operand a; ASSERT: the
-// accumulator gains operand a times the next power of alpha.  Operands are
-// codes whose top two bits give the kind: QO_LDS (the LDS word index + the
-// row), QO_NEXT (LDS word index + the slot row of the row's next row),
-// QO_CONST (index into the constant table: the AIR's constants, then the
-// public values), QO_SEL (is_first_row, is_last_row, is_transition, or the
-// previous arithmetic instruction's result, held in VGPRs).  dst of an
-// arithmetic instruction is the LDS word of its register, or QO_NO_DST for a
-// value only the next instruction reads (forwarded).
-enum : uint32_t { QI_STAGE = 0, QI_ADD, QI_SUB, QI_MUL, QI_NEG, QI_ASSERT };
-constexpr uint32_t QO_LDS = 0u << 30, QO_NEXT = 1u << 30, QO_CONST = 2u << 30, QO_SEL = 3u << 30;
-constexpr uint32_t QO_KIND = 3u << 30, QO_INDEX = ~(3u << 30);
-constexpr uint32_t QO_SEL_FIRST = 0, QO_SEL_LAST = 1, QO_SEL_TRANS = 2, QO_SEL_PREV = 3;
+// An operand is an LDS word index read with one of four row offsets: QK_ROW
+// (the row itself: registers, windows, selectors), QK_NEXT (the slot of the
+// row's next row in a next-row window), QK_BCAST (none: every row reads the
+// same constant) or QK_PREV (no read: the previous arithmetic result, held
+// in VGPRs).  The host folds the operation and the operand kinds into the
+// opcode, so the device runs one straight-line handler per instruction:
+//   16 op + 4 ka + kb   ADD (op 0), SUB (1), MUL (2) of operands a, b
+//   QOP_NEG + ka        -a
+//   QOP_ASSERT + ka     the accumulator gains a times the next power of alpha
+//   QOP_STAGE           stage columns [a, a + QW) of source b (0 main local,
+//                       1 main next, 2 preprocessed local, 3 preprocessed
+//                       next) into window slot dst
+// dst of an arithmetic instruction is the LDS word of its register, or
+// QO_NO_DST for a value only the next instruction reads (forwarded).
+constexpr uint32_t QK_ROW = 0, QK_NEXT = 1, QK_BCAST = 2, QK_PREV = 3;
+constexpr uint32_t QOP_NEG = 48, QOP_ASSERT = 52, QOP_STAGE = 56;
 constexpr uint32_t QO_NO_DST = 0xFFFFFFFFu;
 struct QInstr {
     uint32_t op, dst, a, b;
 };
 constexpr int QT = 64;               // threads per workgroup (one wave)
-constexpr int QR = 4;                // rows per thread
+constexpr int QR = 2;                // rows per thread
 constexpr int QB = QT * QR;          // rows per workgroup
-constexpr int QW = 4;                // columns per staged window (16-byte row segments)
+constexpr int QW = 8;                // columns per staged window (32-byte row segments)
 constexpr int QRS = QB + QT / QW;    // LDS words per window column (conflict-free staging writes)
 constexpr int QSLOT = QW * QRS;      // LDS words per window
 constexpr uint32_t QUOTIENT_SLOTS = 4;
@@ -235,8 +238,10 @@ struct QuotientArgs {
     uint32_t w_pre;
     const QInstr* prog;
     uint32_t nprog, nregs, nslots;
-    const uint32_t* consts;  // constant table (Montgomery)
+    const uint32_t* consts;  // constant table (Montgomery), copied to LDS after the selectors
+    uint32_t nconsts;
     const uint32_t* apow;    // alpha^(m-1-j) for assert j (ext4, Montgomery)
+    uint32_t nasserts;
     const uint32_t* tw;    // w_Q^e, e < |Q|
     const uint32_t* zh;    // Z_H at the 2^qdb values of x^n, then their inverses
     uint32_t zh_mask;      // 2^qdb - 1
@@ -245,6 +250,6 @@ struct QuotientArgs {
     int need_selectors;
     uint32_t* out;         // |Q| x 4
 };
-size_t quotient_lds_bytes(uint32_t nregs, uint32_t nslots);
+size_t quotient_lds_bytes(uint32_t nregs, uint32_t nslots, bool selectors, uint32_t nconsts);
 hipError_t launch_quotient(int field, const QuotientArgs& a, hipStream_t stream);
 }  // namespace pg
