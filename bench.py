@@ -124,9 +124,10 @@ def cpu_model():
 
 def cpu_baseline(log_h, w, lb, runs, gpu_root):
     """The whole job on the host: oracle/cpu_baseline.c (the reference's CPU
-    commit path restated in Montgomery form, AVX2, OpenMP over every host
-    thread) commits the same seeded 2^log_h x w trace; one warm-up run, then
-    the median of `runs` timed runs.  Its root is checked against the GPU's."""
+    commit path restated in Montgomery form, AVX-512 where the host has it,
+    else AVX2, OpenMP over every host thread) commits the same seeded
+    2^log_h x w trace; one warm-up run, then the median of `runs` timed runs.
+    Its root is checked against the GPU's."""
     import numpy as np
 
     from oracle import oracle
@@ -143,14 +144,16 @@ def cpu_baseline(log_h, w, lb, runs, gpu_root):
         phases.append(oracle.cpu_last_phase_seconds())
     del mont
     med = float(np.median(times))
+    isa = oracle.CPU_ISAS[oracle.cpu_isa()]
     return {
         "value": (h * w) / med,
         "unit": "cells/s",
         "cores": oracle.cpu_num_threads(),
         "kind": "port",
         "sample": (f"full job: Pcs::commit of the same seeded 2^{log_h} x {w} KoalaBear trace at blowup "
-                   f"{1 << lb} (C restatement of the reference algorithm in Montgomery form, AVX2, "
+                   f"{1 << lb} (C restatement of the reference algorithm in Montgomery form, {isa}, "
                    f"not the reference binary); median of {runs} runs after 1 warm-up"),
+        "isa": isa,
         "runs": runs,
         "median_s": med,
         "min_s": min(times),

@@ -14,8 +14,10 @@
  * Montgomery form (p3 MontyField31), row-parallel radix-2 butterflies
  * vectorised across the columns of a row-major matrix, and Poseidon2 on
  * packed lanes (the reference builds with `nightly-features`, i.e. p3's
- * packed AVX fields, vm/Cargo.toml:117-121).  Here: AVX2, 8 lanes, OpenMP
- * over rows.  Unlike oracle.c (canonical "% p" arithmetic, the checker), it
+ * packed AVX fields, vm/Cargo.toml:117-121).  Here: AVX-512 (16 lanes) where
+ * the host has it, else AVX2 (8 lanes), chosen at run time; OpenMP over
+ * rows.  The vector code is cpu_baseline_simd.inc, included once per
+ * instruction set.  Unlike oracle.c (canonical "% p" arithmetic, the checker), it
  * is written for speed; tests/test_oracle.py pins its roots and LDEs to
  * oracle.c's.
  *
@@ -85,101 +87,54 @@ static void cb_field_init(int f, CbField *k) {
     for (int i = 0; i < 16; i++) k->diag[i] = mont_of(p, f == OR_KOALABEAR ? kb[i] : bb[i]);
 }
 
-/* -------------------------------------------------------- AVX2 field ops -- */
-typedef struct {
-    __m256i p, pinv;
-} V;
+/* ------------------------------------------------------ shared helpers -- */
+#define CB_BLOCK_ROWS 512
 
-INL __m256i vadd(const V *v, __m256i a, __m256i b) {
-    const __m256i t = _mm256_add_epi32(a, b);
-    return _mm256_min_epu32(t, _mm256_sub_epi32(t, v->p));
-}
-INL __m256i vsub(const V *v, __m256i a, __m256i b) {
-    const __m256i t = _mm256_sub_epi32(a, b);
-    return _mm256_min_epu32(t, _mm256_add_epi32(t, v->p));
-}
-/* Montgomery product a b 2^-32 for a, b < p: with q = lo(a b) p^-1 mod 2^32,
- * a b - q p vanishes mod 2^32 and (a b - q p) / 2^32 = hi(a b) - hi(q p) lies
- * in (-p, p); one conditional addition of p ends it. */
-INL __m256i vmul(const V *v, __m256i a, __m256i b) {
-    const __m256i ao = _mm256_srli_epi64(a, 32), bo = _mm256_srli_epi64(b, 32);
-    const __m256i te = _mm256_mul_epu32(a, b), to = _mm256_mul_epu32(ao, bo);
-    const __m256i qe = _mm256_mul_epu32(te, v->pinv), qo = _mm256_mul_epu32(to, v->pinv);
-    const __m256i qpe = _mm256_mul_epu32(qe, v->p), qpo = _mm256_mul_epu32(qo, v->p);
-    const __m256i ht = _mm256_blend_epi32(_mm256_srli_epi64(te, 32), to, 0xAA);
-    const __m256i hq = _mm256_blend_epi32(_mm256_srli_epi64(qpe, 32), qpo, 0xAA);
-    const __m256i d = _mm256_sub_epi32(ht, hq);
-    return _mm256_min_epu32(d, _mm256_add_epi32(d, v->p));
-}
-INL __m256i vset(uint32_t x) { return _mm256_set1_epi32((int)x); }
-/* x / 2 for x < p: (x >> 1) + (x odd) (p + 1) / 2 */
-INL __m256i v_halve(const V *v, __m256i x) {
-    const __m256i one = _mm256_set1_epi32(1);
-    const __m256i odd = _mm256_sub_epi32(_mm256_setzero_si256(), _mm256_and_si256(x, one));
-    const __m256i half_p1 = _mm256_srli_epi32(_mm256_add_epi32(v->p, one), 1);
-    return _mm256_add_epi32(_mm256_srli_epi32(x, 1), _mm256_and_si256(odd, half_p1));
+static size_t rev_bits(size_t x, int bits) {
+    size_t r = 0;
+    for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
 }
 
-/* ------------------------------------------------------ Poseidon2, 8 lanes -- */
-/* Lane l of s[i] is element i of permutation l. */
-INL void v_ext_layer(const V *v, __m256i s[16]) {
-    /* M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] per 4-block, written as
-     * y_i = (a+b+c+d) + x_i + x_{i+1} + x_{i+1} (indices mod 4): row i of M4
-     * is 1 everywhere plus 1 at column i and 2 at column i+1. */
-    for (int b = 0; b < 16; b += 4) {
-        const __m256i x0 = s[b], x1 = s[b + 1], x2 = s[b + 2], x3 = s[b + 3];
-        const __m256i t = vadd(v, vadd(v, x0, x1), vadd(v, x2, x3));
-        s[b] = vadd(v, t, vadd(v, x0, vadd(v, x1, x1)));
-        s[b + 1] = vadd(v, t, vadd(v, x1, vadd(v, x2, x2)));
-        s[b + 2] = vadd(v, t, vadd(v, x2, vadd(v, x3, x3)));
-        s[b + 3] = vadd(v, t, vadd(v, x3, vadd(v, x0, x0)));
+/* twiddles w^e (Montgomery), e < n/2, w = the order-n root (or its inverse) */
+static uint32_t *make_twiddles(uint32_t p, uint32_t w, size_t n) {
+    const size_t half = n / 2 ? n / 2 : 1;
+    uint32_t *t = alloc64(half * 4);
+    if (!t) return NULL;
+    uint64_t cur = 1;
+    for (size_t e = 0; e < half; e++) {
+        t[e] = mont_of(p, (uint32_t)cur);
+        cur = cur * w % p;
     }
-    for (int i = 0; i < 4; i++) {
-        const __m256i col = vadd(v, vadd(v, s[i], s[4 + i]), vadd(v, s[8 + i], s[12 + i]));
-        for (int b = 0; b < 16; b += 4) s[b + i] = vadd(v, s[b + i], col);
-    }
+    return t;
 }
 
-INL __m256i v_sbox(const V *v, int sbox7, __m256i x) {
-    const __m256i x2 = vmul(v, x, x);
-    if (!sbox7) return vmul(v, x2, x);
-    const __m256i x3 = vmul(v, x2, x), x4 = vmul(v, x2, x2);
-    return vmul(v, x4, x3);
+static double g_last_lde_s, g_last_merkle_s;
+
+static double now_s(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
 }
 
-static void v_permute(const V *v, const CbField *k, const __m256i diag[16], __m256i s[16]) {
-    v_ext_layer(v, s);
-    for (int r = 0; r < 4; r++) {
-        for (int i = 0; i < 16; i++) s[i] = v_sbox(v, k->sbox7, vadd(v, s[i], vset(k->rc_full[r][i])));
-        v_ext_layer(v, s);
-    }
-    for (int r = 0; r < k->rounds_p; r++) {
-        s[0] = v_sbox(v, k->sbox7, vadd(v, s[0], vset(k->rc_part[r])));
-        __m256i sum = s[0];
-        for (int i = 1; i < 16; i++) sum = vadd(v, sum, s[i]);
-        /* s_i D_i + sum.  Both fields' first nine diagonal entries are
-         * -2, 1, 2, 1/2, 3, 4, -1/2, -3, -4 (consts.rs:110-154): additions
-         * and a halving (linear maps, so valid on Montgomery values); the
-         * other entries are Montgomery products. */
-        const __m256i x2 = vadd(v, s[2], s[2]);
-        const __m256i x4a = vadd(v, s[4], s[4]), x5a = vadd(v, s[5], s[5]);
-        const __m256i x7a = vadd(v, s[7], s[7]), x8a = vadd(v, s[8], s[8]);
-        s[0] = vsub(v, sum, vadd(v, s[0], s[0]));
-        s[1] = vadd(v, s[1], sum);
-        s[2] = vadd(v, x2, sum);
-        s[3] = vadd(v, v_halve(v, s[3]), sum);
-        s[4] = vadd(v, vadd(v, x4a, s[4]), sum);
-        s[5] = vadd(v, vadd(v, x5a, x5a), sum);
-        s[6] = vsub(v, sum, v_halve(v, s[6]));
-        s[7] = vsub(v, sum, vadd(v, x7a, s[7]));
-        s[8] = vsub(v, sum, vadd(v, x8a, x8a));
-        for (int i = 9; i < 16; i++) s[i] = vadd(v, vmul(v, s[i], diag[i]), sum);
-    }
-    for (int r = 4; r < 8; r++) {
-        for (int i = 0; i < 16; i++) s[i] = v_sbox(v, k->sbox7, vadd(v, s[i], vset(k->rc_full[r][i])));
-        v_ext_layer(v, s);
-    }
-}
+/* ---------------------------------------------------------- AVX2, 8 lanes -- */
+#define VT __m256i
+#define VL 8
+#define SFX _avx2
+#define V_SET1(x) _mm256_set1_epi32((int)(x))
+#define V_ADD _mm256_add_epi32
+#define V_SUB _mm256_sub_epi32
+#define V_MIN _mm256_min_epu32
+#define V_MULE _mm256_mul_epu32
+#define V_AND _mm256_and_si256
+#define V_SRLI64(a, n) _mm256_srli_epi64(a, n)
+#define V_SRLI32(a, n) _mm256_srli_epi32(a, n)
+#define V_BLEND_ODD(a, b) _mm256_blend_epi32(a, b, 0xAA) /* odd lanes from b */
+#define V_ZERO() _mm256_setzero_si256()
+#define V_LOAD(p) _mm256_load_si256((const __m256i *)(p))
+#define V_STORE(p, x) _mm256_store_si256((__m256i *)(p), x)
 
 /* 8 x 8 transpose of 32-bit words: in[r] = row r; out[c] = column c. */
 INL void transpose8(__m256i m[8]) {
@@ -201,8 +156,9 @@ INL void transpose8(__m256i m[8]) {
     m[7] = _mm256_permute2x128_si256(b3, b7, 0x31);
 }
 
-/* Load words [c0, c0 + len) of rows r[0..8) (len <= 8) as 8 lane vectors. */
-INL void load_cols(const uint32_t *const r[8], size_t c0, size_t len, __m256i out[8]) {
+/* Load words [c0, c0 + len) of rows r[0..8) (len <= 8) as 8 lane vectors
+ * (vectors past len are zero). */
+INL void load_cols_avx2(const uint32_t *const r[8], size_t c0, size_t len, __m256i out[8]) {
     if (len == 8) {
         for (int i = 0; i < 8; i++) out[i] = _mm256_loadu_si256((const __m256i *)(r[i] + c0));
     } else {
@@ -214,115 +170,78 @@ INL void load_cols(const uint32_t *const r[8], size_t c0, size_t len, __m256i ou
     transpose8(out);
 }
 
-INL void store_digests(__m256i s[8], uint32_t *const d[8]) {
+/* digest words s[0..8) of lane l to d[l] (lanes past nvalid go to the
+ * caller's spill rows) */
+INL void store_digests_avx2(__m256i s[8], uint32_t *const d[8], int nvalid) {
+    (void)nvalid;
     __m256i m[8];
     for (int i = 0; i < 8; i++) m[i] = s[i];
     transpose8(m);
     for (int i = 0; i < 8; i++) _mm256_storeu_si256((__m256i *)d[i], m[i]);
 }
 
-/* ------------------------------------------------------------------ LDE -- */
-#define CB_BLOCK_ROWS 512
+#include "cpu_baseline_simd.inc"
 
-/* One DIF layer of half-width m on rows [0, n) of a (n x wp) matrix:
- * (x_a, x_b) -> (x_a + x_b, (x_a - x_b) tw[j * step]) for a = blk + j,
- * b = a + m.  tw holds w^e for e < n / 2 (w of order n). */
-INL void dif_pair(const V *v, uint32_t *a, uint32_t *b, size_t wp, __m256i t) {
-    for (size_t c = 0; c < wp; c += 8) {
-        const __m256i x = _mm256_load_si256((const __m256i *)(a + c));
-        const __m256i y = _mm256_load_si256((const __m256i *)(b + c));
-        _mm256_store_si256((__m256i *)(a + c), vadd(v, x, y));
-        _mm256_store_si256((__m256i *)(b + c), vmul(v, vsub(v, x, y), t));
-    }
+#undef VT
+#undef VL
+#undef SFX
+#undef V_SET1
+#undef V_ADD
+#undef V_SUB
+#undef V_MIN
+#undef V_MULE
+#undef V_AND
+#undef V_SRLI64
+#undef V_SRLI32
+#undef V_BLEND_ODD
+#undef V_ZERO
+#undef V_LOAD
+#undef V_STORE
+
+/* ------------------------------------------------------ AVX-512, 16 lanes -- */
+#pragma GCC push_options
+#pragma GCC target("avx512f,avx512bw,avx512dq,avx512vl")
+#define VT __m512i
+#define VL 16
+#define SFX _avx512
+#define V_SET1(x) _mm512_set1_epi32((int)(x))
+#define V_ADD _mm512_add_epi32
+#define V_SUB _mm512_sub_epi32
+#define V_MIN _mm512_min_epu32
+#define V_MULE _mm512_mul_epu32
+#define V_AND _mm512_and_si512
+#define V_SRLI64(a, n) _mm512_srli_epi64(a, n)
+#define V_SRLI32(a, n) _mm512_srli_epi32(a, n)
+#define V_BLEND_ODD(a, b) _mm512_mask_blend_epi32((__mmask16)0xAAAA, a, b) /* odd lanes from b */
+#define V_ZERO() _mm512_setzero_si512()
+#define V_LOAD(p) _mm512_load_si512((const void *)(p))
+#define V_STORE(p, x) _mm512_store_si512((void *)(p), x)
+
+/* Load words [c0, c0 + len) of rows r[0..16) (len <= 8) as lane vectors:
+ * one gather per word, the rows addressed relative to r[0] (the callers'
+ * rows lie within one array, at or after r[0]); vectors past len are zero. */
+INL void load_cols_avx512(const uint32_t *const r[16], size_t c0, size_t len, __m512i out[8]) {
+    int32_t off[16] __attribute__((aligned(64)));
+    for (int l = 0; l < 16; l++) off[l] = (int32_t)(r[l] - r[0]);
+    const __m512i vi = _mm512_load_si512((const void *)off);
+    for (size_t i = 0; i < 8; i++)
+        out[i] = i < len ? _mm512_i32gather_epi32(vi, (const void *)(r[0] + c0 + i), 4) : _mm512_setzero_si512();
 }
 
-/* Two DIF layers (half-widths m and m/2) in one pass over the rows: rows
- * r, r + m/2, r + m, r + 3m/2 (r = blk + j, j < m/2) take the layer-m
- * butterflies (r, r + m) at w^j and (r + m/2, r + 3m/2) at w^(j + m/2), then
- * the layer-m/2 butterflies (r, r + m/2) and (r + m, r + 3m/2) at w'^j. */
-static void dif_quad_pass(const V *v, uint32_t *x, size_t n, size_t m, size_t wp, const uint32_t *tw) {
-    const size_t q = m / 2, s1 = n / (2 * m), s2 = n / m;
-#pragma omp parallel for schedule(static)
-    for (size_t t = 0; t < n / 4; t++) {
-        const size_t j = t % q, r = (t / q) * 2 * m + j;
-        uint32_t *a = x + r * wp, *b = x + (r + q) * wp, *c = x + (r + m) * wp, *d = x + (r + m + q) * wp;
-        const __m256i t1 = vset(tw[j * s1]), t2 = vset(tw[(j + q) * s1]), t3 = vset(tw[j * s2]);
-        for (size_t k = 0; k < wp; k += 8) {
-            const __m256i va = _mm256_load_si256((const __m256i *)(a + k));
-            const __m256i vb = _mm256_load_si256((const __m256i *)(b + k));
-            const __m256i vc = _mm256_load_si256((const __m256i *)(c + k));
-            const __m256i vd = _mm256_load_si256((const __m256i *)(d + k));
-            const __m256i a1 = vadd(v, va, vc), c1 = vmul(v, vsub(v, va, vc), t1);
-            const __m256i b1 = vadd(v, vb, vd), d1 = vmul(v, vsub(v, vb, vd), t2);
-            _mm256_store_si256((__m256i *)(a + k), vadd(v, a1, b1));
-            _mm256_store_si256((__m256i *)(b + k), vmul(v, vsub(v, a1, b1), t3));
-            _mm256_store_si256((__m256i *)(c + k), vadd(v, c1, d1));
-            _mm256_store_si256((__m256i *)(d + k), vmul(v, vsub(v, c1, d1), t3));
-        }
-    }
+/* digest words s[0..8) of lane l to d[l] for the first nvalid lanes (one
+ * masked scatter per word; the valid destinations are consecutive digests) */
+INL void store_digests_avx512(__m512i s[8], uint32_t *const d[16], int nvalid) {
+    int32_t off[16] __attribute__((aligned(64)));
+    for (int l = 0; l < 16; l++) off[l] = l < nvalid ? (int32_t)(d[l] - d[0]) : 0;
+    const __m512i vi = _mm512_load_si512((const void *)off);
+    const __mmask16 mk = (__mmask16)((1u << nvalid) - 1u);
+    for (int i = 0; i < 8; i++) _mm512_mask_i32scatter_epi32((void *)(d[0] + i), mk, vi, s[i], 4);
 }
 
-static void dif_rows(const V *v, uint32_t *x, size_t n, size_t wp, const uint32_t *tw) {
-    size_t m = n / 2;
-    /* whole-matrix passes while a butterfly spans more than a block: two
-     * layers per pass where both are that wide, else one */
-    while (m >= 1 && 2 * m > CB_BLOCK_ROWS) {
-        if (m >= 2 && m > CB_BLOCK_ROWS) {
-            dif_quad_pass(v, x, n, m, wp, tw);
-            m /= 4;
-            continue;
-        }
-        const size_t step = n / (2 * m);
-#pragma omp parallel for schedule(static)
-        for (size_t t = 0; t < n / 2; t++) {
-            const size_t j = t % m, a = (t / m) * 2 * m + j;
-            dif_pair(v, x + a * wp, x + (a + m) * wp, wp, vset(tw[j * step]));
-        }
-        m /= 2;
-    }
-    if (m == 0) return;
-    /* the rest block by block: every remaining layer stays inside a block */
-    const size_t blk = 2 * m;
-#pragma omp parallel for schedule(static)
-    for (size_t b0 = 0; b0 < n; b0 += blk)
-        for (size_t mm = m; mm >= 1; mm /= 2) {
-            const size_t step = n / (2 * mm);
-            for (size_t g = b0; g < b0 + blk; g += 2 * mm)
-                for (size_t j = 0; j < mm; j++)
-                    dif_pair(v, x + (g + j) * wp, x + (g + j + mm) * wp, wp, vset(tw[j * step]));
-        }
-}
-
-static size_t rev_bits(size_t x, int bits) {
-    size_t r = 0;
-    for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
-    return r;
-}
-
-/* twiddles w^e (Montgomery), e < n/2, w = the order-n root (or its inverse) */
-static uint32_t *make_twiddles(uint32_t p, uint32_t w, size_t n) {
-    const size_t half = n / 2 ? n / 2 : 1;
-    uint32_t *t = alloc64(half * 4);
-    if (!t) return NULL;
-    uint64_t cur = 1;
-    for (size_t e = 0; e < half; e++) {
-        t[e] = mont_of(p, (uint32_t)cur);
-        cur = cur * w % p;
-    }
-    return t;
-}
+#include "cpu_baseline_simd.inc"
+#pragma GCC pop_options
 
 /* ---------------------------------------------------------------- entry -- */
-static double g_last_lde_s, g_last_merkle_s;
-
-static double now_s(void) {
-#ifdef _OPENMP
-    return omp_get_wtime();
-#else
-    return 0.0;
-#endif
-}
-
 /* seconds of the last cb_pcs_commit's LDE (input copy through the forward
  * transform) and Merkle tree (leaves and every compress layer) */
 void cb_last_phase_seconds(double *lde_s, double *merkle_s) {
@@ -330,113 +249,29 @@ void cb_last_phase_seconds(double *lde_s, double *merkle_s) {
     *merkle_s = g_last_merkle_s;
 }
 
+/* 2 when this host runs the AVX-512 code, else 1 (AVX2) */
+int cb_isa(void) {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+                   __builtin_cpu_supports("avx512dq") && __builtin_cpu_supports("avx512vl")
+               ? 2
+               : 1;
+}
+
+/* isa 0: the best this host has; 1: AVX2; 2: AVX-512 (-1 if the host lacks it) */
+int cb_pcs_commit_isa(int isa, int f, const uint32_t *in_mont, size_t h, size_t w, int lb, uint32_t root_mont[8],
+                      uint32_t *lde_out) {
+    if (h == 0 || (h & (h - 1)) || w == 0 || lb < 0 || lb > 8) return -1;
+    const int best = cb_isa();
+    if (isa == 0) isa = best;
+    if (isa == 2 && best < 2) return -1;
+    return isa == 2 ? commit_avx512(f, in_mont, h, w, lb, root_mont, lde_out)
+                    : commit_avx2(f, in_mont, h, w, lb, root_mont, lde_out);
+}
+
 int cb_pcs_commit(int f, const uint32_t *in_mont, size_t h, size_t w, int lb, uint32_t root_mont[8],
                   uint32_t *lde_out) {
-    if (h == 0 || (h & (h - 1)) || w == 0 || lb < 0 || lb > 8) return -1;
-    CbField k;
-    cb_field_init(f, &k);
-    const V v = {vset(k.p), vset(k.pinv)};
-    const uint32_t p = k.p;
-    int L = 0;
-    while (((size_t)1 << L) < h) L++;
-    const size_t N = h << lb, wp = (w + 7) & ~(size_t)7;
-    uint32_t *x = alloc64(N * wp * 4);
-    const uint32_t g = or_generator(f);
-    const uint32_t wh = or_two_adic_generator(f, L), wn = or_two_adic_generator(f, L + lb);
-    uint32_t *tw_inv = make_twiddles(p, cb_pow(p, wh, p - 2), h), *tw_fwd = make_twiddles(p, wn, N);
-    uint32_t *digests = alloc64(2 * N * 32);
-    if (!x || !tw_inv || !tw_fwd || !digests) {
-        free(x), free(tw_inv), free(tw_fwd), free(digests);
-        return -1;
-    }
-    const double t_start = now_s();
-#pragma omp parallel for schedule(static)
-    for (size_t r = 0; r < N; r++) {
-        if (r < h) {
-            memcpy(x + r * wp, in_mont + r * w, w * 4);
-            memset(x + r * wp + w, 0, (wp - w) * 4);
-        } else {
-            memset(x + r * wp, 0, wp * 4);
-        }
-    }
-    /* coefficients: inverse DIF leaves c_{rev(k)} at row k */
-    dif_rows(&v, x, h, wp, tw_inv);
-    /* natural order, each c_i times shift^i / h (pairs k < rev(k) swap) */
-    const uint32_t hinv = cb_pow(p, (uint32_t)(h % p), p - 2);
-#pragma omp parallel for schedule(static)
-    for (size_t kk = 0; kk < h; kk++) {
-        const size_t rk = rev_bits(kk, L);
-        if (rk < kk) continue;
-        const __m256i sk = vset(mont_of(p, (uint32_t)((uint64_t)cb_pow(p, g, rk) * hinv % p)));
-        const __m256i sr = vset(mont_of(p, (uint32_t)((uint64_t)cb_pow(p, g, kk) * hinv % p)));
-        uint32_t *a = x + kk * wp, *b = x + rk * wp;
-        for (size_t c = 0; c < wp; c += 8) {
-            const __m256i va = _mm256_load_si256((const __m256i *)(a + c));
-            const __m256i vb = _mm256_load_si256((const __m256i *)(b + c));
-            /* row k holds c_{rk}: it goes to row rk scaled by g^rk / h */
-            _mm256_store_si256((__m256i *)(b + c), vmul(&v, va, sk));
-            if (rk != kk) _mm256_store_si256((__m256i *)(a + c), vmul(&v, vb, sr));
-        }
-    }
-    /* evaluations on g <w_N>, committed (bit-reversed) order */
-    dif_rows(&v, x, N, wp, tw_fwd);
-    if (lde_out) {
-#pragma omp parallel for schedule(static)
-        for (size_t r = 0; r < N; r++) memcpy(lde_out + r * w, x + r * wp, w * 4);
-    }
-    const double t_lde = now_s();
-    /* Merkle: leaves (8 rows per vector group), then compress layers */
-    __m256i diag[16];
-    for (int i = 0; i < 16; i++) diag[i] = vset(k.diag[i]);
-#pragma omp parallel for schedule(static)
-    for (size_t r0 = 0; r0 < N; r0 += 8) {
-        const uint32_t *rows[8];
-        uint32_t *dst[8];
-        uint32_t spill[8][8];
-        for (int i = 0; i < 8; i++) {
-            const size_t r = r0 + i < N ? r0 + i : N - 1;
-            rows[i] = x + r * wp;
-            dst[i] = r0 + i < N ? digests + 8 * (r0 + i) : spill[i];
-        }
-        __m256i s[16];
-        for (int i = 0; i < 16; i++) s[i] = _mm256_setzero_si256();
-        for (size_t c0 = 0; c0 < w; c0 += 8) {
-            const size_t len = w - c0 < 8 ? w - c0 : 8;
-            __m256i m[8];
-            load_cols(rows, c0, len, m);
-            for (size_t i = 0; i < len; i++) s[i] = m[i]; /* overwrite mode */
-            v_permute(&v, &k, diag, s);
-        }
-        store_digests(s, dst);
-    }
-    size_t off = 0, n = N;
-    while (n > 1) {
-        const size_t next = off + n;
-#pragma omp parallel for schedule(static)
-        for (size_t i0 = 0; i0 < n / 2; i0 += 8) {
-            const uint32_t *lft[8], *rgt[8];
-            uint32_t *dst[8];
-            uint32_t spill[8][8];
-            for (int i = 0; i < 8; i++) {
-                const size_t ii = i0 + i < n / 2 ? i0 + i : n / 2 - 1;
-                lft[i] = digests + 8 * (off + 2 * ii);
-                rgt[i] = digests + 8 * (off + 2 * ii + 1);
-                dst[i] = i0 + i < n / 2 ? digests + 8 * (next + i0 + i) : spill[i];
-            }
-            __m256i s[16];
-            load_cols(lft, 0, 8, s);
-            load_cols(rgt, 0, 8, s + 8);
-            v_permute(&v, &k, diag, s);
-            store_digests(s, dst);
-        }
-        off = next;
-        n /= 2;
-    }
-    memcpy(root_mont, digests + 8 * off, 32);
-    g_last_lde_s = t_lde - t_start;
-    g_last_merkle_s = now_s() - t_lde;
-    free(x), free(tw_inv), free(tw_fwd), free(digests);
-    return 0;
+    return cb_pcs_commit_isa(0, f, in_mont, h, w, lb, root_mont, lde_out);
 }
 
 /* canonical -> Montgomery, in parallel (input preparation, outside timing) */

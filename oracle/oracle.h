@@ -158,12 +158,17 @@ int or_num_threads(void);
  * as the reference's CPU path computes it: Radix2DitParallel coset LDE at
  * blowup 2^log_blowup with shift GENERATOR, rows bit-reversed, MerkleTreeMmcs
  * root.  Unlike the functions above, it takes and returns MONTGOMERY words
- * (p3's in-memory form) and is written for speed (AVX2, OpenMP).  lde_out
+ * (p3's in-memory form) and is written for speed (AVX-512 or AVX2, OpenMP).  lde_out
  * (N x w, committed order) may be NULL.  Returns 0, or -1 on a bad shape or
  * a failed allocation.  Pinned to or_coset_lde + or_mmcs_commit by
  * tests/test_oracle.py. */
 int cb_pcs_commit(int field, const uint32_t *in_mont, size_t h, size_t w, int log_blowup,
                   uint32_t root_mont[8], uint32_t *lde_out);
+/* the same with the instruction set chosen: 0 the best the host has, 1 AVX2,
+ * 2 AVX-512 (-1 if the host lacks it); cb_isa(): what 0 picks (1 or 2) */
+int cb_pcs_commit_isa(int isa, int field, const uint32_t *in_mont, size_t h, size_t w, int log_blowup,
+                      uint32_t root_mont[8], uint32_t *lde_out);
+int cb_isa(void);
 void cb_to_monty(int field, const uint32_t *in, uint32_t *out, size_t n);
 int cb_num_threads(void);
 /* LDE and Merkle seconds of the last cb_pcs_commit */

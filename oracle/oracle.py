@@ -59,6 +59,8 @@ def lib():
             "or_pcs_reduce": (None, [i, sz, P, P, P, P, P, P, P, P]),
             "or_pcs_commit_seeded": (i, [i, sz, P, P, P, P, i, sz, P]),
             "cb_pcs_commit": (i, [i, P, sz, sz, i, P, P]),
+            "cb_pcs_commit_isa": (i, [i, i, P, sz, sz, i, P, P]),
+            "cb_isa": (i, []),
             "cb_to_monty": (None, [i, P, P, sz]),
             "cb_num_threads": (i, []),
             "cb_last_phase_seconds": (None, [P, P]),
@@ -342,8 +344,14 @@ def num_threads():
 
 # --------------------------------------------------- CPU baseline (timing) --
 # cpu_baseline.c: the reference's CPU commit path (Radix2DitParallel LDE +
-# MerkleTreeMmcs, Montgomery form, AVX2, OpenMP), timed by bench.py and
-# pinned to the functions above by tests/test_oracle.py.
+# MerkleTreeMmcs, Montgomery form, AVX-512 or AVX2, OpenMP), timed by
+# bench.py and pinned to the functions above by tests/test_oracle.py.
+CPU_ISAS = {1: "avx2", 2: "avx512"}
+
+
+def cpu_isa():
+    """The instruction set the timed commit uses on this host: 2 (AVX-512) or 1 (AVX2)."""
+    return lib().cb_isa()
 def cpu_to_monty(field, mat):
     """Canonical matrix -> Montgomery words (the input form the timed commit takes)."""
     a = _u32(mat)
@@ -352,23 +360,24 @@ def cpu_to_monty(field, mat):
     return out
 
 
-def cpu_pcs_commit_mont(field, mont, log_blowup, want_lde=False):
+def cpu_pcs_commit_mont(field, mont, log_blowup, want_lde=False, isa=0):
     """TwoAdicFriPcs::commit of one trace held in Montgomery words:
-    (root, LDE or None), both Montgomery, the LDE in committed row order."""
+    (root, LDE or None), both Montgomery, the LDE in committed row order.
+    isa 0: the best this host has; 1 AVX2; 2 AVX-512."""
     mont = _u32(mont)
     h, w = mont.shape
     root = np.zeros(8, np.uint32)
     lde = np.empty((h << log_blowup, w), np.uint32) if want_lde else None
-    rc = lib().cb_pcs_commit(field, _ptr(mont), h, w, log_blowup, _ptr(root),
-                             _ptr(lde) if want_lde else None)
+    rc = lib().cb_pcs_commit_isa(isa, field, _ptr(mont), h, w, log_blowup, _ptr(root),
+                                 _ptr(lde) if want_lde else None)
     if rc != 0:
         raise MemoryError("cb_pcs_commit failed (shape or allocation)")
     return root, lde
 
 
-def cpu_pcs_commit(field, mat, log_blowup, want_lde=False):
+def cpu_pcs_commit(field, mat, log_blowup, want_lde=False, isa=0):
     """The same from canonical values; canonical outputs."""
-    root, lde = cpu_pcs_commit_mont(field, cpu_to_monty(field, mat), log_blowup, want_lde)
+    root, lde = cpu_pcs_commit_mont(field, cpu_to_monty(field, mat), log_blowup, want_lde, isa)
     return from_monty(field, root), (from_monty(field, lde) if want_lde else None)
 
 

@@ -461,17 +461,21 @@ def test_oracle_reduce_matches_verifier_loop(oracle, f):
             assert lhs == rhs, (lh, i)
 
 
+@pytest.mark.parametrize("isa", [1, 2], ids=["avx2", "avx512"])
 @pytest.mark.parametrize("f", [0, 1])
-def test_cpu_baseline_matches_oracle(oracle, f):
+def test_cpu_baseline_matches_oracle(oracle, f, isa):
     """The CPU baseline bench.py times (oracle/cpu_baseline.c: Montgomery form,
-    AVX2, radix-4 passes, blocked layers) commits exactly what the checker
-    commits: the whole LDE and the root, over heights 1..2^13 (whole-matrix,
-    paired and blocked DIF layers), widths that leave a partial sponge chunk
-    and partial 8-row groups, blowups 1..8."""
+    AVX2 and AVX-512, radix-4 passes, blocked layers) commits exactly what the
+    checker commits: the whole LDE and the root, over heights 1..2^13
+    (whole-matrix, paired and blocked DIF layers), widths that leave a
+    partial sponge chunk and partial row groups of either lane count,
+    blowups 1..8."""
+    if isa == 2 and oracle.cpu_isa() < 2:
+        pytest.skip("this host has no AVX-512")
     for (L, w, lb) in [(0, 1, 0), (0, 3, 1), (1, 5, 1), (2, 9, 3), (3, 8, 2), (9, 13, 1), (10, 16, 3),
-                       (11, 24, 1), (12, 7, 2), (13, 9, 1)]:
+                       (11, 24, 1), (12, 7, 2), (13, 9, 1), (4, 17, 0), (5, 33, 2)]:
         a = oracle.random_matrix(f, 100 + L, 1 << L, w)
-        root, lde = oracle.cpu_pcs_commit(f, a, lb, want_lde=True)
+        root, lde = oracle.cpu_pcs_commit(f, a, lb, want_lde=True, isa=isa)
         ref = oracle.coset_lde(f, a, lb, oracle.generator(f))
         assert np.array_equal(lde, ref), (L, w, lb)
         assert np.array_equal(root, oracle.MerkleTree(f, [ref]).root), (L, w, lb)
