@@ -339,9 +339,16 @@ void pg_open_proof_free(pg_open_proof *p);
  * (no readback), and writes the |Q| x 4 quotient values (ext4, natural
  * order on Q: exactly compute_quotient_values' output, whose
  * flatten_to_base is the same memory; split it with pg_buf_view for the
- * quotient commit).  This slice covers base-field constraints over main and
- * preprocessed columns, public values, constants and the three selectors
- * (the permutation trace's extension-field constraints are not covered). */
+ * quotient commit).  Constraints may be base-field (assert_zero) or
+ * extension-field (assert_zero_ext): the permutation trace's LogUp
+ * constraints (vm/src/machine/permutation.rs:183-346) read its ext4 columns
+ * (kind PG_AIR_PERMUTATION: column c is base columns 4c..4c+3 of the
+ * committed, flattened permutation trace, vm/src/machine/prover.rs:404-415),
+ * the permutation challenges, the regional cumulative sum and the global
+ * cumulative sum's 14 coordinates (ProverConstraintFolder, folder.rs:
+ * 178-300).  A node is extension-valued when it is one of those leaves or
+ * has an extension-valued operand; an asserted extension value folds like
+ * assert_zero_ext (acc = acc * alpha + c, c in ext4). */
 #define PG_AIR_MAIN 0       /* a = row offset (0 local, 1 next), b = column */
 #define PG_AIR_PREPROCESSED 1 /* a = row offset, b = column */
 #define PG_AIR_PUBLIC 2     /* b = public value index */
@@ -353,6 +360,10 @@ void pg_open_proof_free(pg_open_proof *p);
 #define PG_AIR_SUB 8
 #define PG_AIR_NEG 9        /* a = operand node index */
 #define PG_AIR_MUL 10
+#define PG_AIR_PERMUTATION 11   /* ext4: a = row offset, b = ext column */
+#define PG_AIR_PERM_CHALLENGE 12 /* ext4: b = challenge index */
+#define PG_AIR_REGIONAL_SUM 13  /* ext4: the regional cumulative sum */
+#define PG_AIR_GLOBAL_SUM 14    /* b = coordinate 0..13 (x_0..x_6, y_0..y_6) */
 typedef struct {
     uint32_t kind, a, b;
 } pg_air_node;
@@ -373,6 +384,19 @@ int pg_quotient_values(pg_ctx *ctx, const pg_air_program *prog, const pg_tree *m
                        const pg_tree *pre, size_t pre_k, uint32_t log_blowup, uint32_t log_quotient_degree,
                        const uint32_t *publics_mont, size_t npublics, const uint32_t alpha_mont[4],
                        pg_buf **out);
+/* The same with the permutation side of compute_quotient_values
+ * (vm/src/machine/utils.rs:201-356): perm is matrix perm_k of the committed
+ * permutation trace (flattened to base columns, the same height; may be null
+ * when the program reads none), challenges_mont the permutation challenges
+ * (nchallenges x 4 words), regional_sum_mont the regional cumulative sum (4
+ * words, may be null when unread), global_sum_mont the global cumulative
+ * sum's coordinates (nglobal words). */
+int pg_quotient_values_perm(pg_ctx *ctx, const pg_air_program *prog, const pg_tree *main, size_t main_k,
+                            const pg_tree *pre, size_t pre_k, const pg_tree *perm, size_t perm_k,
+                            uint32_t log_blowup, uint32_t log_quotient_degree, const uint32_t *publics_mont,
+                            size_t npublics, const uint32_t *challenges_mont, size_t nchallenges,
+                            const uint32_t regional_sum_mont[4], const uint32_t *global_sum_mont, size_t nglobal,
+                            const uint32_t alpha_mont[4], pg_buf **out);
 
 /* ---- batched commits ------------------------------------------------------
  * Many independent single-matrix Pcs::commit jobs (the reference proves the

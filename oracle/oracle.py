@@ -396,11 +396,16 @@ def cpu_last_phase_seconds():
 # compute_quotient_values (vm/src/machine/utils.rs:201-356) restated over the
 # expression DAG directly, vectorised over the rows of the quotient coset
 # Q = GEN <w_Q> in natural order; the fold acc = acc * alpha + c follows
-# vm/src/machine/folder.rs:222-226 and the selectors
-# vm/src/compiler/recursion/circuit/domain.rs:46-60.  Node kinds as in the
-# C ABI (pg_air_node): 0 main, 1 preprocessed (a = row offset, b = column),
-# 2 public (b), 3 is_first_row, 4 is_last_row, 5 is_transition, 6 constant
-# (a), 7 add, 8 sub, 9 neg (a), 10 mul (a, b = earlier nodes).
+# vm/src/machine/folder.rs:222-226 (assert_zero and assert_zero_ext alike)
+# and the selectors vm/src/compiler/recursion/circuit/domain.rs:46-60.  Node
+# kinds as in the C ABI (pg_air_node): 0 main, 1 preprocessed (a = row
+# offset, b = column), 2 public (b), 3 is_first_row, 4 is_last_row,
+# 5 is_transition, 6 constant (a), 7 add, 8 sub, 9 neg (a), 10 mul (a, b =
+# earlier nodes), 11 permutation (a = row offset, b = ext4 column: base
+# columns 4b..4b+3), 12 permutation challenge (b), 13 regional cumulative
+# sum, 14 global cumulative sum coordinate (b).  Every value is held as an
+# element of F[X]/(X^4 - W) (base values with zero higher coordinates), so
+# sums, products and the fold are extension arithmetic throughout.
 _W = {KOALABEAR: 3, BABYBEAR: 11}
 
 
@@ -420,10 +425,13 @@ def _ext_mul_rows(f, a, b):
     return out
 
 
-def quotient_values(field, nodes, asserts, main_q, log_n, log_qd, alpha, publics=(), pre_q=None):
-    """nodes (k x 3), asserts: the DAG; main_q / pre_q: the traces'
-    evaluations on Q, natural order (|Q| x width, canonical).  Returns the
-    |Q| x 4 quotient values (canonical)."""
+def quotient_values(field, nodes, asserts, main_q, log_n, log_qd, alpha, publics=(), pre_q=None, perm_q=None,
+                    challenges=(), regional_sum=None, global_sum=()):
+    """nodes (k x 3), asserts: the DAG; main_q / pre_q / perm_q: the traces'
+    evaluations on Q, natural order (|Q| x width, canonical; perm_q has the
+    permutation trace's base columns, 4 per ext4 column); challenges: ext4
+    each; regional_sum: ext4; global_sum: base values.  Returns the |Q| x 4
+    quotient values (canonical)."""
     p = PRIMES[field]
     P = np.uint64(p)
     q = main_q.shape[0]
@@ -446,22 +454,34 @@ def quotient_values(field, nodes, asserts, main_q, log_n, log_qd, alpha, publics
     nxt = (np.arange(q) + step) % q
     mains = [main_q.astype(np.uint64), main_q[nxt].astype(np.uint64)]
     pres = [pre_q.astype(np.uint64), pre_q[nxt].astype(np.uint64)] if pre_q is not None else None
+    perms = [perm_q.astype(np.uint64), perm_q[nxt].astype(np.uint64)] if perm_q is not None else None
+
+    def base(v):  # (q,) base values -> (q, 4)
+        out = np.zeros((q, 4), np.uint64)
+        out[:, 0] = v
+        return out
+
+    def const(c):  # one ext4 (or base) value for every row
+        out = np.zeros((q, 4), np.uint64)
+        out[:, :] = np.asarray([int(x) % p for x in (c if np.ndim(c) else [c, 0, 0, 0])], np.uint64)
+        return out
+
     vals = []
     for kind, a, b in np.asarray(nodes, np.uint64).reshape(-1, 3).tolist():
         if kind == 0:
-            v = mains[a][:, b]
+            v = base(mains[a][:, b])
         elif kind == 1:
-            v = pres[a][:, b]
+            v = base(pres[a][:, b])
         elif kind == 2:
-            v = np.full(q, publics[b] % p, np.uint64)
+            v = const(publics[b])
         elif kind == 3:
-            v = first
+            v = base(first)
         elif kind == 4:
-            v = last
+            v = base(last)
         elif kind == 5:
-            v = trans
+            v = base(trans)
         elif kind == 6:
-            v = np.full(q, a % p, np.uint64)
+            v = const(a)
         elif kind == 7:
             v = (vals[a] + vals[b]) % P
         elif kind == 8:
@@ -469,22 +489,32 @@ def quotient_values(field, nodes, asserts, main_q, log_n, log_qd, alpha, publics
         elif kind == 9:
             v = (P - vals[a]) % P
         elif kind == 10:
-            v = vals[a] * vals[b] % P
+            v = _ext_mul_rows(field, vals[a], vals[b])
+        elif kind == 11:
+            v = perms[a][:, 4 * b:4 * b + 4] % P
+        elif kind == 12:
+            v = const(challenges[b])
+        elif kind == 13:
+            v = const(regional_sum)
+        elif kind == 14:
+            v = const(global_sum[b])
         else:
             raise ValueError(kind)
         vals.append(v)
     acc = np.zeros((q, 4), np.uint64)
     for k in asserts:
-        acc = _ext_mul_rows(field, acc, alpha)
-        acc[:, 0] = (acc[:, 0] + vals[k]) % P
+        acc = (_ext_mul_rows(field, acc, alpha) + vals[k]) % P
     return (acc * inv(zh)[:, None] % P).astype(np.uint32)
 
 
 def constraints_at_point(field, nodes, asserts, main_local, main_next, zeta, log_n, alpha, publics=(),
-                         pre_local=None, pre_next=None):
+                         pre_local=None, pre_next=None, perm_local=None, perm_next=None, challenges=(),
+                         regional_sum=None, global_sum=()):
     """The verifier's side at one ext4 point (vm/src/machine/verifier.rs:
     307-356): the folded constraints and 1 / Z_H(zeta), from the trace
-    polynomials' values at zeta and zeta w_n (ext4 per column)."""
+    polynomials' values at zeta and zeta w_n (ext4 per column; for the
+    permutation trace, per base column of its flattened form, an ext4 column
+    being sum_e X^e v_{4c+e})."""
     p = PRIMES[field]
 
     def mul(a, b):
@@ -497,7 +527,15 @@ def constraints_at_point(field, nodes, asserts, main_local, main_next, zeta, log
         return [(x - y) % p for x, y in zip(a, b)]
 
     def base(v):
-        return [v % p, 0, 0, 0]
+        return [int(v) % p, 0, 0, 0]
+
+    def ext_col(vals4):  # sum_e X^e v_e
+        acc = [0, 0, 0, 0]
+        for e, v in enumerate(vals4):
+            mono = [0, 0, 0, 0]
+            mono[e] = 1
+            acc = add(acc, mul(mono, [int(x) for x in v]))
+        return acc
 
     z = [int(x) for x in zeta]
     zn = z
@@ -510,6 +548,7 @@ def constraints_at_point(field, nodes, asserts, main_local, main_next, zeta, log
     last = mul(zh, [int(x) for x in ext_inv(field, trans)])
     mains = [main_local, main_next]
     pres = [pre_local, pre_next]
+    perms = [perm_local, perm_next]
     vals = []
     for kind, a, b in np.asarray(nodes, np.uint64).reshape(-1, 3).tolist():
         if kind == 0:
@@ -534,6 +573,14 @@ def constraints_at_point(field, nodes, asserts, main_local, main_next, zeta, log
             v = sub(base(0), vals[a])
         elif kind == 10:
             v = mul(vals[a], vals[b])
+        elif kind == 11:
+            v = ext_col(perms[a][4 * b:4 * b + 4])
+        elif kind == 12:
+            v = [int(x) % p for x in challenges[b]]
+        elif kind == 13:
+            v = [int(x) % p for x in regional_sum]
+        elif kind == 14:
+            v = base(global_sum[b])
         else:
             raise ValueError(kind)
         vals.append(v)

@@ -102,6 +102,8 @@ SIGNATURES = {
     "pg_air_compile": (_i, [_i, _P, _sz, _P, _sz, _pp, _P, _P]),
     "pg_air_program_free": (None, [_P]),
     "pg_quotient_values": (_i, [_P, _P, _P, _sz, _P, _sz, _u32, _u32, _P, _sz, _P, _pp]),
+    "pg_quotient_values_perm": (_i, [_P, _P, _P, _sz, _P, _sz, _P, _sz, _u32, _u32, _P, _sz, _P, _sz, _P, _P, _sz,
+                                     _P, _pp]),
     "pg_batch_create": (_i, [_P, _i, _P, _P, _P, _sz, _i, _pp]),
     "pg_batch_run": (_i, [_P, _P, _P]),
     "pg_batch_take_tree": (_i, [_P, _sz, _pp]),

@@ -5,7 +5,12 @@ An AirBuilder collects constraints the way Pico's SymbolicConstraintFolder
 does (vm/src/machine/folder.rs:30-120): expressions over main and
 preprocessed columns at the local and next row, public values, constants and
 the three selectors, each assert_zero appending one constraint (the fold
-order).  Expressions are hash-consed into the node DAG the C ABI takes (one
+order).  The permutation side (PermutationBuilder, folder.rs:246-276) adds
+extension-valued leaves: the permutation trace's ext4 columns, the
+permutation challenges and the regional cumulative sum, plus the global
+cumulative sum's coordinates; an expression with an extension-valued
+operand is extension-valued and its assert is assert_zero_ext.
+Expressions are hash-consed into the node DAG the C ABI takes (one
 pg_air_node per distinct expression, operands first).  assert_bool and the
 when_* filters follow p3's AirBuilder: assert_bool(x) asserts x (x - 1), and
 a filtered assert multiplies the constraint by its condition.
@@ -17,7 +22,8 @@ import numpy as np
 from ._lib import check, lib
 from .core import PRIMES, DeviceMatrix, _ptr, from_monty, to_monty
 
-MAIN, PREPROCESSED, PUBLIC, IS_FIRST_ROW, IS_LAST_ROW, IS_TRANSITION, CONSTANT, ADD, SUB, NEG, MUL = range(11)
+(MAIN, PREPROCESSED, PUBLIC, IS_FIRST_ROW, IS_LAST_ROW, IS_TRANSITION, CONSTANT, ADD, SUB, NEG, MUL, PERMUTATION,
+ PERM_CHALLENGE, REGIONAL_SUM, GLOBAL_SUM) = range(15)
 
 
 class Expr:
@@ -98,6 +104,20 @@ class AirBuilder:
     def constant(self, value):
         return self._node(CONSTANT, int(value) % PRIMES[self.field], 0)
 
+    def permutation(self, col, next=False):
+        """ext4 column `col` of the permutation trace (base columns 4 col..4 col + 3)"""
+        return self._node(PERMUTATION, int(bool(next)), col)
+
+    def perm_challenge(self, index):
+        return self._node(PERM_CHALLENGE, 0, index)
+
+    def regional_sum(self):
+        return self._node(REGIONAL_SUM, 0, 0)
+
+    def global_sum(self, coordinate):
+        """coordinate 0..6 of the global cumulative sum's x, 7..13 of its y"""
+        return self._node(GLOBAL_SUM, 0, coordinate)
+
     @property
     def is_first_row(self):
         return self._node(IS_FIRST_ROW, 0, 0)
@@ -167,19 +187,30 @@ class AirProgram:
 
 
 def quotient_values(ctx, program, main_data, main_idx, log_blowup, log_quotient_degree, alpha, publics=(),
-                    pre_data=None, pre_idx=0):
+                    pre_data=None, pre_idx=0, perm_data=None, perm_idx=0, challenges=(), regional_sum=None,
+                    global_sum=()):
     """compute_quotient_values on the device: the |Q| x 4 quotient (ext4,
     natural order on the quotient coset) as a new DeviceMatrix, from the
-    committed main (and preprocessed) LDEs.  alpha: canonical ext4;
-    publics: canonical values."""
+    committed main (and preprocessed, permutation) LDEs.  alpha, each
+    challenge and regional_sum: canonical ext4; publics, global_sum:
+    canonical values."""
     f = program.field
-    pub = np.ascontiguousarray(to_monty(f, np.asarray(publics, np.uint64))) if len(publics) else None
-    al = np.ascontiguousarray(to_monty(f, np.asarray(alpha, np.uint64)))
+
+    def mont(vals, shape=None):
+        if vals is None or len(vals) == 0:
+            return None
+        return np.ascontiguousarray(to_monty(f, np.asarray(vals, np.uint64)).reshape(-1))
+
+    pub, al = mont(publics), mont(alpha)
+    chal = mont(np.asarray(challenges, np.uint64).reshape(-1)) if len(challenges) else None
+    rsum, gsum = mont(regional_sum), mont(global_sum)
     h = ctypes.c_void_p()
-    check(lib().pg_quotient_values(ctx.handle, program.handle, main_data.handle, main_idx,
-                                   pre_data.handle if pre_data is not None else None, pre_idx, log_blowup,
-                                   log_quotient_degree, _ptr(pub) if pub is not None else None, len(publics),
-                                   _ptr(al), ctypes.byref(h)))
+    check(lib().pg_quotient_values_perm(
+        ctx.handle, program.handle, main_data.handle, main_idx, pre_data.handle if pre_data is not None else None,
+        pre_idx, perm_data.handle if perm_data is not None else None, perm_idx, log_blowup, log_quotient_degree,
+        _ptr(pub) if pub is not None else None, len(publics), _ptr(chal) if chal is not None else None,
+        len(challenges), _ptr(rsum) if rsum is not None else None, _ptr(gsum) if gsum is not None else None,
+        len(global_sum), _ptr(al), ctypes.byref(h)))
     return DeviceMatrix(ctx, h, owned=True)
 
 

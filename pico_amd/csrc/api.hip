@@ -3596,11 +3596,27 @@ extern "C" int pg_tune_import(const char* text) {
 struct pg_air_program {
     int field = 0;
     std::vector<QInstr> code;
-    std::vector<uint32_t> consts;  // Montgomery; the public values follow them at launch
+    // Montgomery; at launch the public values, challenge words, regional sum
+    // and global-sum coordinates follow them
+    std::vector<uint32_t> consts;
     uint32_t nregs = 0, nslots = 0, nasserts = 0, nstages = 0;
-    bool need_selectors = false, reads_pre = false;
-    uint32_t max_main_col = 0, max_pre_col = 0, max_public = 0;  // one past the largest
+    // per base assert: (component e, original assert j), folded with X^e alpha^(m-1-j)
+    std::vector<std::pair<uint32_t, uint32_t>> afactor;
+    uint32_t norig = 0;  // m, the asserts as given
+    bool need_selectors = false, reads_pre = false, reads_perm = false, reads_rsum = false;
+    // one past the largest index read
+    uint32_t max_main_col = 0, max_pre_col = 0, max_perm_col = 0, max_public = 0, max_chal_word = 0, max_gsum = 0;
 };
+
+// Leaf kinds of the lowered (base-field) DAG beyond the ABI's: a base column
+// of the permutation trace, a word of the challenges, of the regional sum,
+// a global-sum coordinate.
+enum : uint32_t { LK_PERM = 32, LK_CHAL, LK_RSUM, LK_GSUM };
+struct LAssert {
+    uint32_t node, comp, orig;
+};
+static int air_compile_lowered(int field, const std::vector<pg_air_node>& lnodes, const std::vector<LAssert>& las,
+                               uint32_t norig, pg_air_program** out, uint32_t* nregs_out, uint32_t* ninstr_out);
 
 extern "C" void pg_air_program_free(pg_air_program* prog) { delete prog; }
 
@@ -3609,7 +3625,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
     if (!out || !field_ok(field) || (nnodes && !nodes) || (nasserts && !asserts))
         return fail(PG_E_INVALID, "bad argument");
     *out = nullptr;
-    if (nnodes > (1u << 28) || nasserts > (1u << 28)) return fail(PG_E_INVALID, "too many nodes");
+    if (nnodes > (1u << 26) || nasserts > (1u << 26)) return fail(PG_E_INVALID, "too many nodes");
     for (size_t k = 0; k < nnodes; k++) {
         const pg_air_node& nd = nodes[k];
         switch (nd.kind) {
@@ -3617,6 +3633,10 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             case PG_AIR_PREPROCESSED:
                 if (nd.a > 1) return fail(PG_E_INVALID, "node %zu: row offset %u (0 local, 1 next)", k, nd.a);
                 if (nd.b >= (1u << 24)) return fail(PG_E_INVALID, "node %zu: column %u", k, nd.b);
+                break;
+            case PG_AIR_PERMUTATION:
+                if (nd.a > 1) return fail(PG_E_INVALID, "node %zu: row offset %u (0 local, 1 next)", k, nd.a);
+                if (nd.b >= (1u << 22)) return fail(PG_E_INVALID, "node %zu: permutation column %u", k, nd.b);
                 break;
             case PG_AIR_ADD:
             case PG_AIR_SUB:
@@ -3629,10 +3649,17 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             case PG_AIR_PUBLIC:
                 if (nd.b >= (1u << 24)) return fail(PG_E_INVALID, "node %zu: public value %u", k, nd.b);
                 break;
+            case PG_AIR_PERM_CHALLENGE:
+                if (nd.b >= (1u << 20)) return fail(PG_E_INVALID, "node %zu: challenge %u", k, nd.b);
+                break;
+            case PG_AIR_GLOBAL_SUM:
+                if (nd.b >= 14) return fail(PG_E_INVALID, "node %zu: global-sum coordinate %u (0..13)", k, nd.b);
+                break;
             case PG_AIR_IS_FIRST_ROW:
             case PG_AIR_IS_LAST_ROW:
             case PG_AIR_IS_TRANSITION:
             case PG_AIR_CONSTANT:
+            case PG_AIR_REGIONAL_SUM:
                 break;
             default:
                 return fail(PG_E_INVALID, "node %zu: unknown kind %u", k, nd.kind);
@@ -3640,11 +3667,113 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
     }
     for (size_t j = 0; j < nasserts; j++)
         if (asserts[j] >= nnodes) return fail(PG_E_INVALID, "assert %zu: node %u out of range", j, asserts[j]);
+    // Lowering to the base field.  A node is extension-valued if it is an
+    // ext4 leaf or has an extension-valued operand; such a node becomes its
+    // four coordinates (base nodes; -1 is a zero coordinate), a base node one
+    // node.  In F[X]/(X^4 - W): sums and negations coordinate-wise, a base
+    // factor scales every coordinate, and a product of two extension values
+    // is c_k = sum_{i+j=k} a_i b_j + W sum_{i+j=k+4} a_i b_j.
+    const int64_t NONE = -1;
+    std::vector<pg_air_node> ln;
+    std::vector<std::array<int64_t, 4>> lo(nnodes, {NONE, NONE, NONE, NONE});
+    std::vector<uint8_t> is_ext(nnodes, 0);
+    std::map<std::array<uint32_t, 3>, uint32_t> memo;  // hash-consing of the lowered nodes
+    auto node = [&](uint32_t kind, uint32_t a, uint32_t b) -> int64_t {
+        const std::array<uint32_t, 3> key{kind, a, b};
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+        ln.push_back(pg_air_node{kind, a, b});
+        memo[key] = (uint32_t)(ln.size() - 1);
+        return (int64_t)(ln.size() - 1);
+    };
+    auto add = [&](int64_t x, int64_t y) -> int64_t {
+        return x == NONE ? y : y == NONE ? x : node(PG_AIR_ADD, (uint32_t)x, (uint32_t)y);
+    };
+    auto neg = [&](int64_t x) -> int64_t { return x == NONE ? NONE : node(PG_AIR_NEG, (uint32_t)x, 0); };
+    auto sub = [&](int64_t x, int64_t y) -> int64_t {
+        return y == NONE ? x : x == NONE ? neg(y) : node(PG_AIR_SUB, (uint32_t)x, (uint32_t)y);
+    };
+    auto mul = [&](int64_t x, int64_t y) -> int64_t {
+        return x == NONE || y == NONE ? NONE : node(PG_AIR_MUL, (uint32_t)x, (uint32_t)y);
+    };
+    const uint32_t W = field == PG_KOALABEAR ? 3u : 11u;  // X^4 = W (vm/src/primitives/consts.rs:107,134)
+    for (size_t k = 0; k < nnodes; k++) {
+        const pg_air_node& nd = nodes[k];
+        auto& o = lo[k];
+        switch (nd.kind) {
+            case PG_AIR_PERMUTATION:
+                is_ext[k] = 1;
+                for (uint32_t e = 0; e < 4; e++) o[e] = node(LK_PERM, nd.a, 4 * nd.b + e);
+                break;
+            case PG_AIR_PERM_CHALLENGE:
+                is_ext[k] = 1;
+                for (uint32_t e = 0; e < 4; e++) o[e] = node(LK_CHAL, 0, 4 * nd.b + e);
+                break;
+            case PG_AIR_REGIONAL_SUM:
+                is_ext[k] = 1;
+                for (uint32_t e = 0; e < 4; e++) o[e] = node(LK_RSUM, 0, e);
+                break;
+            case PG_AIR_GLOBAL_SUM: o[0] = node(LK_GSUM, 0, nd.b); break;
+            case PG_AIR_ADD:
+            case PG_AIR_SUB: {
+                is_ext[k] = is_ext[nd.a] | is_ext[nd.b];
+                for (int e = 0; e < 4; e++)
+                    o[e] = nd.kind == PG_AIR_ADD ? add(lo[nd.a][e], lo[nd.b][e]) : sub(lo[nd.a][e], lo[nd.b][e]);
+                break;
+            }
+            case PG_AIR_NEG:
+                is_ext[k] = is_ext[nd.a];
+                for (int e = 0; e < 4; e++) o[e] = neg(lo[nd.a][e]);
+                break;
+            case PG_AIR_MUL: {
+                is_ext[k] = is_ext[nd.a] | is_ext[nd.b];
+                const auto& x = lo[nd.a];
+                const auto& y = lo[nd.b];
+                if (!is_ext[nd.a] || !is_ext[nd.b]) {  // a base factor scales every coordinate
+                    const int64_t f = !is_ext[nd.a] ? x[0] : y[0];
+                    const auto& v = !is_ext[nd.a] ? y : x;
+                    for (int e = 0; e < 4; e++) o[e] = mul(f, v[e]);
+                    break;
+                }
+                int64_t wnode = NONE;
+                for (int c = 0; c < 4; c++) {
+                    int64_t lowp = NONE, highp = NONE;
+                    for (int i = 0; i < 4; i++) {
+                        const int j = c - i;
+                        if (j >= 0) lowp = add(lowp, mul(x[i], y[j]));
+                        if (c + 4 - i < 4) highp = add(highp, mul(x[i], y[c + 4 - i]));
+                    }
+                    if (highp != NONE && wnode == NONE) wnode = node(PG_AIR_CONSTANT, W, 0);
+                    o[c] = add(lowp, mul(highp, wnode));
+                }
+                break;
+            }
+            default: o[0] = node(nd.kind, nd.a, nd.b); break;  // base leaves as they are
+        }
+    }
+    // Asserts: a base value folds as itself, an extension value as its
+    // coordinates with the factors X^e (zero coordinates drop out).
+    std::vector<LAssert> las;
+    for (size_t j = 0; j < nasserts; j++) {
+        const uint32_t k = asserts[j];
+        for (uint32_t e = 0; e < (is_ext[k] ? 4u : 1u); e++)
+            if (lo[k][e] != NONE) las.push_back(LAssert{(uint32_t)lo[k][e], e, (uint32_t)j});
+    }
+    return air_compile_lowered(field, ln, las, (uint32_t)nasserts, out, nregs_out, ninstr_out);
+}
+
+static int air_compile_lowered(int field, const std::vector<pg_air_node>& lnodes, const std::vector<LAssert>& las,
+                               uint32_t norig, pg_air_program** out, uint32_t* nregs_out, uint32_t* ninstr_out) {
+    const pg_air_node* const nodes = lnodes.data();
+    const size_t nnodes = lnodes.size(), nasserts = las.size();
     // Leaves (columns, constants, selectors, public values) are operands read
     // in place: columns from staged windows, the rest from the constant table
     // or the selectors.  Only inner nodes take registers, so the register
     // count follows the depth of the expressions, not the columns read.
-    auto is_leaf = [&](uint32_t k) { return nodes[k].kind < PG_AIR_ADD; };
+    auto is_leaf = [&](uint32_t k) {
+        const uint32_t kd = nodes[k].kind;
+        return kd != PG_AIR_ADD && kd != PG_AIR_SUB && kd != PG_AIR_NEG && kd != PG_AIR_MUL;
+    };
     auto is_binary = [&](uint32_t k) {
         return nodes[k].kind == PG_AIR_ADD || nodes[k].kind == PG_AIR_SUB || nodes[k].kind == PG_AIR_MUL;
     };
@@ -3658,7 +3787,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
     std::vector<uint8_t> state(nnodes, 0);  // 0 new, 1 on stack, 2 emitted
     std::vector<uint32_t> stack;
     for (size_t j = 0; j < nasserts; j++) {
-        if (!is_leaf(asserts[j])) stack.push_back(asserts[j]);
+        if (!is_leaf(las[j].node)) stack.push_back(las[j].node);
         while (!stack.empty()) {
             const uint32_t k = stack.back();
             if (state[k] == 2) {
@@ -3676,7 +3805,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             stack.pop_back();
             order.push_back(Item{false, k});
         }
-        order.push_back(Item{true, asserts[j]});
+        order.push_back(Item{true, las[j].node});
     }
     // last reader of every computed inner node
     std::vector<size_t> last_use(nnodes, 0);
@@ -3701,6 +3830,8 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
         std::unique_ptr<pg_air_program> prog(new pg_air_program);
         prog->field = field;
         prog->nasserts = (uint32_t)nasserts;
+        prog->norig = norig;
+        for (const LAssert& la : las) prog->afactor.push_back({la.comp, la.orig});
         std::vector<uint32_t> reg(nnodes, 0), free_regs;
         auto take_reg = [&]() {
             if (free_regs.empty()) return prog->nregs++;
@@ -3711,7 +3842,7 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
         };
         // Symbolic operands, encoded once the register count (which places
         // the windows after the registers) and the constant count are known.
-        enum : uint8_t { OP_REG, OP_WIN, OP_WIN_NEXT, OP_CONST, OP_PUBLIC, OP_SEL };
+        enum : uint8_t { OP_REG, OP_WIN, OP_WIN_NEXT, OP_CONST, OP_PUBLIC, OP_SEL, OP_CHAL, OP_RSUM, OP_GSUM };
         enum : uint32_t { P_STAGE, P_ADD, P_SUB, P_MUL, P_NEG, P_ASSERT };  // pending instructions
         enum : uint32_t { S_FIRST, S_LAST, S_TRANS, S_PREV };               // OP_SEL operands
         struct Op {
@@ -3779,15 +3910,21 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
             }
             switch (nd.kind) {
                 case PG_AIR_MAIN:
-                case PG_AIR_PREPROCESSED: {
-                    const bool pre = nd.kind == PG_AIR_PREPROCESSED;
-                    if (pre) {
+                case PG_AIR_PREPROCESSED:
+                case LK_PERM: {
+                    uint32_t src = 0;  // window source: main, preprocessed or permutation rows
+                    if (nd.kind == PG_AIR_PREPROCESSED) {
                         prog->reads_pre = true;
                         prog->max_pre_col = std::max(prog->max_pre_col, nd.b + 1);
+                        src = 2;
+                    } else if (nd.kind == LK_PERM) {
+                        prog->reads_perm = true;
+                        prog->max_perm_col = std::max(prog->max_perm_col, nd.b + 1);
+                        src = 4;
                     } else {
                         prog->max_main_col = std::max(prog->max_main_col, nd.b + 1);
                     }
-                    const int sl = window((pre ? 2u : 0u) + nd.a, nd.b, keep);
+                    const int sl = window(src + nd.a, nd.b, keep);
                     *slot = sl;
                     o.kind = nd.a ? OP_WIN_NEXT : OP_WIN;
                     o.x = (uint32_t)sl * QSLOT + (nd.b % QW) * QRS;
@@ -3796,6 +3933,21 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
                 case PG_AIR_PUBLIC:
                     prog->max_public = std::max(prog->max_public, nd.b + 1);
                     o.kind = OP_PUBLIC;
+                    o.x = nd.b;
+                    return o;
+                case LK_CHAL:
+                    prog->max_chal_word = std::max(prog->max_chal_word, nd.b + 1);
+                    o.kind = OP_CHAL;
+                    o.x = nd.b;
+                    return o;
+                case LK_RSUM:
+                    prog->reads_rsum = true;
+                    o.kind = OP_RSUM;
+                    o.x = nd.b;
+                    return o;
+                case LK_GSUM:
+                    prog->max_gsum = std::max(prog->max_gsum, nd.b + 1);
+                    o.kind = OP_GSUM;
                     o.x = nd.b;
                     return o;
                 case PG_AIR_IS_FIRST_ROW:
@@ -3848,11 +4000,18 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
         const uint32_t wbase = prog->nregs * QB;
         const uint32_t sbase = wbase + prog->nslots * QSLOT;
         const uint32_t cbase = sbase + (prog->need_selectors ? 3u * QB : 0u);
+        // the constant table at launch: constants, public values, challenge
+        // words, the regional sum, the global-sum coordinates
+        const uint32_t pub0 = cbase + nconst, chal0 = pub0 + prog->max_public;
+        const uint32_t rsum0 = chal0 + prog->max_chal_word, gsum0 = rsum0 + (prog->reads_rsum ? 4u : 0u);
         auto kind = [&](const Op& o) -> uint32_t {
             switch (o.kind) {
                 case OP_WIN_NEXT: return QK_NEXT;
                 case OP_CONST:
-                case OP_PUBLIC: return QK_BCAST;
+                case OP_PUBLIC:
+                case OP_CHAL:
+                case OP_RSUM:
+                case OP_GSUM: return QK_BCAST;
                 case OP_SEL: return o.x == S_PREV ? QK_PREV : QK_ROW;
                 default: return QK_ROW;
             }
@@ -3863,7 +4022,10 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
                 case OP_WIN:
                 case OP_WIN_NEXT: return wbase + o.x;
                 case OP_CONST: return cbase + o.x;
-                case OP_PUBLIC: return cbase + nconst + o.x;
+                case OP_PUBLIC: return pub0 + o.x;
+                case OP_CHAL: return chal0 + o.x;
+                case OP_RSUM: return rsum0 + o.x;
+                case OP_GSUM: return gsum0 + o.x;
                 default: return o.x == S_PREV ? 0u : sbase + o.x * QB;
             }
         };
@@ -3904,13 +4066,18 @@ extern "C" int pg_air_compile(int field, const pg_air_node* nodes, size_t nnodes
 
 template <class F>
 static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, const pg_buf* mm, const pg_buf* pm,
-                      int log_n, uint32_t qdb, const uint32_t* publics, const uint32_t* alpha_w, pg_buf** out) {
+                      const pg_buf* qm, int log_n, uint32_t qdb, const uint32_t* publics, const uint32_t* chal,
+                      const uint32_t* rsum, const uint32_t* gsum, const uint32_t* alpha_w, pg_buf** out) {
     const int log_q = log_n + (int)qdb;
     const uint32_t q = 1u << log_q;
-    // constant table (the program's constants, then the public values), the
-    // Z_H table and the powers of alpha
+    // constant table (the program's constants, then the public values, the
+    // challenge words, the regional sum, the global-sum coordinates), the Z_H
+    // table and the powers of alpha
     std::vector<uint32_t> consts = prog->consts;
     consts.insert(consts.end(), publics, publics + prog->max_public);
+    consts.insert(consts.end(), chal, chal + prog->max_chal_word);
+    if (prog->reads_rsum) consts.insert(consts.end(), rsum, rsum + 4);
+    consts.insert(consts.end(), gsum, gsum + prog->max_gsum);
     const uint32_t nz = 1u << qdb;
     std::vector<uint32_t> zh(2 * nz);
     const uint32_t gen = to_monty<F>(F::GEN);
@@ -3925,14 +4092,23 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
     if (quotient_lds_bytes(prog->nregs, prog->nslots, prog->need_selectors, (uint32_t)consts.size()) > 65536)
         return fail(PG_E_UNSUPPORTED, "the program's registers, windows and %zu constants need more than 64 KiB of LDS",
                     consts.size());
+    // base assert i folds coordinate e of original assert j: X^e alpha^(m-1-j)
     const uint32_t m = prog->nasserts;
     std::vector<uint32_t> apow((size_t)m * 4);
     {
+        std::vector<Ext4> pw(prog->norig);
         const Ext4 al = ext_from<F>(alpha_w);
-        Ext4 pw{{F::R1, 0u, 0u, 0u}};
-        for (uint32_t j = m; j-- > 0;) {  // assert j is multiplied by alpha^(m-1-j)
-            for (int e = 0; e < 4; e++) apow[(size_t)j * 4 + e] = pw.c[e];
-            pw = emul<F>(pw, al);
+        Ext4 cur{{F::R1, 0u, 0u, 0u}};
+        for (uint32_t j = prog->norig; j-- > 0;) {
+            pw[j] = cur;
+            cur = emul<F>(cur, al);
+        }
+        const uint32_t W = to_monty<F>(F::P == KoalaBear::P ? 3u : 11u);  // X^4 = W
+        for (uint32_t i = 0; i < m; i++) {
+            Ext4 v = pw[prog->afactor[i].second];
+            for (uint32_t e = 0; e < prog->afactor[i].first; e++)  // times X
+                v = Ext4{{fmul<F>(v.c[3], W), v.c[0], v.c[1], v.c[2]}};
+            for (int c = 0; c < 4; c++) apow[(size_t)i * 4 + c] = v.c[c];
         }
     }
     // the kernel reads one instruction past the end (zero padding)
@@ -3964,6 +4140,9 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
     a.pre = pm ? pm->ptr : nullptr;
     a.ld_pre = pm ? pm->ld : 0;
     a.w_pre = pm ? (uint32_t)pm->cols : 0;
+    a.perm = qm ? qm->ptr : nullptr;
+    a.ld_perm = qm ? qm->ld : 0;
+    a.w_perm = qm ? (uint32_t)qm->cols : 0;
     a.prog = (const QInstr*)dd;
     a.nprog = (uint32_t)prog->code.size();
     a.nregs = prog->nregs;
@@ -3999,9 +4178,11 @@ static int quotient_t(pg_ctx* ctx, hipStream_t s, const pg_air_program* prog, co
     return PG_OK;
 }
 
-extern "C" int pg_quotient_values(pg_ctx* ctx, const pg_air_program* prog, const pg_tree* main, size_t main_k,
-                                  const pg_tree* pre, size_t pre_k, uint32_t log_blowup, uint32_t log_qd,
-                                  const uint32_t* publics, size_t npublics, const uint32_t alpha[4], pg_buf** out) {
+extern "C" int pg_quotient_values_perm(pg_ctx* ctx, const pg_air_program* prog, const pg_tree* main, size_t main_k,
+                                       const pg_tree* pre, size_t pre_k, const pg_tree* perm, size_t perm_k,
+                                       uint32_t log_blowup, uint32_t log_qd, const uint32_t* publics, size_t npublics,
+                                       const uint32_t* chal, size_t nchal, const uint32_t rsum[4],
+                                       const uint32_t* gsum, size_t nglobal, const uint32_t alpha[4], pg_buf** out) {
     if (!ctx || !prog || !main || !alpha || !out || main_k >= main->mats.size())
         return fail(PG_E_INVALID, "bad argument");
     *out = nullptr;
@@ -4016,22 +4197,40 @@ extern "C" int pg_quotient_values(pg_ctx* ctx, const pg_air_program* prog, const
     if (log_n + (int)log_qd > 32 - 1) return fail(PG_E_UNSUPPORTED, "quotient domain too large");
     if (prog->max_main_col > mm->cols)
         return fail(PG_E_INVALID, "the program reads main column %u of %zu", prog->max_main_col - 1, mm->cols);
-    const pg_buf* pm = nullptr;
-    if (prog->reads_pre) {
-        if (!pre || pre_k >= pre->mats.size()) return fail(PG_E_INVALID, "the program reads a preprocessed trace");
-        if (pre->field != prog->field) return fail(PG_E_INVALID, "preprocessed trace over another field");
-        pm = pre->mats[pre_k];
-        if (pm->rows != mm->rows)
-            return fail(PG_E_INVALID, "preprocessed height %zu differs from the main trace's %zu", pm->rows, mm->rows);
-        if (prog->max_pre_col > pm->cols)
-            return fail(PG_E_INVALID, "the program reads preprocessed column %u of %zu", prog->max_pre_col - 1,
-                        pm->cols);
-    }
+    // the preprocessed and permutation traces: committed at the same height
+    auto side = [&](bool reads, const pg_tree* t, size_t k, uint32_t max_col, const char* what,
+                    const pg_buf** m) -> int {
+        *m = nullptr;
+        if (!reads) return PG_OK;
+        if (!t || k >= t->mats.size()) return fail(PG_E_INVALID, "the program reads a %s trace", what);
+        if (t->field != prog->field) return fail(PG_E_INVALID, "%s trace over another field", what);
+        *m = t->mats[k];
+        if ((*m)->rows != mm->rows)
+            return fail(PG_E_INVALID, "%s height %zu differs from the main trace's %zu", what, (*m)->rows, mm->rows);
+        if (max_col > (*m)->cols)
+            return fail(PG_E_INVALID, "the program reads %s column %u of %zu", what, max_col - 1, (*m)->cols);
+        return PG_OK;
+    };
+    const pg_buf *pm, *qm;
+    PG_TRY(side(prog->reads_pre, pre, pre_k, prog->max_pre_col, "preprocessed", &pm));
+    PG_TRY(side(prog->reads_perm, perm, perm_k, prog->max_perm_col, "permutation", &qm));
     if (prog->max_public > npublics || (npublics && !publics))
         return fail(PG_E_INVALID, "the program reads public value %u of %zu", prog->max_public - 1, npublics);
+    if (prog->max_chal_word > 4 * nchal || (nchal && !chal))
+        return fail(PG_E_INVALID, "the program reads challenge %u of %zu", (prog->max_chal_word - 1) / 4, nchal);
+    if (prog->reads_rsum && !rsum) return fail(PG_E_INVALID, "the program reads the regional cumulative sum");
+    if (prog->max_gsum > nglobal || (nglobal && !gsum))
+        return fail(PG_E_INVALID, "the program reads global-sum coordinate %u of %zu", prog->max_gsum - 1, nglobal);
     PG_TRY(bind_device(ctx));
     Lease l(ctx);
     return prog->field == PG_KOALABEAR
-               ? quotient_t<KoalaBear>(ctx, l.s, prog, mm, pm, log_n, log_qd, publics, alpha, out)
-               : quotient_t<BabyBear>(ctx, l.s, prog, mm, pm, log_n, log_qd, publics, alpha, out);
+               ? quotient_t<KoalaBear>(ctx, l.s, prog, mm, pm, qm, log_n, log_qd, publics, chal, rsum, gsum, alpha, out)
+               : quotient_t<BabyBear>(ctx, l.s, prog, mm, pm, qm, log_n, log_qd, publics, chal, rsum, gsum, alpha, out);
+}
+
+extern "C" int pg_quotient_values(pg_ctx* ctx, const pg_air_program* prog, const pg_tree* main, size_t main_k,
+                                  const pg_tree* pre, size_t pre_k, uint32_t log_blowup, uint32_t log_qd,
+                                  const uint32_t* publics, size_t npublics, const uint32_t alpha[4], pg_buf** out) {
+    return pg_quotient_values_perm(ctx, prog, main, main_k, pre, pre_k, nullptr, 0, log_blowup, log_qd, publics,
+                                   npublics, nullptr, 0, nullptr, nullptr, 0, alpha, out);
 }

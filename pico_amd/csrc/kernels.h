@@ -212,7 +212,8 @@ hipError_t launch_fill_random(int field, uint64_t seed, uint32_t* out, size_t n,
 //   QOP_ASSERT + ka     the accumulator gains a times the next power of alpha
 //   QOP_STAGE           stage columns [a, a + QW) of source b (0 main local,
 //                       1 main next, 2 preprocessed local, 3 preprocessed
-//                       next) into window slot dst
+//                       next, 4 permutation local, 5 permutation next) into
+//                       window slot dst
 // dst of an arithmetic instruction is the LDS word of its register, or
 // QO_NO_DST for a value only the next instruction reads (forwarded).
 constexpr uint32_t QK_ROW = 0, QK_NEXT = 1, QK_BCAST = 2, QK_PREV = 3;
@@ -236,6 +237,9 @@ struct QuotientArgs {
     const uint32_t* pre;   // may be null when the program reads no preprocessed column
     uint64_t ld_pre;
     uint32_t w_pre;
+    const uint32_t* perm;  // the permutation trace, flattened to base columns (or null)
+    uint64_t ld_perm;
+    uint32_t w_perm;
     const QInstr* prog;
     uint32_t nprog, nregs, nslots;
     const uint32_t* consts;  // constant table (Montgomery), copied to LDS after the selectors

@@ -205,10 +205,11 @@ __global__ void __launch_bounds__(QT) quotient_kernel(QuotientArgs a) {
         } else {
             // window of columns [ia, ia + QW) of the QB slot rows: thread t
             // loads column ia + t % QW of slot rows t / QW + (QT / QW) m
-            const bool pre = ib >= 2, next_rows = (ib & 1) != 0;
-            const gcptr base = (gcptr)(pre ? a.pre : a.main);
-            const uint64_t ld = pre ? a.ld_pre : a.ld_main;
-            const uint32_t width = pre ? a.w_pre : a.w_main;
+            const uint32_t which = ib >> 1;  // main, preprocessed, permutation
+            const bool next_rows = (ib & 1) != 0;
+            const gcptr base = (gcptr)(which == 0 ? a.main : which == 1 ? a.pre : a.perm);
+            const uint64_t ld = which == 0 ? a.ld_main : which == 1 ? a.ld_pre : a.ld_perm;
+            const uint32_t width = which == 0 ? a.w_main : which == 1 ? a.w_pre : a.w_perm;
             const uint32_t rb = next_rows ? r0n : r0;
             const uint32_t c = t % (uint32_t)QW, u0 = t / (uint32_t)QW;
             const uint32_t col = ia + c < width ? ia + c : ia;  // columns past the width: never read

@@ -149,5 +149,109 @@ def random_air(f, rng, width, pre_width, nconstraints, npublics=3):
     return b, 2
 
 
+def lookup_air(f):
+    """The permutation (LogUp) constraints Pico builds for a chip's lookups
+    (vm/src/machine/permutation.rs:183-346), here for one batch of two
+    interactions over main columns (a, b, m1, m2, g): a send of (kind 1, a)
+    with multiplicity m1 and a receive of (kind 2, a, b) with multiplicity
+    m2.  rlc_i = alpha + kind_i + beta a (+ beta^2 b) with (alpha, beta) the
+    permutation challenges; the permutation trace has the batch entry and
+    the running sum phi (two ext4 columns):
+      rlc_1 rlc_2 entry = m1 rlc_2 - m2 rlc_1
+      first row: phi = entry;  transition: phi' - phi = entry'
+      last row: phi = regional cumulative sum
+    plus the global-sum check of the last row, g = global sum coordinate 0.
+    Degree 3 (log quotient degree 1)."""
+    b = AirBuilder(f)
+    a, bb, m1, m2, g = (b.main(i) for i in range(5))
+    alpha, beta = b.perm_challenge(0), b.perm_challenge(1)
+    entry, phi = b.permutation(0), b.permutation(1)
+    entry_n, phi_n = b.permutation(0, next=True), b.permutation(1, next=True)
+    rlc1 = alpha + b.constant(1) + beta * a
+    rlc2 = alpha + b.constant(2) + beta * a + beta * beta * bb
+    b.assert_zero(rlc1 * rlc2 * entry - (m1 * rlc2 - m2 * rlc1))
+    b.when_first_row().assert_zero(phi - entry)
+    b.when_transition().assert_zero(phi_n - phi - entry_n)
+    b.when_last_row().assert_zero(phi - b.regional_sum())
+    b.when_last_row().assert_eq(g, b.global_sum(0))
+    return b, 1
+
+
+def lookup_trace(f, n, rng, oracle):
+    """(main n x 5, permutation n x 8 (ext4 columns entry, phi flattened),
+    challenges [alpha, beta], regional sum, global sum (14 coordinates)),
+    all canonical; a valid trace of lookup_air."""
+    p = P[f]
+    main = rng.integers(0, p, (n, 5)).astype(np.uint64)
+    main[:, 2:4] = rng.integers(0, 5, (n, 2))  # small multiplicities
+    alpha = [int(x) for x in rng.integers(0, p, 4)]
+    beta = [int(x) for x in rng.integers(0, p, 4)]
+
+    def ext(v):
+        return [int(v) % p, 0, 0, 0]
+
+    def add(x, y):
+        return [(u + v) % p for u, v in zip(x, y)]
+
+    def mul(x, y):
+        return [int(t) for t in oracle.ext_mul(f, x, y)]
+
+    beta2 = mul(beta, beta)
+    perm = np.zeros((n, 8), np.uint64)
+    phi = [0, 0, 0, 0]
+    for i in range(n):
+        a, bb, m1, m2 = (int(x) for x in main[i, :4])
+        rlc1 = add(add(alpha, ext(1)), mul(beta, ext(a)))
+        rlc2 = add(add(add(alpha, ext(2)), mul(beta, ext(a))), mul(beta2, ext(bb)))
+        entry = add(mul(ext(m1), [int(t) for t in oracle.ext_inv(f, rlc1)]),
+                    mul(ext(p - m2 if m2 else 0), [int(t) for t in oracle.ext_inv(f, rlc2)]))
+        phi = add(phi, entry)
+        perm[i, 0:4] = entry
+        perm[i, 4:8] = phi
+    gsum = [int(x) for x in rng.integers(0, p, 14)]
+    gsum[0] = int(main[n - 1, 4])
+    return main.astype(np.uint32), perm.astype(np.uint32), [alpha, beta], list(phi), gsum
+
+
+def random_ext_air(f, rng, width, perm_cols, nconstraints):
+    """Random extension-field constraints: sums, differences and products of
+    up to four leaves drawn from main columns, permutation ext4 columns (local
+    or next), two challenges, the regional sum, global-sum coordinates and
+    constants, some under a when_* filter; every coordinate of the lowered
+    extension arithmetic shows."""
+    b = AirBuilder(f)
+    p = P[f]
+
+    def leaf():
+        r = rng.integers(0, 10)
+        if r < 3:
+            return b.main(int(rng.integers(0, width)), next=bool(rng.integers(0, 2)))
+        if r < 6:
+            return b.permutation(int(rng.integers(0, perm_cols)), next=bool(rng.integers(0, 2)))
+        if r == 6:
+            return b.perm_challenge(int(rng.integers(0, 2)))
+        if r == 7:
+            return b.regional_sum()
+        if r == 8:
+            return b.global_sum(int(rng.integers(0, 14)))
+        return b.constant(int(rng.integers(0, p)))
+
+    for _ in range(nconstraints):
+        e = leaf()
+        for _ in range(int(rng.integers(0, 4))):
+            op = rng.integers(0, 3)
+            e = e * leaf() if op == 0 else e + leaf() if op == 1 else e - leaf()
+        if rng.integers(0, 5) == 0:
+            e = -e
+        sel = rng.integers(0, 4)
+        if sel == 0:
+            b.when_first_row().assert_zero(e)
+        elif sel == 1:
+            b.when_transition().assert_zero(e)
+        else:
+            b.assert_zero(e)
+    return b, 2
+
+
 AIRS = {"toy": (toy_air, toy_trace), "fibonacci": (fibonacci_air, fibonacci_trace),
         "cube": (cube_air, cube_trace), "gated": (gated_air, None)}

@@ -103,6 +103,79 @@ def test_quotient_wide_random_air_matches_oracle(ctx, oracle, f):
             assert np.array_equal(got, want), (log_n, qdb, lb)
 
 
+@pytest.mark.parametrize("f", FIELDS)
+def test_quotient_lookup_matches_oracle(ctx, oracle, f):
+    """The permutation (LogUp) constraints of a two-interaction batch
+    (tests/airs.py lookup_air: extension-valued columns, challenges, the
+    regional sum, the global-sum check) over the committed main and
+    permutation traces, heights 2^0..2^10, blowup 2 and 4, valid and random
+    traces, against the oracle's direct extension arithmetic."""
+    rng = np.random.default_rng(71 + f)
+    p = pico_amd.PRIMES[f]
+    g = pico_amd.GENERATORS[f]
+    builder, qdb = airs.lookup_air(f)
+    prog = builder.compile()
+    nodes, asserts = builder.node_array(), builder.asserts
+    for log_n in (0, 1, 4, 7, 10):
+        for lb in (1, 2):
+            for valid in (True, False):
+                n = 1 << log_n
+                print(f"lookup quotient field {f} 2^{log_n} b={lb} valid={valid}", flush=True)
+                main, perm, chal, rsum, gsum = airs.lookup_trace(f, n, rng, oracle)
+                if not valid:
+                    perm = rng.integers(0, p, perm.shape).astype(np.uint32)
+                    rsum = [int(x) for x in rng.integers(0, p, 4)]
+                alpha = [int(x) for x in rng.integers(0, p, 4)]
+                _, md = _commit(ctx, f, lb, main)
+                _, qd = _commit(ctx, f, lb, perm)
+                q = air.quotient_values(ctx, prog, md, 0, lb, qdb, alpha, perm_data=qd, challenges=chal,
+                                        regional_sum=rsum, global_sum=gsum)
+                got = q.to_host(f)
+                q.free()
+                md.free()
+                qd.free()
+                want = oracle.quotient_values(f, nodes, asserts, oracle.coset_lde(f, main, qdb, g, bit_reverse=False),
+                                              log_n, qdb, alpha, (), None,
+                                              oracle.coset_lde(f, perm, qdb, g, bit_reverse=False), chal, rsum, gsum)
+                assert np.array_equal(got, want), (log_n, lb, valid)
+
+
+@pytest.mark.parametrize("f", FIELDS)
+def test_quotient_random_extension_air_matches_oracle(ctx, oracle, f):
+    """Random extension-field constraints (every leaf kind, extension sums,
+    differences, products and negations, base factors, filters) over 20 main
+    and 6 permutation ext4 columns (24 base columns: windows past the width),
+    heights 2^0..2^8, quotient degree 2^0..2^2."""
+    rng = np.random.default_rng(83 + f)
+    p = pico_amd.PRIMES[f]
+    g = pico_amd.GENERATORS[f]
+    builder, _ = airs.random_ext_air(f, rng, 20, 6, 60)
+    prog = builder.compile()
+    nodes, asserts = builder.node_array(), builder.asserts
+    for log_n in (0, 3, 6, 8):
+        for qdb, lb in ((0, 1), (1, 1), (2, 2)):
+            n = 1 << log_n
+            print(f"random ext quotient field {f} 2^{log_n} qdb={qdb} b={lb}", flush=True)
+            main = rng.integers(0, p, (n, 20)).astype(np.uint32)
+            perm = rng.integers(0, p, (n, 24)).astype(np.uint32)
+            chal = [[int(x) for x in rng.integers(0, p, 4)] for _ in range(2)]
+            rsum = [int(x) for x in rng.integers(0, p, 4)]
+            gsum = [int(x) for x in rng.integers(0, p, 14)]
+            alpha = [int(x) for x in rng.integers(0, p, 4)]
+            _, md = _commit(ctx, f, lb, main)
+            _, qd = _commit(ctx, f, lb, perm)
+            q = air.quotient_values(ctx, prog, md, 0, lb, qdb, alpha, perm_data=qd, challenges=chal,
+                                    regional_sum=rsum, global_sum=gsum)
+            got = q.to_host(f)
+            q.free()
+            md.free()
+            qd.free()
+            want = oracle.quotient_values(f, nodes, asserts, oracle.coset_lde(f, main, qdb, g, bit_reverse=False),
+                                          log_n, qdb, alpha, (), None,
+                                          oracle.coset_lde(f, perm, qdb, g, bit_reverse=False), chal, rsum, gsum)
+            assert np.array_equal(got, want), (log_n, qdb, lb)
+
+
 def test_quotient_rejects_bad_requests(ctx):
     f = KOALABEAR
     builder, _ = airs.cube_air(f)
@@ -122,6 +195,26 @@ def test_quotient_rejects_bad_requests(ctx):
         gated, _ = airs.gated_air(f)
         with pytest.raises(pico_amd.PgError):  # preprocessed trace missing
             air.quotient_values(ctx, gated.compile(), md, 0, 1, 0, [1, 2, 3, 4])
+        look, _ = airs.lookup_air(f)
+        lp = look.compile()
+        ch = [[1, 2, 3, 4], [5, 6, 7, 8]]
+        gs = list(range(14))
+        with pytest.raises(pico_amd.PgError):  # permutation trace missing
+            air.quotient_values(ctx, lp, md, 0, 1, 1, [1, 2, 3, 4], challenges=ch, regional_sum=[1, 0, 0, 0],
+                                global_sum=gs)
+        with pytest.raises(pico_amd.PgError):  # permutation trace of another height
+            air.quotient_values(ctx, lp, md, 0, 1, 1, [1, 2, 3, 4], perm_data=_commit(
+                ctx, f, 1, np.zeros((8, 8), np.uint32))[1], challenges=ch, regional_sum=[1, 0, 0, 0], global_sum=gs)
+        qd = _commit(ctx, f, 1, np.zeros((16, 8), np.uint32))[1]
+        with pytest.raises(pico_amd.PgError):  # one challenge of two
+            air.quotient_values(ctx, lp, md, 0, 1, 1, [1, 2, 3, 4], perm_data=qd, challenges=ch[:1],
+                                regional_sum=[1, 0, 0, 0], global_sum=gs)
+        with pytest.raises(pico_amd.PgError):  # regional sum missing
+            air.quotient_values(ctx, lp, md, 0, 1, 1, [1, 2, 3, 4], perm_data=qd, challenges=ch, global_sum=gs)
+        with pytest.raises(pico_amd.PgError):  # global sum missing
+            air.quotient_values(ctx, lp, md, 0, 1, 1, [1, 2, 3, 4], perm_data=qd, challenges=ch,
+                                regional_sum=[1, 0, 0, 0])
+        qd.free()
     finally:
         md.free()
 

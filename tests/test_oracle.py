@@ -554,6 +554,51 @@ def test_quotient_oracle_meets_the_verifier_identity(oracle, f, name):
     assert not check(bad)
 
 
+@pytest.mark.parametrize("f", [0, 1])
+def test_quotient_oracle_lookup_meets_the_verifier_identity(oracle, f):
+    """The permutation (LogUp) constraints of a valid two-interaction batch
+    (tests/airs.py lookup_air): the oracle's quotient on Q, interpolated and
+    evaluated at a random zeta, equals the constraints folded at zeta from
+    the main and permutation traces' values at zeta and zeta w_n times
+    1 / Z_H(zeta) (vm/src/machine/verifier.rs:307-356, the permutation
+    columns recombined from their base coordinates).  A wrong running-sum
+    entry breaks it, and so does a wrong regional sum."""
+    from tests import airs
+    rng = np.random.default_rng(60 + f)
+    log_n = 5
+    n = 1 << log_n
+    builder, qdb = airs.lookup_air(f)
+    nodes, asserts = builder.node_array(), builder.asserts
+    main, perm, chal, rsum, gsum = airs.lookup_trace(f, n, rng, oracle)
+    g = oracle.generator(f)
+    p = oracle.PRIMES[f]
+    wn = oracle.two_adic_generator(f, log_n)
+    zeta = [int(x) for x in rng.integers(0, p, 4)]
+    alpha = [int(x) for x in rng.integers(0, p, 4)]
+
+    def check(perm, rsum):
+        main_q = oracle.coset_lde(f, main, qdb, g, bit_reverse=False)
+        perm_q = oracle.coset_lde(f, perm, qdb, g, bit_reverse=False)
+        quot = oracle.quotient_values(f, nodes, asserts, main_q, log_n, qdb, alpha, (), None, perm_q, chal, rsum,
+                                      gsum)
+        qn = quot.shape[0]
+        rev = [int(format(i, f"0{log_n + qdb}b")[::-1], 2) for i in range(qn)]
+        q_at = _ext_from_coords(oracle, f, oracle.pcs_open_values(f, quot[rev], 0, zeta))
+        zn = _ext_scale(f, zeta, wn)
+        lde, plde = oracle.coset_lde(f, main, 1, g), oracle.coset_lde(f, perm, 1, g)
+        folded, inv_zh = oracle.constraints_at_point(
+            f, nodes, asserts, oracle.pcs_open_values(f, lde, 1, zeta), oracle.pcs_open_values(f, lde, 1, zn), zeta,
+            log_n, alpha, (), None, None, oracle.pcs_open_values(f, plde, 1, zeta),
+            oracle.pcs_open_values(f, plde, 1, zn), chal, rsum, gsum)
+        return [int(x) for x in oracle.ext_mul(f, folded, inv_zh)] == q_at
+
+    assert check(perm, rsum)
+    bad = perm.copy()
+    bad[n // 3, 5] = (int(bad[n // 3, 5]) + 1) % p  # a running-sum coordinate
+    assert not check(bad, rsum)
+    assert not check(perm, [(rsum[0] + 1) % p] + list(rsum[1:]))
+
+
 def test_air_compile_reuses_registers():
     """pg_air_compile: a chain of 200 asserts over shared subexpressions
     needs few registers (each freed after its last reader; leaves are read
@@ -580,3 +625,37 @@ def test_air_compile_reuses_registers():
         pico_amd.air.AirProgram(0, np.array([[0, 2, 0]], np.uint32), np.zeros(1, np.uint32))  # offset 2
     with pytest.raises(pico_amd.PgError):
         pico_amd.air.AirProgram(0, np.array([[0, 0, 0]], np.uint32), np.array([3], np.uint32))  # assert range
+    with pytest.raises(pico_amd.PgError):  # permutation row offset 2
+        pico_amd.air.AirProgram(0, np.array([[11, 2, 0]], np.uint32), np.zeros(1, np.uint32))
+    with pytest.raises(pico_amd.PgError):  # global-sum coordinate beyond 13
+        pico_amd.air.AirProgram(0, np.array([[14, 0, 14]], np.uint32), np.zeros(1, np.uint32))
+    with pytest.raises(pico_amd.PgError):  # unknown kind
+        pico_amd.air.AirProgram(0, np.array([[15, 0, 0]], np.uint32), np.zeros(1, np.uint32))
+
+
+def test_air_compile_lowers_extension_arithmetic():
+    """pg_air_compile: an extension-valued assert becomes four base asserts
+    (one per coordinate), a base-by-extension product four products, an
+    extension product 16 products and the W terms; a base assert stays one.
+    Counted through the program length of single-assert AIRs whose leaves all
+    sit in one column window."""
+    import pico_amd
+    from pico_amd.air import AirBuilder
+
+    def ninstr(make):
+        b = AirBuilder(0)
+        b.assert_zero(make(b))
+        return b.compile().ninstr
+
+    stages = ninstr(lambda b: b.main(0) * b.main(1)) - 2  # one mul + one assert
+    assert stages == 1
+    # ext4 column: four coordinate asserts (+ the window of base columns 0..3)
+    assert ninstr(lambda b: b.permutation(0)) == 1 + 4
+    # base x ext: four products and four asserts (+ the main column's window)
+    assert ninstr(lambda b: b.main(0) * b.perm_challenge(0)) == 1 + 4 + 4
+    # ext x ext: 16 products, 12 sums (3 per coordinate: c_0 = a0b0 +
+    # W(a1b3 + a2b2 + a3b1), ..., c_3 = a0b3 + a1b2 + a2b1 + a3b0), 3 products
+    # by W, 4 asserts
+    assert ninstr(lambda b: b.permutation(0) * b.perm_challenge(1)) == 1 + 16 + 12 + 3 + 4
+    # a base constant added to an ext value changes coordinate 0 only
+    assert ninstr(lambda b: b.regional_sum() + 5) == 1 + 4
