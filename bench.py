@@ -493,6 +493,7 @@ def main():
         out["chip_tables"] = workloads.chip_tables(ctx)
         progress("chip tables done")
         out["quotient_eval"] = workloads.quotient_eval(ctx)
+        out["quotient_eval_lookups"] = workloads.quotient_eval_lookups(ctx)
         progress("quotient evaluation done")
         out["batched_jobs"] = workloads.batched_jobs(ctx)
         progress("batched jobs done")

@@ -19,7 +19,9 @@ device through the C ABI and reports wall time from the host clock.
                      call per table then one MMCS commit
   quotient_eval      compute_quotient_values on the device for a 256-column
                      chip (2^20 rows, degree 3) against the readback it
-                     replaces
+                     replaces; and the LogUp side for 32 interactions in
+                     batches of 2 (extension-field constraints over a
+                     17-column permutation trace)
   batched_jobs       64 single-matrix commits, log-heights 10..18, widths
                      8..512, blowup 2/4/8: one launch sequence (hipGraph) vs
                      the same jobs committed one by one
@@ -320,6 +322,94 @@ def quotient_eval(ctx, log_n=20, width=256, log_blowup=1, reps=3):
             "program_instructions": prog.ninstr, "program_registers": prog.nregs,
             "ms": min(ts) * 1e3, "kernel_ms": kms, "rows_per_s": rows_q / min(ts),
             "readback_it_replaces_ms": min(rb) * 1e3}
+
+
+def lookup_air(width, interactions, batch):
+    """The LogUp constraints Pico builds for a chip's lookups
+    (vm/src/machine/permutation.rs:183-346) for `interactions` interactions
+    over a `width`-column trace in batches of `batch` (the reference's batch
+    size at blowup 2 keeps the degree at 3): interaction i sends (kind i % 8,
+    main[2i], main[2i+1]) with multiplicity main[2i+2] (every other one is a
+    receive); per batch rlc_1 ... rlc_b entry = sum_i m_i prod_{j != i}
+    rlc_j; the running sum phi with its first-row, transition and last-row
+    constraints."""
+    from . import air
+    f = core.KOALABEAR
+    b = air.AirBuilder(f)
+    alpha, beta = b.perm_challenge(0), b.perm_challenge(1)
+    beta2 = beta * beta
+    nb = (interactions + batch - 1) // batch
+    col = lambda i: b.main(i % width)
+    for k in range(nb):
+        rlcs, mults = [], []
+        for i in range(k * batch, min(interactions, (k + 1) * batch)):
+            rlcs.append(alpha + b.constant(i % 8) + beta * col(2 * i) + beta2 * col(2 * i + 1))
+            m = col(2 * i + 2)
+            mults.append(m if i % 2 == 0 else -m)
+        prod, num = None, None
+        for i, (m, r) in enumerate(zip(mults, rlcs)):
+            prod = r if prod is None else prod * r
+            others = None
+            for j, o in enumerate(rlcs):
+                if j != i:
+                    others = o if others is None else others * o
+            term = m * others if others is not None else m
+            num = term if num is None else num + term
+        b.assert_zero(prod * b.permutation(k) - num)
+    entries = [b.permutation(k) for k in range(nb)]
+    entries_n = [b.permutation(k, next=True) for k in range(nb)]
+    phi, phi_n = b.permutation(nb), b.permutation(nb, next=True)
+    s_loc, s_nxt = entries[0], entries_n[0]
+    for k in range(1, nb):
+        s_loc, s_nxt = s_loc + entries[k], s_nxt + entries_n[k]
+    b.when_first_row().assert_zero(phi - s_loc)
+    b.when_transition().assert_zero(phi_n - phi - s_nxt)
+    b.when_last_row().assert_zero(phi - b.regional_sum())
+    return b, nb + 1
+
+
+def quotient_eval_lookups(ctx, log_n=20, width=256, interactions=32, batch=2, log_blowup=1, reps=3):
+    """The permutation (LogUp) side of compute_quotient_values on the device
+    for a lookup-heavy chip: `interactions` interactions in batches of
+    `batch` over the 2^log_n x width main trace and a permutation trace of
+    interactions / batch + 1 ext4 columns, both committed at blowup
+    2^log_blowup (random values: the timing does not depend on validity);
+    log quotient degree 1."""
+    from . import air
+    f = core.KOALABEAR
+    b, perm_cols = lookup_air(width, interactions, batch)
+    prog = b.compile()
+    n = 1 << log_n
+    pcs = core.TwoAdicFriPcs(ctx, f, log_blowup)
+    dom = pcs.natural_domain_for_degree(n)
+    trace = ctx.random_matrix(f, (2 << 32) | 0, n, width)
+    perm = ctx.random_matrix(f, (2 << 32) | 1, n, 4 * perm_cols)
+    _, md = pcs.commit([(dom, trace)])
+    _, qd = pcs.commit([(dom, perm)])
+    trace.free()
+    perm.free()
+    alpha = [31, 41, 59, 26]
+    chal = [[2, 7, 1, 8], [3, 1, 4, 1]]
+    kw = dict(perm_data=qd, challenges=chal, regional_sum=[5, 9, 2, 6])
+    air.quotient_values(ctx, prog, md, 0, log_blowup, 1, alpha, **kw).free()  # warm
+    ts = []
+    for _ in range(reps):
+        core.device_synchronize()
+        t0 = time.perf_counter()
+        q = air.quotient_values(ctx, prog, md, 0, log_blowup, 1, alpha, **kw)
+        ts.append(time.perf_counter() - t0)
+        q.free()
+    ctx.profile(True)
+    ctx.profile_reset()
+    air.quotient_values(ctx, prog, md, 0, log_blowup, 1, alpha, **kw).free()
+    ctx.profile(False)
+    kms = ctx.profile_get("quotient")[0]
+    md.free()
+    qd.free()
+    rows_q = 1 << (log_n + 1)
+    return {"trace": [n, width], "interactions": interactions, "batch": batch, "perm_ext_columns": perm_cols,
+            "quotient_rows": rows_q, "constraints": len(b.asserts), "program_instructions": prog.ninstr,
+            "program_registers": prog.nregs, "ms": min(ts) * 1e3, "kernel_ms": kms, "rows_per_s": rows_q / min(ts)}
 
 
 # ---------------------------------------------------------------- batched --
