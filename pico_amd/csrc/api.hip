@@ -2704,7 +2704,9 @@ extern "C" int pg_service_submit_host(pg_service* sv, int field, const uint32_t*
     j->rows = rows;
     j->cols = cols;
     j->lb = (int)log_blowup;
-    j->footprint = job_footprint(rows, cols, j->lb, true);
+    // plus the sponge states (64 bytes per LDE row) of a commit that absorbs
+    // the trace in column blocks
+    j->footprint = job_footprint(rows, cols, j->lb, true) + (rows << j->lb) * 64;
     return svc_enqueue(sv, std::move(j), id_out);
 }
 
