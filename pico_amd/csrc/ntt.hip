@@ -118,10 +118,24 @@ __device__ __forceinline__ uint32_t rev_dev(uint32_t x, int bits) {
     return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
 }
 
-// w_h^{+-e} from the table of w_h^e, e < h (mask = h - 1)
+// w_h^{+-e} from the table of w_h^e (h entries), given e4 = 4 e and
+// mask4 = 4 (h - 1).  Exponents are carried as byte offsets formed in 32 bits
+// (tables hold at most 2^27 words): the load takes the table address from
+// scalar registers and only the offset from a VGPR.
 template <bool INV>
-__device__ __forceinline__ uint32_t tw_at(gcptr __restrict__ tw, uint32_t e, uint32_t mask) {
-    return tw[INV ? ((0u - e) & mask) : e];
+__device__ __forceinline__ uint32_t tw_at(gcptr __restrict__ tw, uint32_t e4, uint32_t mask4) {
+    const uint32_t o = INV ? ((0u - e4) & mask4) : e4;
+    return *(gcptr)((GLOBAL const char*)tw + o);
+}
+
+// o[t] = q rev(t) (mod 2^32) for t < 2^LOG_N, with rev the LOG_N-bit
+// reversal: rev is additive over disjoint bits, so each entry is an earlier
+// entry plus one shifted copy of q instead of a product.
+template <int LOG_N>
+__device__ __forceinline__ void rev_multiples(uint32_t q, uint32_t (&o)[1 << LOG_N]) {
+    o[0] = 0;
+#pragma unroll
+    for (int t = 1; t < (1 << LOG_N); t++) o[t] = o[t & (t - 1)] + (q << (LOG_N - 1 - __builtin_ctz(t)));
 }
 
 // Scheduling fence: the compiler may not move any instruction across it.
@@ -131,17 +145,17 @@ __device__ __forceinline__ uint32_t tw_at(gcptr __restrict__ tw, uint32_t e, uin
 // earlier access (s_waitcnt vmcnt(0)): one HBM round trip per element.
 __device__ __forceinline__ void issue_fence() { __builtin_amdgcn_sched_barrier(0); }
 
-// x[i] *= tw_at(tw, e(i)) for i in [I0, N), the twiddle loads of each chunk
+// x[i] *= tw_at(tw, e4(i)) for i in [I0, N), the twiddle loads of each chunk
 // of 16 issued together ahead of the products.
 template <class F, bool INV, int N, int I0, class Exp>
-__device__ __forceinline__ void twiddle_mul(uint32_t* x, gcptr __restrict__ tw, uint32_t mask, Exp e) {
+__device__ __forceinline__ void twiddle_mul(uint32_t* x, gcptr __restrict__ tw, uint32_t mask4, Exp e4) {
     constexpr int CH = 16;
 #pragma unroll
     for (int b = I0; b < N; b += CH) {
         uint32_t v[CH];
 #pragma unroll
         for (int i = 0; i < CH; i++)
-            if (b + i < N) v[i] = tw_at<INV>(tw, e(b + i), mask);
+            if (b + i < N) v[i] = tw_at<INV>(tw, e4(b + i), mask4);
         issue_fence();
 #pragma unroll
         for (int i = 0; i < CH; i++)
@@ -149,38 +163,60 @@ __device__ __forceinline__ void twiddle_mul(uint32_t* x, gcptr __restrict__ tw, 
     }
 }
 
-// LDS word of (tile position, column).  A wave's lanes are C columns x
-// (64 / C) values of p; positions strided by E p would all map to the same
-// banks (E C is a multiple of 64), so C extra words follow every E positions:
-// lanes with different p then land C banks apart.
-template <int LOG_E, int C>
-__device__ __forceinline__ int lds_at(int pos, int c) {
-    return pos * C + c + (pos >> LOG_E) * C;
+// LDS layout of one exchange.  A wave's lanes are C columns x (64 / C)
+// values of p, and the 32 banks serve a half-wave (32 lanes) per pass, so
+// the words of a half-wave's lanes should differ mod 32.  Position pos of
+// column c is word (pos + (pos >> SH)) C + c: C extra words after every 2^SH
+// positions move lanes whose positions differ by multiples of 2^SH onto
+// other banks.  Every position an exchange touches is the OR of a thread part
+// and a compile-time part on disjoint bits, and on disjoint bits the layout
+// is additive, so an access is base(thread part) + off(constant part): the
+// constant lands in the LDS instruction's offset field and the access costs
+// no address arithmetic.
+template <int SH, int C>
+struct Lay {
+    static __device__ __forceinline__ int base(int t, int c) { return (t + (t >> SH)) * C + c; }
+    static constexpr int off(int k) { return (k + (k >> SH)) * C; }
+    static constexpr int words(int s) { return (s + ((s - 1) >> SH)) * C; }
+};
+// Pad of the three-level DIF's second exchange (its round-3 reads step by R
+// positions across lanes): 2^LOG_E as elsewhere, except on the 16 x 64 tile,
+// where a half-wave's reads then share banks (2.5 and 1.5 passes per access
+// at 8 and 16 columns) and a pad every 4 or 8 positions spreads them.
+constexpr int dif3_pad2(int log_e, int log_p, int c) {
+    return log_e == 4 && log_p == 6 ? (c == 8 ? 2 : 3) : log_e;
 }
-template <int LOG_E, int LOG_P, int C>
+// exchange words per tile group (DIF: the kernel runs a tile DIF)
+template <int LOG_E, int LOG_P, int C, bool DIF>
 constexpr int lds_words() {
-    return ((1 << (LOG_E + LOG_P)) + (1 << LOG_P)) * C;
+    constexpr int S = 1 << (LOG_E + LOG_P);
+    constexpr int w1 = Lay<LOG_E, C>::words(S);
+    constexpr int w2 = LOG_P > LOG_E && DIF ? Lay<dif3_pad2(LOG_E, LOG_P, C), C>::words(S) : 0;
+    return w1 > w2 ? w1 : w2;
 }
 
 // Tile DIF.  In: x[m] = tile position p + P m.  Out: x[t] = tile position
 // p E + t, holding DFT output index rev_S(p E + t).
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                         gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+                                         gcptr __restrict__ tw, int tw_sh, uint32_t mask4) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    using L = Lay<LOG_E, C>;
     static_assert(LOG_P <= LOG_E, "two-level tile needs P <= E");
     reg_dif<F, LOG_E, INV>(x);
     if constexpr (LOG_P > 0) {
-        twiddle_mul<F, INV, E, 1>(x, tw, mask, [&](int m) {
-            return ((uint32_t)p * (uint32_t)crev(m, LOG_E)) << tw_sh;
-        });
+        uint32_t e4[E];  // 4 p rev_E(m) 2^tw_sh
+        rev_multiples<LOG_E>((uint32_t)p << (tw_sh + 2), e4);
+        twiddle_mul<F, INV, E, 1>(x, tw, mask4, [&](int m) { return e4[m]; });
+        const int wb = L::base(p, c);
 #pragma unroll
-        for (int m = 0; m < E; m++) lds[lds_at<LOG_E, C>(p + P * m, c)] = x[m];
+        for (int m = 0; m < E; m++) lds[wb + L::off(P * m)] = x[m];
         __syncthreads();
+        const int rb = L::base(p * E, c);  // position p E + P u + i
 #pragma unroll
         for (int u = 0; u < E / P; u++)
 #pragma unroll
-            for (int i = 0; i < P; i++) x[u * P + i] = lds[lds_at<LOG_E, C>(P * (p * (E / P) + u) + i, c)];
+            for (int i = 0; i < P; i++) x[u * P + i] = lds[rb + L::off(P * u + i)];
 #pragma unroll
         for (int u = 0; u < E / P; u++) reg_dif<F, LOG_P, INV>(&x[u * P]);
         __syncthreads();
@@ -191,20 +227,24 @@ __device__ __forceinline__ void tile_dif(uint32_t (&x)[1 << LOG_E], uint32_t* ld
 // Out: x[u P + a] = natural output index (p E/P + u) + E a.
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                         gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+                                         gcptr __restrict__ tw, int tw_sh, uint32_t mask4) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    using L = Lay<LOG_E, C>;
     static_assert(LOG_P <= LOG_E, "two-level tile needs P <= E");
     reg_dit<F, LOG_E, INV>(x);
     if constexpr (LOG_P > 0) {
         const uint32_t rp = rev_dev((uint32_t)p, LOG_P);
-        twiddle_mul<F, INV, E, 1>(x, tw, mask, [&](int j) { return ((uint32_t)j * rp) << tw_sh; });
+        const uint32_t s4 = rp << (tw_sh + 2);  // exponent 4 j rp 2^tw_sh
+        twiddle_mul<F, INV, E, 1>(x, tw, mask4, [&](int j) { return (uint32_t)j * s4; });
+        const int wb = L::base(E * p, c);
 #pragma unroll
-        for (int j = 0; j < E; j++) lds[lds_at<LOG_E, C>(E * p + j, c)] = x[j];
+        for (int j = 0; j < E; j++) lds[wb + L::off(j)] = x[j];
         __syncthreads();
+        const int rb = L::base(p * (E / P), c);  // position p E/P + u + E a
 #pragma unroll
         for (int u = 0; u < E / P; u++)
 #pragma unroll
-            for (int a = 0; a < P; a++) x[u * P + a] = lds[lds_at<LOG_E, C>(p * (E / P) + u + E * a, c)];
+            for (int a = 0; a < P; a++) x[u * P + a] = lds[rb + L::off(u + E * a)];
 #pragma unroll
         for (int u = 0; u < E / P; u++) reg_dit<F, LOG_P, INV>(&x[u * P]);
         __syncthreads();
@@ -222,46 +262,53 @@ __device__ __forceinline__ void tile_dit(uint32_t (&x)[1 << LOG_E], uint32_t* ld
 // exchange between rounds.  Same contracts as the two-level tiles.
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dif3(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                          gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+                                          gcptr __restrict__ tw, int tw_sh, uint32_t mask4) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
     constexpr int LOG_G = LOG_E - LOG_R, G = 1 << LOG_G;
+    using L1 = Lay<LOG_E, C>;
+    using L2 = Lay<dif3_pad2(LOG_E, LOG_P, C), C>;
     // round 1: thread p owns positions p + P m (n1 = m, n2 R + n3 = p); the
     // round's twiddles are fetched before its register DFT
-    uint32_t tv[E];
+    uint32_t tv[E], e4[E];
+    rev_multiples<LOG_E>((uint32_t)p << (tw_sh + 2), e4);  // 4 p rev_E(t) 2^tw_sh
 #pragma unroll
-    for (int t = 1; t < E; t++) tv[t] = tw_at<INV>(tw, ((uint32_t)p * (uint32_t)crev(t, LOG_E)) << tw_sh, mask);
+    for (int t = 1; t < E; t++) tv[t] = tw_at<INV>(tw, e4[t], mask4);
     issue_fence();
     reg_dif<F, LOG_E, INV>(x);  // x[t]: k1 = rev_E(t)
 #pragma unroll
     for (int t = 1; t < E; t++) x[t] = fmul<F>(x[t], tv[t]);
+    const int wb1 = L1::base(p, c);  // position rev_E(t) P + p
 #pragma unroll
-    for (int t = 0; t < E; t++) lds[lds_at<LOG_E, C>(crev(t, LOG_E) * P + p, c)] = x[t];
+    for (int t = 0; t < E; t++) lds[wb1 + L1::off(crev(t, LOG_E) * P)] = x[t];
     // round 2: thread p -> (k1, n3) = (p / R, p % R), values over n2; its
     // twiddles are in flight across the barrier and the exchange
     const int k1 = p >> LOG_R, n3 = p & (R - 1);
+    rev_multiples<LOG_E>((uint32_t)n3 << (tw_sh + LOG_E + 2), e4);  // 4 n3 rev_E(t) 2^(tw_sh + LOG_E)
 #pragma unroll
-    for (int t = 1; t < E; t++)
-        tv[t] = tw_at<INV>(tw, ((uint32_t)n3 * (uint32_t)crev(t, LOG_E)) << (tw_sh + LOG_E), mask);
+    for (int t = 1; t < E; t++) tv[t] = tw_at<INV>(tw, e4[t], mask4);
     issue_fence();
     __syncthreads();
+    const int rb2 = L1::base(k1 * P + n3, c);  // position k1 P + m R + n3
 #pragma unroll
-    for (int m = 0; m < E; m++) x[m] = lds[lds_at<LOG_E, C>(k1 * P + m * R + n3, c)];
+    for (int m = 0; m < E; m++) x[m] = lds[rb2 + L1::off(m * R)];
     reg_dif<F, LOG_E, INV>(x);  // x[t]: k2 = rev_E(t)
 #pragma unroll
     for (int t = 1; t < E; t++) x[t] = fmul<F>(x[t], tv[t]);
     __syncthreads();
+    const int wb2 = L2::base(k1 * P + n3, c);  // position (k1 E + rev_E(t)) R + n3
 #pragma unroll
-    for (int t = 0; t < E; t++) lds[lds_at<LOG_E, C>((k1 * E + crev(t, LOG_E)) * R + n3, c)] = x[t];
+    for (int t = 0; t < E; t++) lds[wb2 + L2::off(crev(t, LOG_E) * R)] = x[t];
     __syncthreads();
     // round 3: thread p -> G pairs (k1, k2) = (rev_E(p / R), rev_E((p % R) G + g)),
-    // values over n3; output position p E + g R + t holds k3 = rev_R(t)
-    const uint32_t r1 = rev_dev((uint32_t)(p >> LOG_R), LOG_E);
+    // values over n3; output position p E + g R + t holds k3 = rev_R(t).  The
+    // position (r1 E + rev_E(n3 G + g)) R + i splits into the thread part
+    // (r1 E + rev_E(n3 G)) R and the constant part rev_E(g) R + i.
+    const uint32_t r1 = rev_dev((uint32_t)k1, LOG_E);
+    const int rb3 = L2::base((int)((r1 * E + rev_dev((uint32_t)n3 << LOG_G, LOG_E)) * R), c);
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-        const uint32_t r2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
+    for (int g = 0; g < G; g++)
 #pragma unroll
-        for (int i = 0; i < R; i++) x[g * R + i] = lds[lds_at<LOG_E, C>((int)((r1 * E + r2) * R) + i, c)];
-    }
+        for (int i = 0; i < R; i++) x[g * R + i] = lds[rb3 + L2::off(crev(g, LOG_E) * R + i)];
 #pragma unroll
     for (int g = 0; g < G; g++) reg_dif<F, LOG_R, INV>(&x[g * R]);
     __syncthreads();
@@ -271,9 +318,10 @@ __device__ __forceinline__ void tile_dif3(uint32_t (&x)[1 << LOG_E], uint32_t* l
 // order); out, x[r] = natural output index p + P r.
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                          gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
-    constexpr int E = 1 << LOG_E, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
+                                          gcptr __restrict__ tw, int tw_sh, uint32_t mask4) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P, LOG_R = LOG_P - LOG_E, R = 1 << LOG_R;
     constexpr int LOG_G = LOG_E - LOG_R, G = 1 << LOG_G;
+    using L = Lay<LOG_E, C>;
     // round A: groups g of R values over n3 (bit-reversed), n1 = rev_E(p / R),
     // n2 = rev_E((p % R) G + g)
     const uint32_t n1 = rev_dev((uint32_t)(p >> LOG_R), LOG_E);
@@ -282,39 +330,45 @@ __device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* l
     for (int g = 0; g < G; g++) {
         const uint32_t n2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
 #pragma unroll
-        for (int i = 1; i < R; i++) tv[g * R + i] = tw_at<INV>(tw, (n2 * (uint32_t)i) << (tw_sh + LOG_E), mask);
+        for (int i = 1; i < R; i++) tv[g * R + i] = tw_at<INV>(tw, (uint32_t)i * (n2 << (tw_sh + LOG_E + 2)), mask4);
     }
     issue_fence();
 #pragma unroll
     for (int g = 0; g < G; g++) reg_dit<F, LOG_R, INV>(&x[g * R]);  // x[g R + i]: k3 = i
+    // position (n1 R + i) E + n2: thread part n1 P + rev_E((p % R) G), constant
+    // part i E + rev_E(g)
+    const int wa = L::base((int)(n1 * P + rev_dev((uint32_t)(p & (R - 1)) << LOG_G, LOG_E)), c);
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const uint32_t n2 = rev_dev((uint32_t)(((p & (R - 1)) << LOG_G) + g), LOG_E);
 #pragma unroll
         for (int i = 0; i < R; i++) {
             if (i > 0) x[g * R + i] = fmul<F>(x[g * R + i], tv[g * R + i]);
-            lds[lds_at<LOG_E, C>((int)((n1 * R + i) * E + n2), c)] = x[g * R + i];
+            lds[wa + L::off(i * E + crev(g, LOG_E))] = x[g * R + i];
         }
     }
     // round B's twiddles are in flight across the barrier and the exchange
     const int m1 = p >> LOG_R, k3 = p & (R - 1);
+    // exponent m1 (k3 + R i) 2^tw_sh: a progression in i
+    const uint32_t b4 = ((uint32_t)m1 * (uint32_t)k3) << (tw_sh + 2), st4 = (uint32_t)m1 << (LOG_R + tw_sh + 2);
 #pragma unroll
-    for (int i = 0; i < E; i++) tv[i] = tw_at<INV>(tw, ((uint32_t)m1 * (uint32_t)(k3 + R * i)) << tw_sh, mask);
+    for (int i = 0; i < E; i++) tv[i] = tw_at<INV>(tw, b4 + (uint32_t)i * st4, mask4);
     issue_fence();
     __syncthreads();
     // round B: thread p -> (n1, k3) = (p / R, p % R), values over n2 (bit-reversed)
+    const int rb = L::base(p * E, c);  // position (m1 R + k3) E + rev_E(i) = p E + rev_E(i)
 #pragma unroll
-    for (int i = 0; i < E; i++) x[i] = lds[lds_at<LOG_E, C>((m1 * R + k3) * E + crev(i, LOG_E), c)];
+    for (int i = 0; i < E; i++) x[i] = lds[rb + L::off(crev(i, LOG_E))];
     reg_dit<F, LOG_E, INV>(x);  // x[i]: k2 = i
 #pragma unroll
     for (int i = 0; i < E; i++) x[i] = fmul<F>(x[i], tv[i]);
     __syncthreads();
+    const int wb = L::base(k3 * E + m1, c);  // position (k3 + R i) E + m1
 #pragma unroll
-    for (int i = 0; i < E; i++) lds[lds_at<LOG_E, C>((k3 + R * i) * E + m1, c)] = x[i];
+    for (int i = 0; i < E; i++) lds[wb + L::off(P * i)] = x[i];
     __syncthreads();
     // round C: thread p -> kr = p, values over n1 (bit-reversed)
 #pragma unroll
-    for (int i = 0; i < E; i++) x[i] = lds[lds_at<LOG_E, C>(p * E + crev(i, LOG_E), c)];
+    for (int i = 0; i < E; i++) x[i] = lds[rb + L::off(crev(i, LOG_E))];
     reg_dit<F, LOG_E, INV>(x);  // x[i] = X[p + P i]
     __syncthreads();
 }
@@ -322,28 +376,36 @@ __device__ __forceinline__ void tile_dit3(uint32_t (&x)[1 << LOG_E], uint32_t* l
 // tile transform of either kind, by shape
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dif_any(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                             gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+                                             gcptr __restrict__ tw, int tw_sh, uint32_t mask4) {
     if constexpr (LOG_P > LOG_E)
-        tile_dif3<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+        tile_dif3<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask4);
     else
-        tile_dif<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+        tile_dif<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask4);
 }
 template <class F, int LOG_E, int LOG_P, int C, bool INV>
 __device__ __forceinline__ void tile_dit_any(uint32_t (&x)[1 << LOG_E], uint32_t* lds, int p, int c,
-                                             gcptr __restrict__ tw, int tw_sh, uint32_t mask) {
+                                             gcptr __restrict__ tw, int tw_sh, uint32_t mask4) {
     if constexpr (LOG_P > LOG_E)
-        tile_dit3<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+        tile_dit3<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask4);
     else
-        tile_dit<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask);
+        tile_dit<F, LOG_E, LOG_P, C, INV>(x, lds, p, c, tw, tw_sh, mask4);
 }
 
+// Natural output index of x[r] after a tile DIT, as the OR of a thread part
+// and a compile-time part on disjoint bits: p + P r (three-level tiles) or
+// p E/P + (r / P + E (r % P)) (two-level tiles).
 template <int LOG_E, int LOG_P>
-__device__ __forceinline__ uint32_t dit_out_index(int p, int r) {
+__device__ __forceinline__ uint32_t dit_out_thread(int p) {
     constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
     if constexpr (LOG_P > LOG_E)
-        return (uint32_t)(p + P * r);  // three-level tile
+        return (uint32_t)p;
     else
-        return (uint32_t)(p * (E / P) + r / P + E * (r % P));
+        return (uint32_t)(p * (E / P));
+}
+template <int LOG_E, int LOG_P>
+constexpr uint32_t dit_out_const(int r) {
+    constexpr int E = 1 << LOG_E, P = 1 << LOG_P;
+    return LOG_P > LOG_E ? (uint32_t)(P * r) : (uint32_t)(r / P + E * (r % P));
 }
 
 // Tile shape per transform size 2^log_s and pass: E = 2^log_e values per
@@ -394,10 +456,16 @@ constexpr int groups_for(int log_e, int log_p, int kind) {
 // kinds that keep the inverse-DFT output while looping over cosets
 constexpr bool parks_coefficients(int kind) { return kind == KIND_BC || kind == KIND_SINGLE; }
 
+// Committed (bit-reversed) or natural output row of natural index j of coset
+// block cs.  For j = t | k on disjoint bits the row is out_row(cs, t) +
+// out_row_add(k): a lane computes its base row once and each element adds a
+// wave-uniform offset.
 __device__ __forceinline__ uint64_t out_row(const LdeArgs& a, uint32_t cs, uint32_t natural) {
-    // committed (bit-reversed) or natural row of coset block cs, natural index j
     return a.bitrev ? (uint64_t)cs * (1ull << a.L) + rev_dev(natural, a.L)
                     : (uint64_t)rev_dev(cs, a.b) + ((uint64_t)natural << a.b);
+}
+__device__ __forceinline__ uint64_t out_row_add(const LdeArgs& a, uint32_t k) {
+    return a.bitrev ? (uint64_t)rev_dev(k, a.L) : (uint64_t)k << a.b;
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
@@ -416,7 +484,9 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // D) fit two 1024-thread workgroups per CU at <= 64 VGPRs without spilling;
 // the coset-looping passes do not, and are left unconstrained.
 constexpr int waves_for(int log_e, int log_p, int kind) {
-    return log_p > log_e && threads_for(log_e, log_p, kind) == 1024 && (kind == KIND_A || kind == KIND_D) ? 8 : 1;
+    return log_p > log_e && threads_for(log_e, log_p, kind) == 1024 && (kind == KIND_A || kind == KIND_D) ? 8
+           : log_e == 4 && log_p == 6 && kind == KIND_BC ? 6
+                                                          : 1;
 }
 
 template <class F, int LOG_E, int LOG_P, int KIND>
@@ -427,7 +497,7 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     constexpr int C = cols_for(LOG_E, LOG_P, KIND), G = groups_for(LOG_E, LOG_P, KIND);
     constexpr int NT = threads_for(LOG_E, LOG_P, KIND);
     constexpr bool PARK = parks_coefficients(KIND) && E >= 32;
-    constexpr int LW = lds_words<LOG_E, LOG_P, C>();  // exchange words per tile group
+    constexpr int LW = lds_words<LOG_E, LOG_P, C, KIND != KIND_D>();
     __shared__ uint32_t lds_all[LOG_P > 0 ? G * LW : 1];
     __shared__ uint32_t lds_coef[PARK ? E * NT : 1];
     const int tid = threadIdx.x;
@@ -451,7 +521,7 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         tile -= prefix[lo];
     }
     const bool valid = tile < a.ntiles;
-    const uint32_t h = 1u << a.L, mask = h - 1;
+    const uint32_t h = 1u << a.L, mask4 = (h - 1) << 2;
     const uint32_t w = a.w;
     const uint32_t ncb = (w + C - 1) / C;
     const uint32_t col = (uint32_t)(tile % ncb) * C + c;  // column block fastest
@@ -466,6 +536,10 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
     const gptr g_out = (gptr)a.out, g_tmp = (gptr)a.tmp;
     uint32_t x[E];
 
+    // Addresses: each lane forms one 64-bit base per access group and every
+    // element adds a wave-uniform offset (a compile-time index times a row
+    // stride, computed on the scalar unit), instead of a 64-bit product per
+    // element.
     if constexpr (KIND == KIND_SINGLE || KIND == KIND_BC) {
         // SINGLE: the whole column is one tile.  BC: contiguous block `blk`
         // of n_lo rows of the pass-A output.
@@ -473,11 +547,11 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t blk = KIND == KIND_BC ? (uint32_t)rest : 0u;
         const gcptr src = KIND == KIND_BC ? (gcptr)g_out : g_in;
         const uint64_t ld = KIND == KIND_BC ? a.ld_out : a.ld_in;
+        const gcptr s0 = src + ((uint64_t)(n_lo * blk + (uint32_t)p) * ld + colr);  // row n_lo blk + p + P m
 #pragma unroll
-        for (int m = 0; m < E; m++)
-            x[m] = src[(uint64_t)(n_lo * blk + (uint32_t)(p + P * m)) * ld + colr];
+        for (int m = 0; m < E; m++) x[m] = s0[(uint64_t)(P * m) * ld];
         issue_fence();
-        tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k1, mask);
+        tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k1, mask4);
         // The coefficients are reused for every coset: large tiles park them
         // in a thread-private LDS slice (VGPRs would spill), small ones keep a
         // register copy.
@@ -497,22 +571,38 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
         const uint32_t ncos = njobs > 0 ? (1u << a0.b) : ncos_job;
         for (uint32_t cs = 0; cs < ncos; cs++) {
             const bool live = cs < ncos_job;
-            const gcptr sc = g_scale + (uint64_t)(live ? cs : 0u) * h;
-            // Opaque copy of the twiddle pointer: keeps the (coset-invariant)
-            // twiddle loads inside the loop instead of hoisting ~64 of them
-            // into VGPRs for all cosets, which spills.
+            // Opaque copies of the twiddle pointer and of the lane's tile
+            // position: keep the (coset-invariant) twiddle and address
+            // arithmetic inside the loop instead of hoisting it into VGPRs
+            // for all cosets, which spills.
+            // (a scalar copy when the wave's job is uniform: the loads then
+            // take the table address from SGPRs)
             gcptr twl = g_tw;
-            asm volatile("" : "+v"(twl));
-            // Likewise re-derive the lane coordinates per coset so the 64-bit
-            // gather/store addresses are not all precomputed outside the loop.
+            if constexpr (C * P >= 64)
+                asm volatile("" : "+s"(twl));
+            else
+                asm volatile("" : "+v"(twl));
             int pl = p;
-            uint32_t coll = col;
-            if constexpr (LOG_P > 0) asm volatile("" : "+v"(pl), "+v"(coll));
+            if constexpr (LOG_P > 0) asm volatile("" : "+v"(pl));
+            // scale factors of this lane's E coefficients: contiguous, read
+            // as 16-byte vectors when E >= 4 (device allocations are aligned)
+            const gcptr sc = g_scale + (uint64_t)(live ? cs : 0u) * h + n_lo * blk + (uint32_t)pl * E;
 #pragma unroll
             for (int b0 = 0; b0 < E; b0 += 16) {
                 constexpr int CH = E < 16 ? E : 16;
+                if constexpr (E >= 4) {
 #pragma unroll
-                for (int t = 0; t < CH; t++) x[b0 + t] = sc[n_lo * blk + pl * E + b0 + t];
+                    for (int t = 0; t < CH; t += 4) {
+                        const u32x4 v = *(GLOBAL const u32x4*)(sc + b0 + t);
+                        x[b0 + t] = v.x;
+                        x[b0 + t + 1] = v.y;
+                        x[b0 + t + 2] = v.z;
+                        x[b0 + t + 3] = v.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < CH; t++) x[b0 + t] = sc[b0 + t];
+                }
                 issue_fence();
 #pragma unroll
                 for (int t = 0; t < CH; t++) {
@@ -520,54 +610,56 @@ __global__ void __launch_bounds__(threads_for(LOG_E, LOG_P, KIND))
                     x[b0 + t] = fmul<F>(v, x[b0 + t]);
                 }
             }
-            tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask);
+            tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, pl, c, twl, a.k1, mask4);
             if (act && live) {
+                const uint32_t jt = dit_out_thread<LOG_E, LOG_P>(pl);
                 if constexpr (KIND == KIND_BC) {
-                    twiddle_mul<F, false, E, 0>(x, twl, mask, [&](int r) {
-                        return dit_out_index<LOG_E, LOG_P>(pl, r) * rblk;
-                    });
+                    // times w_h^{j rblk}, j = jt | dit_out_const(r) < n_lo and
+                    // rblk < h / n_lo, so the exponent stays below h
+                    const uint32_t e0 = (jt * rblk) << 2, rblk4 = rblk << 2;
+                    twiddle_mul<F, false, E, 0>(x, twl, mask4,
+                                                [&](int r) { return e0 + dit_out_const<LOG_E, LOG_P>(r) * rblk4; });
+                    const gptr d0 = g_tmp + (((uint64_t)cs * h + n_lo * blk + jt) * w + col);
 #pragma unroll
-                    for (int r = 0; r < E; r++) {
-                        const uint32_t j = dit_out_index<LOG_E, LOG_P>(pl, r);
-                        g_tmp[((uint64_t)cs * h + n_lo * blk + j) * w + coll] = x[r];
-                    }
+                    for (int r = 0; r < E; r++) d0[(uint64_t)dit_out_const<LOG_E, LOG_P>(r) * w] = x[r];
                 } else {
+                    const gptr d0 = g_out + (out_row(a, cs, jt) * a.ld_out + col);
 #pragma unroll
                     for (int r = 0; r < E; r++)
-                        g_out[out_row(a, cs, dit_out_index<LOG_E, LOG_P>(pl, r)) * a.ld_out + coll] = x[r];
+                        d0[out_row_add(a, dit_out_const<LOG_E, LOG_P>(r)) * a.ld_out] = x[r];
                 }
             }
         }
     } else if constexpr (KIND == KIND_A) {
         const uint32_t n_lo = 1u << a.k2;
         const uint32_t j_lo = (uint32_t)rest;
+        const gcptr s0 = g_in + ((uint64_t)(j_lo + n_lo * (uint32_t)p) * a.ld_in + colr);  // row j_lo + n_lo (p + P m)
 #pragma unroll
-        for (int m = 0; m < E; m++)
-            x[m] = g_in[(uint64_t)(j_lo + n_lo * (uint32_t)(p + P * m)) * a.ld_in + colr];
+        for (int m = 0; m < E; m++) x[m] = s0[(uint64_t)(n_lo * (uint32_t)(P * m)) * a.ld_in];
         issue_fence();
-        tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k2, mask);
+        tile_dif_any<F, LOG_E, LOG_P, C, true>(x, lds, p, c, g_tw, a.k2, mask4);
         if (act) {
-            twiddle_mul<F, true, E, 0>(x, g_tw, mask,
-                                       [&](int t) { return j_lo * rev_dev((uint32_t)(p * E + t), a.k1); });
+            // times w_h^{-j_lo rev_k1(p E + t)}, rev_k1(p E + t) = rev_P(p) + rev_E(t) P
+            const uint32_t e0 = (j_lo * rev_dev((uint32_t)p, LOG_P)) << 2, j4 = j_lo << 2;
+            twiddle_mul<F, true, E, 0>(x, g_tw, mask4, [&](int t) { return e0 + (uint32_t)(crev(t, LOG_E) * P) * j4; });
+            const gptr d0 = g_out + ((uint64_t)(j_lo + n_lo * (uint32_t)(p * E)) * a.ld_out + col);  // row j_lo + n_lo (p E + t)
 #pragma unroll
-            for (int t = 0; t < E; t++)
-                g_out[(uint64_t)(j_lo + n_lo * (uint32_t)(p * E + t)) * a.ld_out + col] = x[t];
+            for (int t = 0; t < E; t++) d0[(uint64_t)(n_lo * (uint32_t)t) * a.ld_out] = x[t];
         }
     } else {  // KIND_D
         const uint32_t n_lo = 1u << a.k2;
         const uint32_t j_lo = (uint32_t)(rest % n_lo);
         const uint32_t cs = (uint32_t)(rest / n_lo);
+        const gcptr s0 = g_tmp + (((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E)) * w + colr);  // row cs h + j_lo + n_lo (p E + t)
 #pragma unroll
-        for (int t = 0; t < E; t++)
-            x[t] = g_tmp[((uint64_t)cs * h + j_lo + n_lo * (uint32_t)(p * E + t)) * w + colr];
+        for (int t = 0; t < E; t++) x[t] = s0[(uint64_t)(n_lo * (uint32_t)t) * w];
         issue_fence();
-        tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, p, c, g_tw, a.k2, mask);
+        tile_dit_any<F, LOG_E, LOG_P, C, false>(x, lds, p, c, g_tw, a.k2, mask4);
         if (act) {
+            const gptr d0 = g_out + (out_row(a, cs, j_lo + n_lo * dit_out_thread<LOG_E, LOG_P>(p)) * a.ld_out + col);
 #pragma unroll
-            for (int r = 0; r < E; r++) {
-                const uint32_t natural = j_lo + n_lo * dit_out_index<LOG_E, LOG_P>(p, r);
-                g_out[out_row(a, cs, natural) * a.ld_out + col] = x[r];
-            }
+            for (int r = 0; r < E; r++)
+                d0[out_row_add(a, n_lo * dit_out_const<LOG_E, LOG_P>(r)) * a.ld_out] = x[r];
         }
     }
 }
