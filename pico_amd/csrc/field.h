@@ -78,6 +78,20 @@ PG_HD uint32_t fmul_lazy(uint32_t a, uint32_t b) {
     uint32_t m = (uint32_t)t * F::MU;
     return (uint32_t)((t + (uint64_t)m * F::P) >> 32);
 }
+// t + k x for 32-bit x and a small constant k.  On the device this is one
+// v_mad_u64_u32: written as a 64-bit add of the zero-extended x, the compiler
+// spends a move clearing the high word of a register pair per addend.
+template <uint32_t K>
+PG_HD uint64_t mad_wide(uint32_t x, uint64_t t) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint64_t r, carry;  // the carry-out (an SGPR pair) is zero here: t + k x < 2^64
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(x), "i"(K), "v"(t));
+    (void)carry;
+    return r;
+#else
+    return t + (uint64_t)K * x;
+#endif
+}
 // [0, 2p) -> [0, p)
 template <class F>
 PG_HD uint32_t freduce(uint32_t u) {

@@ -295,9 +295,9 @@ PG_HD void internal_layer(uint32_t (&s)[16]) {
 // a 64-bit multiply-add).  The outputs stay lazy (< 2p).
 template <class F>
 PG_HD void internal_layer_redc(uint32_t (&s)[16]) {
-    uint64_t t = 0;
+    uint64_t t = s[0];
 #pragma unroll
-    for (int i = 0; i < 16; i++) t += s[i];  // < 31 p < 2^36
+    for (int i = 1; i < 16; i++) t = mad_wide<1>(s[i], t);  // < 31 p < 2^36
     uint32_t sm;
     if constexpr (F::P == KoalaBear::P) {
         // q = t >> 31 <= 30 and t - q p = (t - q 2^31) + q (2^31 - p) < 2^31 + 30 (2^24 - 1) < 2p;
