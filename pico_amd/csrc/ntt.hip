@@ -440,12 +440,18 @@ constexpr int shape_alt_log_e(int log_s, int kind) {
 // 8 for the coset-looping passes' three-level tiles (two 512-thread
 // workgroups per CU desynchronise their barriers better than one of 1024:
 // measured 8% on 2^10 BC) and for the two-level 2^11 tile when it also parks
-// coefficients in LDS (exchange + coefficient buffers <= 128 KiB).
+// coefficients in LDS (exchange + coefficient buffers <= 128 KiB).  Pass A's
+// 32 x 32 tile takes 32 columns (128-byte row runs of its strided reads and
+// writes): one 1024-thread workgroup with 135 KB of LDS per CU, the same 16
+// waves as two 512-thread ones of 16 columns, measured 6% faster on 2^20 x
+// 256; pass D's 32 x 32 tile ran 15% slower that way and keeps 16.
 constexpr int cols_for(int log_e, int log_p, int kind) {
     const bool coset_loop = kind == KIND_BC || kind == KIND_SINGLE;
     const int cmax = coset_loop ? 8 : 16;
     return log_p > log_e ? (1024 >> log_p < cmax ? 1024 >> log_p : cmax)
-           : (log_e + log_p >= 11 && coset_loop) ? 8 : 16;
+           : (log_e + log_p >= 11 && coset_loop) ? 8
+           : (kind == KIND_A && log_e == 5 && log_p == 5) ? 32
+                                                          : 16;
 }
 constexpr int threads_for(int log_e, int log_p, int kind) {
     return cols_for(log_e, log_p, kind) * (1 << log_p) >= 256 ? cols_for(log_e, log_p, kind) * (1 << log_p) : 256;
