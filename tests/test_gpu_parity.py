@@ -39,6 +39,8 @@ def test_upload_download_roundtrip(ctx, oracle, f):
 LDE_CASES = [(L, w, b) for L in range(0, 12) for (w, b) in [(1, 1), (5, 2), (16, 1), (33, 3)]]
 LDE_CASES += [(L, w, b) for L in range(12, 17) for (w, b) in [(3, 1), (17, 2), (40, 1)]]
 LDE_CASES += [(18, 2, 3), (20, 3, 1), (21, 5, 2), (22, 1, 1), (22, 2, 2)]
+# 2^20 rows at a width above 8: pass A's 32-column tile with a partial last block
+LDE_CASES += [(20, 37, 1)]
 # log_blowup 4: the compress config (vm/src/configs/stark_config/kb_poseidon2.rs:111-140)
 LDE_CASES += [(0, 3, 4), (5, 9, 4), (9, 16, 4), (11, 8, 4), (12, 5, 4), (15, 24, 4), (18, 3, 4)]
 

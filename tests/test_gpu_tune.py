@@ -25,7 +25,7 @@ def _all_alternative(text):
 # (log_h, width, log_blowup): single-pass sizes 2^7..2^11 and four-step
 # sizes whose strided and block transforms are 2^7..2^11
 CASES = [(7, 9, 1), (8, 3, 2), (9, 40, 1), (10, 5, 3), (11, 17, 1), (14, 9, 1), (16, 33, 2), (18, 4, 1),
-         (20, 2, 1), (21, 3, 1), (22, 2, 1)]
+         (20, 2, 1), (20, 37, 1), (21, 3, 1), (22, 2, 1)]
 
 
 @pytest.mark.parametrize("f", [KOALABEAR, BABYBEAR])
