@@ -48,12 +48,27 @@ __device__ __forceinline__ uint32_t rev_bits(uint32_t x, uint32_t bits) {
 template <uint32_t N>
 using IC = std::integral_constant<uint32_t, N>;
 
+// Row block of this workgroup.  The next rows of block b are block
+// rev(rev(b) + 2^qdb) (in bit-reversed order the block index is the low bits
+// of the natural index), thousands of blocks away in launch order.  Blocks
+// are therefore numbered by L = rev(b), whose next block is L + 2^qdb, and
+// logical blocks are dealt to the 8 XCDs (round-robin by launch index) in
+// contiguous ranges: a block's next rows are then staged by a neighbour on
+// the same XCD at about the same time, so the second read of each line can
+// hit that XCD's L2, and neighbours' output elements share lines.
+__device__ __forceinline__ uint32_t quotient_block(uint32_t nblocks, uint32_t log_nblocks) {
+    const uint32_t b = blockIdx.x, x = b & 7u, y = b >> 3, q = nblocks >> 3, r = nblocks & 7u;
+    const uint32_t logical = x * q + (x < r ? x : r) + y;
+    return rev_bits(logical, log_nblocks);
+}
+
 template <class F>
 __global__ void __launch_bounds__(QT) quotient_kernel(QuotientArgs a) {
     extern __shared__ uint32_t lds[];
     const uint32_t t = threadIdx.x;
     const uint32_t B = a.q < (uint32_t)QB ? a.q : (uint32_t)QB;  // rows of the block (a power of two)
-    const uint32_t r0 = blockIdx.x * (uint32_t)QB;
+    const uint32_t log_nblocks = a.q > (uint32_t)QB ? (uint32_t)a.log_q - (uint32_t)__builtin_ctz((uint32_t)QB) : 0u;
+    const uint32_t r0 = quotient_block(gridDim.x, log_nblocks) * (uint32_t)QB;
     const uint32_t mask = a.q - 1u;
     // base of the block of next rows (uniform)
     const uint32_t r0n = rev_bits((rev_bits(r0, a.log_q) + a.next_step) & mask, a.log_q) & ~(B - 1u);
