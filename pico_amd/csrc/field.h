@@ -92,6 +92,19 @@ PG_HD uint64_t mad_wide(uint32_t x, uint64_t t) {
     return t + (uint64_t)K * x;
 #endif
 }
+// a b + t for 32-bit a, b: one v_mad_u64_u32 on the device, where the
+// compiler turns a product by a constant power of two into a 64-bit shift
+// and a 64-bit add (v_lshl_add_u64 shifts by at most 7).
+PG_HD uint64_t mad_u32(uint32_t a, uint32_t b, uint64_t t) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint64_t r, carry;  // the carry-out (an SGPR pair) is zero here: a b + t < 2^64 for t < 2^33
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "s"(b), "v"(t));
+    (void)carry;
+    return r;
+#else
+    return (uint64_t)a * b + t;
+#endif
+}
 // [0, 2p) -> [0, p)
 template <class F>
 PG_HD uint32_t freduce(uint32_t u) {

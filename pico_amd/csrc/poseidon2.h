@@ -314,7 +314,7 @@ PG_HD void internal_layer_redc(uint32_t (&s)[16]) {
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const uint64_t v = (uint64_t)P2<F>::K.diag[i] * s[i] + sm;
+        const uint64_t v = mad_u32(s[i], P2<F>::K.diag[i], sm);
         const uint32_t m = (uint32_t)v * F::MU;
         s[i] = (uint32_t)((v + (uint64_t)m * F::P) >> 32);
     }
