@@ -94,7 +94,8 @@ PG_HD uint64_t mad_wide(uint32_t x, uint64_t t) {
 }
 // a b + t for 32-bit a, b: one v_mad_u64_u32 on the device, where the
 // compiler turns a product by a constant power of two into a 64-bit shift
-// and a 64-bit add (v_lshl_add_u64 shifts by at most 7).
+// and a 64-bit add (v_lshl_add_u64 shifts by at most 7).  b must be the same
+// in every lane of a wave (a constant): it is passed in a scalar register.
 PG_HD uint64_t mad_u32(uint32_t a, uint32_t b, uint64_t t) {
 #ifdef __HIP_DEVICE_COMPILE__
     uint64_t r, carry;  // the carry-out (an SGPR pair) is zero here: a b + t < 2^64 for t < 2^33
