@@ -113,15 +113,29 @@ struct P2 {
     static constexpr P2Const<F> K = make_p2const<F>();
 };
 
+// Exclusive upper bound of a Montgomery product (a b + m p) / 2^32 of
+// factors below `a` and `b` before its final subtraction (compile time).
+constexpr uint64_t redc_bound(uint64_t a, uint64_t b, uint64_t p) {
+    return (uint64_t)(((unsigned __int128)a * b + ((unsigned __int128)p << 32) + 0xffffffffu) >> 32);
+}
+
 template <class F>
 PG_HD uint32_t sbox(uint32_t x) {
-    // x < p.  fmul_lazy leaves a product below 1.5p; a full product with one
-    // such factor and one factor below p is below 1.75p before its final
-    // subtraction, so it still reduces to [0, p).
+    // x < p.  fmul_lazy leaves a product unreduced; fmul's single final
+    // subtraction suffices while its product bound stays below 2p.
+    constexpr uint64_t p = F::P;
+    constexpr uint64_t b2 = redc_bound(p, p, p);  // x^2 unreduced
+    static_assert(redc_bound(b2, p, p) <= 2 * p, "x^2 x reduces with one subtraction");
     if constexpr (F::SBOX == 3) {
         return fmul<F>(fmul_lazy<F>(x, x), x);
     } else {
-        const uint32_t x2 = fmul<F>(x, x);
+        // x^4 from the unreduced x^2 stays a 32-bit word, and x^4 x^3
+        // reduces with one subtraction (BabyBear: x^2 < 1.47p, x^4 < 2.02p)
+        constexpr uint64_t b4 = redc_bound(b2, b2, p);
+        static_assert(b4 <= (1ull << 32) && redc_bound(b4, p, p) <= 2 * p, "x^7 bounds");
+        static_assert((unsigned __int128)b2 * b2 + ((unsigned __int128)p << 32) <= ((unsigned __int128)1 << 64),
+                      "x^2 x^2 + m p fits 64 bits");
+        const uint32_t x2 = fmul_lazy<F>(x, x);
         const uint32_t x3 = fmul<F>(x2, x);
         return fmul<F>(fmul_lazy<F>(x2, x2), x3);
     }
