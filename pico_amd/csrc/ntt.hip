@@ -491,12 +491,13 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // pass BC's 16 x 64 tile fits three 512-thread workgroups (6 waves, <= 80
 // VGPRs: it needs 77; measured 1.69 -> 1.56 ms on 2^20 x 256) and its
 // 16 x 128 tile two 1024-thread ones (8 waves, <= 64 VGPRs: 62 instead of
-// 70; 4.30 -> 3.55 ms on 2^22 x 128); the other coset-looping tiles would
-// spill and are left unconstrained.
+// 70; 4.30 -> 3.55 ms on 2^22 x 128), as does the single-pass 16 x 128 tile
+// (2^11 x 4096: -6..9%); the other coset-looping tiles would spill or gain
+// no workgroup (LDS) and are left unconstrained.
 constexpr int waves_for(int log_e, int log_p, int kind) {
     return log_p > log_e && threads_for(log_e, log_p, kind) == 1024 && (kind == KIND_A || kind == KIND_D) ? 8
            : log_e == 4 && log_p == 6 && kind == KIND_BC ? 6
-           : log_e == 4 && log_p == 7 && kind == KIND_BC ? 8
+           : log_e == 4 && log_p == 7 && (kind == KIND_BC || kind == KIND_SINGLE) ? 8
                                                           : 1;
 }
 
