@@ -102,7 +102,7 @@ struct LdeArgs {
 };
 enum LdeKind { KIND_SINGLE = 0, KIND_A = 1, KIND_BC = 2, KIND_D = 3 };
 // Split of a size-2^L transform: L <= 11 is one pass (k1 = 0, k2 = L);
-// otherwise k1 = ceil(L/2) strided bits, k2 = L - k1 block bits.
+// otherwise k1 = ceil(L/2) strided bits (10 at L = 21), k2 = L - k1 block bits.
 void lde_split(int L, int* k1, int* k2);
 // Columns per tile of the kernel that runs pass `kind` with tile size 2^log_s
 // in tile-shape variant `variant`.
